@@ -1,0 +1,227 @@
+/*
+ * sps_fd.h - C ABI of the MI355X explicit finite-difference solver step.
+ *
+ * This is the boundary between host code that drives a SymPhas-style model
+ * (the C++ adapter SolverFT_MI355X in symphas_amd/host, ctypes from Python, or
+ * any FFI) and the hand-written HIP kernels for gfx950.  Signatures carry only
+ * plain C types, pointers and sizes.
+ *
+ * Semantics follow the reference SolverFT step (SymPhas,
+ * examples/solvers/solverft.h:41-184):
+ *   one step = RHS evaluation into dframe  (solverft.h:90-103)
+ *            + forward Euler psi += dt*dframe over the interior (solverft.h:46-50)
+ *            + periodic ghost refresh (sol/inc/boundarygroup.h:373-381,
+ *              datatypes/inc/boundaryupdate.h:154-356 (3D), 650-708 (2D)).
+ * Ghosts are treated as fresh before the first RHS evaluation, which is what
+ * the IO driver does (modules-io.h:450-452).  On the device the field is stored
+ * without ghost cells and the periodic wrap is applied inside the kernels; the
+ * SymPhas padded layout (BOUNDARY_DEPTH = 3 ghost cells per side,
+ * datatypes/inc/gridinfo.h:46, x fastest, datatypes/inc/grid.h:537-578) exists
+ * only at upload/download.
+ *
+ * Errors: every entry point returns an sps_status; sps_fd_last_error() gives
+ * the text of the most recent failure on the calling thread.  (The reference
+ * prints and exit()s or throws, lib/inc/definitions.h:42-50,
+ * sol/inc/stencil.h:671-679, sol/inc/boundarygroup.h:315-316.)
+ *
+ * Threading: a plan or batch may be used by one host thread at a time.  Work is
+ * enqueued on the object's own HIP stream; calls that return host data
+ * (download, reduce) synchronise that stream.
+ */
+#ifndef SPS_FD_H
+#define SPS_FD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPS_FD_ABI_VERSION 1
+
+/* Ghost width of the SymPhas padded layout (datatypes/inc/gridinfo.h:46). */
+#define SPS_BOUNDARY_DEPTH 3
+/* Highest power of psi inside one term; pow<N> with N <= 6 is a product of N
+ * copies of psi in the reference (lib/inc/spslibmath.h:255-269). */
+#define SPS_MAX_DEGREE 6
+#define SPS_MAX_TERMS 8
+
+typedef enum {
+  SPS_OK = 0,
+  SPS_E_INVALID = -1,     /* bad argument or descriptor */
+  SPS_E_UNSUPPORTED = -2, /* valid request this build does not implement */
+  SPS_E_HIP = -3,         /* HIP runtime error */
+  SPS_E_NOMEM = -4,       /* device or host allocation failed */
+  SPS_E_STATE = -5        /* call made in the wrong state (e.g. no data) */
+} sps_status;
+
+typedef enum { SPS_F32 = 0, SPS_F64 = 1 } sps_dtype;
+
+/* Derivative applied to a term's operand (sol/inc/solver.h:877-915: order 2
+ * laplacian, 3 gradlaplacian, 4 bilaplacian; order 0 = the operand itself). */
+typedef enum {
+  SPS_OP_ID = 0,
+  SPS_OP_LAP = 1,
+  SPS_OP_BILAP = 2,
+  SPS_OP_GRADLAP = 3
+} sps_op;
+
+typedef enum { SPS_AXIS_X = 0, SPS_AXIS_Y = 1, SPS_AXIS_Z = 2 } sps_axis;
+
+typedef enum {
+  SPS_LAYOUT_INTERIOR = 0, /* n[0]*n[1]*(n[2]) values, x fastest, no ghosts */
+  SPS_LAYOUT_PADDED = 1    /* SymPhas BoundaryGrid layout: (n+6) per axis */
+} sps_layout;
+
+/* Which kernel family advances the field.
+ *  REFERENCE: one site per thread, every term evaluated in descriptor order
+ *             with the reference's stencil groupings and no fused
+ *             multiply-add; the FP64 form reproduces the CPU SolverFT bits.
+ *  FUSED:     LDS/register-tiled kernel on the canonical form
+ *             dpsi/dt = A(psi) + lap(B(psi) + lap(C(psi))), valid when the
+ *             bilaplacian stencil equals laplacian∘laplacian (2D <5,*,13>,
+ *             3D <7,*,25>) and no gradlaplacian term is present; several
+ *             steps may be fused per launch.
+ *  AUTO:      FUSED when the descriptor allows it, else REFERENCE. */
+typedef enum {
+  SPS_PATH_AUTO = 0,
+  SPS_PATH_REFERENCE = 1,
+  SPS_PATH_FUSED = 2
+} sps_path;
+
+/* One additive term of the EVOLUTION right-hand side:
+ *     coeff * D[ P(psi) ],   P(psi) = sum_k poly[k] * psi^k.
+ * This is the canonical form SolverFT::form_expr_one produces
+ * (solverft.h:112-119): apply_operators distributes derivatives over sums
+ * (sym/inc/expressionapply.h:1476-1488) and a derivative of a non-trivial
+ * operand is evaluated through a temporary grid
+ * (sym/inc/expressionderivatives.h:1545-1593).  When P is exactly psi the
+ * stencil reads the field directly (expressionderivatives.h:1658-1682). */
+typedef struct {
+  int32_t op;        /* sps_op */
+  int32_t axis;      /* sps_axis, GRADLAP only: get_stride<axis> convention */
+  int32_t component; /* GRADLAP with the 2D 6-point stencil only: 0 or 1 */
+  int32_t reserved;
+  double coeff;
+  double poly[SPS_MAX_DEGREE + 1];
+} sps_fd_term;
+
+/* Problem descriptor: SolverFT<Stencil{2,3}d2h<L,G,B>> on one periodic field. */
+typedef struct {
+  int32_t dim;       /* 2 or 3 */
+  int32_t dtype;     /* sps_dtype: storage and arithmetic of the field */
+  int32_t n[3];      /* interior counts per axis (n[2] ignored in 2D) */
+  int32_t lap_points, gradlap_points, bilap_points; /* <L, G, B> */
+  int32_t nterms;
+  int32_t path;      /* sps_path */
+  double h;          /* grid spacing, the same on every axis */
+  double dt;
+  sps_fd_term terms[SPS_MAX_TERMS];
+} sps_fd_desc;
+
+typedef struct sps_fd_plan sps_fd_plan;
+typedef struct sps_fd_batch sps_fd_batch;
+
+/* ---------------------------------------------------------------- library */
+
+/* Text of the last error on this thread ("" if none). */
+const char* sps_fd_last_error(void);
+/* ABI version and sizeof(sps_fd_desc) the library was built with. */
+int sps_fd_abi_version(void);
+size_t sps_fd_desc_size(void);
+/* Number of HIP devices visible (0 when none); never fails. */
+int sps_fd_device_count(void);
+
+/* ------------------------------------------------------------ single field */
+
+/* Validate the descriptor, lower its terms, allocate device storage for the
+ * field on the current device.
+ * Replaces: SolverFT::make_solver (solverft.h:121-161), the SolverSystemFD
+ * constructor (sol/inc/solversystem.h:50-60, sol/inc/boundarysystem.h:
+ * 155-214) and SolverFT::form_expr_one (solverft.h:112-119). */
+int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out);
+int sps_fd_destroy(sps_fd_plan* plan);
+
+/* Copy a host field to the device.  PADDED ignores the ghost cells.
+ * Replaces: populating the system's interior from initial conditions
+ * (sol/inc/boundarysystem.h:498-508, sol/inc/systemlib.h:1949-1959). */
+int sps_fd_upload(sps_fd_plan* plan, const void* host, int layout);
+
+/* Copy the field to the host.  PADDED fills the ghost cells with their
+ * periodic values, i.e. the state the reference holds after update().
+ * Replaces: reading system.values / persist (sol/inc/solversystem.h:835-838). */
+int sps_fd_download(sps_fd_plan* plan, void* host, int layout);
+
+/* Advance nsteps forward-Euler steps.
+ * Replaces: model_iteration's equation → step → update sequence
+ * (modules-models.h:436-467) driven by run_model (modules-models.h:481-492). */
+int sps_fd_step(sps_fd_plan* plan, int nsteps);
+
+/* Evaluate the right-hand side (dframe over the interior) of the current
+ * field without advancing it, and copy it to host in the given layout
+ * (PADDED leaves the ghosts of dframe at zero, as the reference does).
+ * Replaces: SolverFT::equation (solverft.h:90-103). */
+int sps_fd_equation(sps_fd_plan* plan, void* host_dframe, int layout);
+
+/* Apply one derivative (op, axis, component as in sps_fd_term) to the current
+ * field and copy the interior result to host.
+ * Replaces: Stencil::applied_laplacian / applied_bilaplacian /
+ * applied_gradlaplacian<ax> (sol/inc/stencil.h:130-159). */
+int sps_fd_apply(sps_fd_plan* plan, int op, int axis, int component,
+                 void* host_out, int layout);
+
+/* FP64 global sums of the current field: mass = sum psi over the interior,
+ * energy = h^D * sum [ -1/2 psi^2 + 1/4 psi^4 + 1/2 |grad+ psi|^2 ] with
+ * forward differences for grad+.  Either pointer may be NULL.
+ * Replaces: DensityData (dataprocesses/datadensity.h:11-41) and the Landau
+ * free energy landau_fe (sol/inc/modelspecialized.h:31-45). */
+int sps_fd_reduce(sps_fd_plan* plan, double* mass, double* energy);
+
+/* Wait for all work enqueued on the plan. */
+int sps_fd_sync(sps_fd_plan* plan);
+
+/* Enqueue the plan's later work on a caller-owned hipStream_t (NULL restores
+ * the plan's own stream).  The caller keeps the stream alive while the plan
+ * uses it.  No counterpart in the reference, whose CUDA path has no streams
+ * (sym/inc/expressioncuda.cuh:1989-2031 synchronises the device after every
+ * evaluation); it lets a host share one stream between the solver and its own
+ * work, events and captures. */
+int sps_fd_set_stream(sps_fd_plan* plan, void* hip_stream);
+
+/* Device pointer of the current field (interior layout, dtype of the plan),
+ * valid until the next sps_fd_step on the plan. */
+int sps_fd_device_field(sps_fd_plan* plan, void** dptr);
+
+/* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
+int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
+
+/* Override the fused path's launch shape (0 keeps the current value):
+ * steps_per_launch in 1..4, rows_per_tile a multiple of 4 in 4..512. */
+int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile);
+
+/* ---------------------------------------------------------------- batched */
+
+/* Many independent 2D fields of uneven sizes and coefficients, advanced by one
+ * persistent launch per step, captured once in a hipGraph and replayed.
+ * Every descriptor must be 2D, share dtype, lower to the fused form and have
+ * n[0] = n[1] = 64*v for v in {2,3,4,6,8}.
+ * Replaces: find_solution over an array of models, which steps each model in
+ * turn (modules-io.h:368-474, loop at :462-465). */
+int sps_fd_batch_create(const sps_fd_desc* descs, int count,
+                        sps_fd_batch** out);
+int sps_fd_batch_destroy(sps_fd_batch* batch);
+int sps_fd_batch_upload(sps_fd_batch* batch, int index, const void* host,
+                        int layout);
+int sps_fd_batch_download(sps_fd_batch* batch, int index, void* host,
+                          int layout);
+int sps_fd_batch_step(sps_fd_batch* batch, int nsteps);
+/* mass and energy are arrays of count doubles (either may be NULL). */
+int sps_fd_batch_reduce(sps_fd_batch* batch, double* mass, double* energy);
+int sps_fd_batch_sync(sps_fd_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPS_FD_H */

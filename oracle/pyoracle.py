@@ -1,0 +1,138 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, always as the checker.  It takes the same descriptor structure as the
+product's C ABI (the two layouts are identical).
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_double, c_int, c_int64, c_size_t, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libsps_oracle.so"
+DEPTH = 3
+
+_lib = None
+
+
+def build() -> Path:
+    res = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stderr}")
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = c_void_p
+    for name, res, args in [
+        ("orc_desc_size", c_size_t, []),
+        ("orc_check", c_int, [P]),
+        ("orc_padded_len", c_size_t, [P]),
+        ("orc_pad", None, [P, P, P]),
+        ("orc_unpad", None, [P, P, P]),
+        ("orc_refresh", None, [P, P]),
+        ("orc_refresh_modular", None, [P, P]),
+        ("orc_stencil", c_double, [P, c_int, c_int, c_int, P, c_int64]),
+        ("orc_apply", None, [P, c_int, c_int, c_int, P, P, c_int]),
+        ("orc_equation", c_int, [P, P, P, c_int]),
+        ("orc_euler", None, [P, P, P, c_int]),
+        ("orc_run", c_int, [P, P, c_int, c_int]),
+        ("orc_mass", c_double, [P, P]),
+        ("orc_energy", c_double, [P, P]),
+        ("orc_fill_uniform", None, [P, c_size_t, c_uint64, c_double, c_double]),
+    ]:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+def _d(desc):
+    return ctypes.cast(ctypes.byref(desc), c_void_p)
+
+
+def padded_shape(desc):
+    return tuple(int(desc.n[a]) + 2 * DEPTH for a in reversed(range(desc.dim)))
+
+
+def interior_shape(desc):
+    return tuple(int(desc.n[a]) for a in reversed(range(desc.dim)))
+
+
+def pad(desc, interior):
+    out = np.zeros(padded_shape(desc))
+    load().orc_pad(_d(desc), _p(np.ascontiguousarray(interior, dtype=np.float64)), _p(out))
+    return out
+
+
+def unpad(desc, padded):
+    out = np.empty(interior_shape(desc))
+    load().orc_unpad(_d(desc), _p(np.ascontiguousarray(padded, dtype=np.float64)), _p(out))
+    return out
+
+
+def refresh(desc, padded, modular=False):
+    fn = load().orc_refresh_modular if modular else load().orc_refresh
+    fn(_d(desc), _p(padded))
+    return padded
+
+
+def apply(desc, op, axis, component, padded, nthreads=1):
+    out = np.zeros_like(padded)
+    load().orc_apply(_d(desc), op, axis, component, _p(padded), _p(out), nthreads)
+    return out
+
+
+def stencil_at(desc, op, axis, component, padded, idx):
+    return load().orc_stencil(_d(desc), op, axis, component, _p(padded), int(idx))
+
+
+def equation(desc, padded, nthreads=1):
+    out = np.zeros_like(padded)
+    rc = load().orc_equation(_d(desc), _p(padded), _p(out), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_equation failed ({rc})")
+    return out
+
+
+def run(desc, interior, nsteps, nthreads=1):
+    """Return the padded field after nsteps (ghosts refreshed)."""
+    g = pad(desc, interior)
+    rc = load().orc_run(_d(desc), _p(g), int(nsteps), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_run failed ({rc})")
+    return g
+
+
+def mass(desc, padded):
+    return load().orc_mass(_d(desc), _p(padded))
+
+
+def energy(desc, padded):
+    return load().orc_energy(_d(desc), _p(padded))
+
+
+def fill_uniform(count, seed, lo, hi):
+    out = np.empty(count)
+    load().orc_fill_uniform(_p(out), count, seed, lo, hi)
+    return out
+
+
+def check(desc):
+    return load().orc_check(_d(desc))

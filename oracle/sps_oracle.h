@@ -1,0 +1,117 @@
+/*
+ * sps_oracle.h - CPU restatement of the SymPhas SolverFT step, FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP solver:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it.  Nothing in the product links or calls it.
+ *
+ * It follows the reference's algorithm on the reference's padded layout
+ * (BOUNDARY_DEPTH = 3 ghost cells per side, datatypes/inc/gridinfo.h:46; x
+ * fastest, datatypes/inc/grid.h:537-578):
+ *   - periodic ghost refresh       datatypes/inc/boundaryupdate.h:154-356,650-708
+ *   - stencils, reference grouping sol/inc/stencilh2.h:87-507
+ *   - strides per axis             datatypes/inc/grid.h:537-578
+ *   - derivative of a non-trivial operand through a temporary grid filled
+ *     over the whole padded array  sym/inc/expressionderivatives.h:1583-1588
+ *   - RHS over the interior        examples/solvers/solverft.h:90-103,
+ *                                  datatypes/inc/dataiterator.h:2443-2450
+ *   - forward Euler, interior only examples/solvers/solverft.h:46-50
+ *   - sequencing                   modules-models.h:436-492, modules-io.h:450-452
+ *
+ * Parity pins (tests/test_oracle.py): every stencil is checked against the
+ * moment conditions of a consistent 2nd-order stencil, the Fourier symbols of
+ * lap5/lap9/lap7 and bilap = lap∘lap for <13>/<25>, an independent numpy
+ * np.roll implementation on tiny grids, the exact single-mode recursion of
+ * dpsi/dt = lap psi, constant-field recursions, mass conservation, and the
+ * 2D ghost-strip copies against modular indexing.  The gradlaplacian Z-axis
+ * sign and the lap19 "vz + vz" term are reproduced and pinned by special
+ * cases, and so is the 2D 21-point bilaplacian's 1/12 factor (stencilh2.h:
+ * 143-157), which makes it return a quarter of the bilaplacian.  Parity is
+ * unpinned only for the reference's choice of term order
+ * inside its expression optimiser (expr::transform::optimize may reorder or
+ * distribute terms, e.g. -(psi^2-1)*psi versus psi - psi^3); the oracle sums
+ * terms in descriptor order and powers in increasing degree.  That choice
+ * changes FP64 rounding only (a relative 1e-16 per step).
+ */
+#ifndef SPS_ORACLE_H
+#define SPS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_DEPTH 3
+#define ORC_MAX_DEGREE 6
+#define ORC_MAX_TERMS 8
+
+enum { ORC_OP_ID = 0, ORC_OP_LAP = 1, ORC_OP_BILAP = 2, ORC_OP_GRADLAP = 3 };
+
+typedef struct {
+  int32_t op, axis, component, reserved;
+  double coeff;
+  double poly[ORC_MAX_DEGREE + 1];
+} orc_term;
+
+/* Same memory layout as the product's problem descriptor, so one host-side
+ * structure can describe a problem to both. */
+typedef struct {
+  int32_t dim, dtype;
+  int32_t n[3];
+  int32_t lap_points, gradlap_points, bilap_points;
+  int32_t nterms, path;
+  double h, dt;
+  orc_term terms[ORC_MAX_TERMS];
+} orc_desc;
+
+size_t orc_desc_size(void);
+/* 0 if the descriptor is valid for the oracle, else a negative code. */
+int orc_check(const orc_desc* d);
+/* Number of values in the padded array. */
+size_t orc_padded_len(const orc_desc* d);
+
+/* interior <-> padded copies (ghosts of the padded array are left alone) */
+void orc_pad(const orc_desc* d, const double* interior, double* padded);
+void orc_unpad(const orc_desc* d, const double* padded, double* interior);
+
+/* Periodic ghost refresh of a padded array. 2D follows the reference's strip
+ * copies; 3D writes each ghost from its modular interior image (the combined
+ * effect of the 6 face, 12 edge and 8 corner copies). */
+void orc_refresh(const orc_desc* d, double* g);
+/* Modular-index refresh, used by the tests to pin orc_refresh. */
+void orc_refresh_modular(const orc_desc* d, double* g);
+
+/* One derivative at padded linear index idx (op: ORC_OP_LAP/BILAP/GRADLAP). */
+double orc_stencil(const orc_desc* d, int op, int axis, int component,
+                   const double* g, int64_t idx);
+/* Apply one derivative over the interior into out (padded array; ghosts of
+ * out untouched). */
+void orc_apply(const orc_desc* d, int op, int axis, int component,
+               const double* g, double* out, int nthreads);
+
+/* RHS of the descriptor's terms over the interior into dframe (padded). */
+int orc_equation(const orc_desc* d, const double* psi, double* dframe,
+                 int nthreads);
+/* psi = psi + dt*dframe over the interior. */
+void orc_euler(const orc_desc* d, double* psi, const double* dframe,
+               int nthreads);
+/* Refresh, then nsteps x (equation, euler, refresh).  psi is padded. */
+int orc_run(const orc_desc* d, double* psi, int nsteps, int nthreads);
+
+/* Sequential FP64 sum of the interior (dataprocesses/datadensity.h:11-41). */
+double orc_mass(const orc_desc* d, const double* psi);
+/* h^D * sum[-1/2 psi^2 + 1/4 psi^4 + 1/2 |grad+ psi|^2], forward differences;
+ * psi must have fresh ghosts. */
+double orc_energy(const orc_desc* d, const double* psi);
+
+/* SplitMix64 stream -> 53-bit uniform doubles in [lo, hi). */
+void orc_fill_uniform(double* out, size_t count, uint64_t seed, double lo,
+                      double hi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

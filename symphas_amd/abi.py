@@ -1,0 +1,100 @@
+"""ctypes view of include/sps_fd.h.
+
+The structures mirror the C declarations field by field; `Desc` is also the
+layout of the CPU oracle's problem descriptor, so one instance can describe a
+problem to both.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_double, c_int, c_int32, c_size_t, c_void_p, POINTER
+from pathlib import Path
+
+MAX_DEGREE = 6
+MAX_TERMS = 8
+BOUNDARY_DEPTH = 3
+
+OK, E_INVALID, E_UNSUPPORTED, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
+F32, F64 = 0, 1
+OP_ID, OP_LAP, OP_BILAP, OP_GRADLAP = 0, 1, 2, 3
+AXIS_X, AXIS_Y, AXIS_Z = 0, 1, 2
+LAYOUT_INTERIOR, LAYOUT_PADDED = 0, 1
+PATH_AUTO, PATH_REFERENCE, PATH_FUSED = 0, 1, 2
+
+STATUS_NAMES = {OK: "SPS_OK", E_INVALID: "SPS_E_INVALID", E_UNSUPPORTED: "SPS_E_UNSUPPORTED",
+                E_HIP: "SPS_E_HIP", E_NOMEM: "SPS_E_NOMEM", E_STATE: "SPS_E_STATE"}
+
+
+class Term(ctypes.Structure):
+    _fields_ = [("op", c_int32), ("axis", c_int32), ("component", c_int32),
+                ("reserved", c_int32), ("coeff", c_double),
+                ("poly", c_double * (MAX_DEGREE + 1))]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [("dim", c_int32), ("dtype", c_int32), ("n", c_int32 * 3),
+                ("lap_points", c_int32), ("gradlap_points", c_int32),
+                ("bilap_points", c_int32), ("nterms", c_int32), ("path", c_int32),
+                ("h", c_double), ("dt", c_double), ("terms", Term * MAX_TERMS)]
+
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libsps_fd.so"
+
+_lib = None
+
+
+class SpsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+def load() -> ctypes.CDLL:
+    """Load the in-tree HIP library; fails loudly when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (there is no CPU fallback for the solver step)")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = c_void_p
+    sig = {
+        "sps_fd_last_error": (ctypes.c_char_p, []),
+        "sps_fd_abi_version": (c_int, []),
+        "sps_fd_desc_size": (c_size_t, []),
+        "sps_fd_device_count": (c_int, []),
+        "sps_fd_create": (c_int, [POINTER(Desc), POINTER(P)]),
+        "sps_fd_destroy": (c_int, [P]),
+        "sps_fd_upload": (c_int, [P, P, c_int]),
+        "sps_fd_download": (c_int, [P, P, c_int]),
+        "sps_fd_step": (c_int, [P, c_int]),
+        "sps_fd_equation": (c_int, [P, P, c_int]),
+        "sps_fd_apply": (c_int, [P, c_int, c_int, c_int, P, c_int]),
+        "sps_fd_reduce": (c_int, [P, POINTER(c_double), POINTER(c_double)]),
+        "sps_fd_sync": (c_int, [P]),
+        "sps_fd_device_field": (c_int, [P, POINTER(P)]),
+        "sps_fd_set_stream": (c_int, [P, P]),
+        "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
+        "sps_fd_tune": (c_int, [P, c_int, c_int]),
+        "sps_fd_batch_create": (c_int, [POINTER(Desc), c_int, POINTER(P)]),
+        "sps_fd_batch_destroy": (c_int, [P]),
+        "sps_fd_batch_upload": (c_int, [P, c_int, P, c_int]),
+        "sps_fd_batch_download": (c_int, [P, c_int, P, c_int]),
+        "sps_fd_batch_step": (c_int, [P, c_int]),
+        "sps_fd_batch_reduce": (c_int, [P, POINTER(c_double), POINTER(c_double)]),
+        "sps_fd_batch_sync": (c_int, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = load().sps_fd_last_error()
+        raise SpsError(status, msg.decode() if msg else "")

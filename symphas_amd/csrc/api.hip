@@ -1,0 +1,648 @@
+// api.hip - the C ABI declared in include/sps_fd.h (single-field plans).
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "fused.hpp"
+#include "plan.hpp"
+#include "ref_kernels.hpp"
+
+namespace sps {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+const char* last_error() { return g_err; }
+
+static bool valid_points(const sps_fd_desc& d) {
+  const int L = d.lap_points, G = d.gradlap_points, B = d.bilap_points;
+  if (d.dim == 2)
+    return (L == 5 || L == 9) && (G == 6 || G == 8 || G == 12 || G == 16) &&
+           (B == 13 || B == 17 || B == 21);
+  return (L == 7 || L == 15 || L == 19 || L == 21 || L == 27) &&
+         (G == 10 || G == 12 || G == 28 || G == 36 || G == 40) &&
+         (B == 21 || B == 25 || B == 41 || B == 52 || B == 57);
+}
+
+int validate_desc(const sps_fd_desc* d) {
+  if (!d) {
+    set_error("descriptor is NULL");
+    return SPS_E_INVALID;
+  }
+  if (d->dim != 2 && d->dim != 3) {
+    set_error("dim must be 2 or 3 (got %d)", d->dim);
+    return SPS_E_INVALID;
+  }
+  if (d->dtype != SPS_F32 && d->dtype != SPS_F64) {
+    set_error("dtype must be SPS_F32 or SPS_F64 (got %d)", d->dtype);
+    return SPS_E_INVALID;
+  }
+  for (int a = 0; a < d->dim; ++a) {
+    // the reference's periodic ghost copy needs at least BOUNDARY_DEPTH
+    // interior cells per axis (datatypes/inc/boundaryupdate.h:650-708)
+    if (d->n[a] < SPS_BOUNDARY_DEPTH || d->n[a] > (1 << 20)) {
+      set_error("n[%d] = %d out of range [%d, 2^20]", a, d->n[a],
+                SPS_BOUNDARY_DEPTH);
+      return SPS_E_INVALID;
+    }
+  }
+  double sites = (double)d->n[0] * d->n[1] * (d->dim == 3 ? d->n[2] : 1);
+  if (sites > 2147483647.0) {
+    set_error("field of %.0f sites exceeds 2^31-1", sites);
+    return SPS_E_INVALID;
+  }
+  if (!(d->h > 0) || !std::isfinite(d->h) || !std::isfinite(d->dt)) {
+    set_error("h must be positive and finite, dt finite");
+    return SPS_E_INVALID;
+  }
+  if (!valid_points(*d)) {
+    set_error("no Stencil%dd2h<%d,%d,%d> in the reference's point lists",
+              d->dim, d->lap_points, d->gradlap_points, d->bilap_points);
+    return SPS_E_INVALID;
+  }
+  if (d->nterms < 0 || d->nterms > SPS_MAX_TERMS) {
+    set_error("nterms %d not in [0, %d]", d->nterms, SPS_MAX_TERMS);
+    return SPS_E_INVALID;
+  }
+  for (int t = 0; t < d->nterms; ++t) {
+    const sps_fd_term& tm = d->terms[t];
+    if (tm.op < SPS_OP_ID || tm.op > SPS_OP_GRADLAP) {
+      set_error("term %d: unknown op %d", t, tm.op);
+      return SPS_E_INVALID;
+    }
+    if (tm.op == SPS_OP_GRADLAP) {
+      if (tm.axis < 0 || tm.axis >= d->dim) {
+        set_error("term %d: gradlaplacian axis %d", t, tm.axis);
+        return SPS_E_INVALID;
+      }
+      const int ncomp = (d->dim == 2 && d->gradlap_points == 6) ? 2 : 1;
+      if (tm.component < 0 || tm.component >= ncomp) {
+        set_error("term %d: component %d (stencil has %d)", t, tm.component,
+                  ncomp);
+        return SPS_E_INVALID;
+      }
+    }
+    if (!std::isfinite(tm.coeff)) {
+      set_error("term %d: coefficient not finite", t);
+      return SPS_E_INVALID;
+    }
+    for (int k = 0; k <= SPS_MAX_DEGREE; ++k)
+      if (!std::isfinite(tm.poly[k])) {
+        set_error("term %d: polynomial coefficient %d not finite", t, k);
+        return SPS_E_INVALID;
+      }
+  }
+  if (d->path < SPS_PATH_AUTO || d->path > SPS_PATH_FUSED) {
+    set_error("path %d unknown", d->path);
+    return SPS_E_INVALID;
+  }
+  return SPS_OK;
+}
+
+// Lower the descriptor's terms into dpsi/dt = A + lap(B) + kc*bilap(psi).
+Lowered lower(const sps_fd_desc& d) {
+  Lowered lw;
+  const int lapL = d.dim == 2 ? 5 : 7;
+  const int bilB = d.dim == 2 ? 13 : 25;
+  bool any_lap = false, any_bilap = false;
+  for (int t = 0; t < d.nterms; ++t) {
+    const sps_fd_term& tm = d.terms[t];
+    switch (tm.op) {
+      case SPS_OP_ID:
+        for (int k = 0; k <= kMaxDeg; ++k) lw.a[k] += tm.coeff * tm.poly[k];
+        break;
+      case SPS_OP_LAP:
+        any_lap = true;
+        for (int k = 1; k <= kMaxDeg; ++k) lw.b[k] += tm.coeff * tm.poly[k];
+        break;
+      case SPS_OP_BILAP:
+        any_bilap = true;
+        for (int k = 2; k <= kMaxDeg; ++k)
+          if (tm.poly[k] != 0.0) {
+            lw.why_not = "bilaplacian of a non-linear operand";
+            return lw;
+          }
+        lw.kc += tm.coeff * tm.poly[1];
+        break;
+      default:
+        lw.why_not = "gradlaplacian terms";
+        return lw;
+    }
+  }
+  if (any_lap && d.lap_points != lapL) {
+    lw.why_not = d.dim == 2 ? "laplacian stencil is not the 5-point one"
+                            : "laplacian stencil is not the 7-point one";
+    return lw;
+  }
+  if (any_bilap && d.bilap_points != bilB) {
+    lw.why_not = d.dim == 2 ? "bilaplacian stencil is not the 13-point one"
+                            : "bilaplacian stencil is not the 25-point one";
+    return lw;
+  }
+  lw.fusable = true;
+  return lw;
+}
+
+template <typename T>
+void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c) {
+  const double divh = 1.0 / d.h;
+  const double divh2 = divh * divh;
+  int dega = -1, degb = -1;
+  for (int k = 0; k <= kMaxDeg; ++k) {
+    c.adt[k] = T(d.dt * lw.a[k]);
+    c.b[k] = T(lw.b[k]);
+    if (lw.a[k] != 0.0) dega = k;
+    if (lw.b[k] != 0.0) degb = k;
+  }
+  c.kmu = T(lw.kc * divh2);
+  c.kd = T(d.dt * divh2);
+  c.k1 = T(d.dt * divh2 * lw.b[1]);
+  c.dega = dega;
+  c.degb = degb;
+  c.radius = (lw.kc == 0.0 && degb <= 1) ? 1 : 2;
+}
+
+template <typename T>
+void make_ref_params(const sps_fd_desc& d, RefParams<T>& p) {
+  std::memset(&p, 0, sizeof(p));
+  for (int a = 0; a < 3; ++a) p.n[a] = a < d.dim ? d.n[a] : 1;
+  p.L = d.lap_points;
+  p.G = d.gradlap_points;
+  p.B = d.bilap_points;
+  p.nterms = d.nterms;
+  // divh = 1/h; divh2..4 as StencilBase{2,3}d2h builds them
+  // (sol/inc/stencilincludes.h:100-104,123-127)
+  const double divh = 1.0 / d.h;
+  const double divh2 = divh * divh;
+  const double divh3 = divh * divh2;
+  const double divh4 = divh2 * divh2;
+  p.divh2 = T(divh2);
+  p.divh3 = T(divh3);
+  p.divh4 = T(divh4);
+  p.dt = T(d.dt);
+  for (int t = 0; t < d.nterms; ++t) {
+    const sps_fd_term& tm = d.terms[t];
+    DevTerm<T>& dt = p.terms[t];
+    dt.op = tm.op;
+    dt.axis = tm.axis;
+    dt.component = tm.component;
+    bool ident = tm.poly[1] == 1.0;
+    for (int k = 0; k <= kMaxDeg; ++k) {
+      dt.poly[k] = T(tm.poly[k]);
+      if (k != 1 && tm.poly[k] != 0.0) ident = false;
+    }
+    dt.identity = ident ? 1 : 0;
+    dt.coeff = T(tm.coeff);
+  }
+}
+
+template void make_fused_coef<float>(const sps_fd_desc&, const Lowered&,
+                                     FusedCoef<float>&);
+template void make_fused_coef<double>(const sps_fd_desc&, const Lowered&,
+                                      FusedCoef<double>&);
+
+}  // namespace sps
+
+using namespace sps;
+
+// ----------------------------------------------------------------- plan
+
+static size_t sites_of(const sps_fd_desc& d) {
+  size_t n = (size_t)d.n[0] * d.n[1];
+  if (d.dim == 3) n *= d.n[2];
+  return n;
+}
+
+static size_t padded_sites(const sps_fd_desc& d) {
+  size_t n = (size_t)(d.n[0] + 2 * SPS_BOUNDARY_DEPTH) *
+             (d.n[1] + 2 * SPS_BOUNDARY_DEPTH);
+  if (d.dim == 3) n *= d.n[2] + 2 * SPS_BOUNDARY_DEPTH;
+  return n;
+}
+
+namespace sps {
+
+// interior (unpadded) <-> SymPhas padded layout, element size es bytes
+void interior_to_padded(const sps_fd_desc& d, const char* src, char* dst,
+                        size_t es, bool fill_ghosts) {
+  const int D = SPS_BOUNDARY_DEPTH;
+  const int64_t X = d.n[0] + 2 * D, Y = d.n[1] + 2 * D;
+  const int64_t Z = d.dim == 3 ? d.n[2] + 2 * D : 1;
+  const int64_t nx = d.n[0], ny = d.n[1], nz = d.dim == 3 ? d.n[2] : 1;
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      for (int64_t x = 0; x < X; ++x) {
+        const bool ghost = x < D || x >= X - D || y < D || y >= Y - D ||
+                           (d.dim == 3 && (z < D || z >= Z - D));
+        char* out = dst + es * (x + X * (y + Y * z));
+        if (ghost && !fill_ghosts) {
+          std::memset(out, 0, es);
+          continue;
+        }
+        int64_t ix = ((x - D) % nx + nx) % nx;
+        int64_t iy = ((y - D) % ny + ny) % ny;
+        int64_t iz = d.dim == 3 ? ((z - D) % nz + nz) % nz : 0;
+        std::memcpy(out, src + es * (ix + nx * (iy + ny * iz)), es);
+      }
+}
+
+void padded_to_interior(const sps_fd_desc& d, const char* src, char* dst,
+                        size_t es) {
+  const int D = SPS_BOUNDARY_DEPTH;
+  const int64_t X = d.n[0] + 2 * D, Y = d.n[1] + 2 * D;
+  const int64_t nx = d.n[0], ny = d.n[1], nz = d.dim == 3 ? d.n[2] : 1;
+  const int64_t zo = d.dim == 3 ? D : 0;
+  for (int64_t z = 0; z < nz; ++z)
+    for (int64_t y = 0; y < ny; ++y)
+      std::memcpy(dst + es * (nx * (y + ny * z)),
+                  src + es * (D + X * ((y + D) + Y * (z + zo))), es * nx);
+}
+
+}  // namespace sps
+
+int sps_fd_plan::launch_one(int k) {
+  const void* in = buf[cur];
+  void* out = buf[1 - cur];
+  int rc;
+  if (path == SPS_PATH_REFERENCE) {
+    rc = d.dtype == SPS_F32
+             ? launch_ref<float>((const float*)in, (float*)out, rp32, d.dim,
+                                 stream)
+             : launch_ref<double>((const double*)in, (double*)out, rp64,
+                                  d.dim, stream);
+  } else if (d.dim == 2) {
+    rc = d.dtype == SPS_F32
+             ? launch_fused2d<float>((const float*)in, (float*)out, d.n[0],
+                                     d.n[1], k, TY, fc32, stream)
+             : launch_fused2d<double>((const double*)in, (double*)out, d.n[0],
+                                      d.n[1], k, TY, fc64, stream);
+  } else {
+    rc = d.dtype == SPS_F32
+             ? launch_fused3d<float>((const float*)in, (float*)out, d.n[0],
+                                     d.n[1], d.n[2], TY, fc32, stream)
+             : launch_fused3d<double>((const double*)in, (double*)out, d.n[0],
+                                      d.n[1], d.n[2], TY, fc64, stream);
+  }
+  if (rc == SPS_OK) cur = 1 - cur;
+  return rc;
+}
+
+int sps_fd_plan::steps_per_launch() const {
+  return (path == SPS_PATH_FUSED && d.dim == 2) ? K : 1;
+}
+
+void sps_fd_plan::drop_graphs() {
+  for (auto& g : graphs) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    g = GraphSlot{};
+  }
+}
+
+int sps_fd_plan::step(int nsteps) {
+  const int spl = steps_per_launch();
+  int launches = nsteps / spl;
+  const int rem = nsteps % spl;
+  if (launches >= 2 * kGraphLaunches) {
+    GraphSlot& g = graphs[cur];
+    if (!g.exec) {
+      const int start = cur;
+      SPS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+      int rc = SPS_OK;
+      for (int i = 0; i < kGraphLaunches && rc == SPS_OK; ++i)
+        rc = launch_one(spl);
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamEndCapture(stream, &graph);
+      cur = start;  // capture records work; nothing ran
+      if (rc != SPS_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) {
+        set_error("graph capture failed: %s", hipGetErrorString(e));
+        return SPS_E_HIP;
+      }
+      hipGraphExec_t exec = nullptr;
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        set_error("graph instantiate failed: %s", hipGetErrorString(e));
+        return SPS_E_HIP;
+      }
+      g.graph = graph;
+      g.exec = exec;
+    }
+    while (launches >= kGraphLaunches) {
+      SPS_HIP_TRY(hipGraphLaunch(g.exec, stream));
+      launches -= kGraphLaunches;  // even count: parity unchanged
+    }
+  }
+  for (int i = 0; i < launches; ++i) SPS_TRY(launch_one(spl));
+  if (rem) SPS_TRY(launch_one(rem));
+  return SPS_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* sps_fd_last_error(void) { return sps::last_error(); }
+int sps_fd_abi_version(void) { return SPS_FD_ABI_VERSION; }
+size_t sps_fd_desc_size(void) { return sizeof(sps_fd_desc); }
+
+int sps_fd_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
+  if (!out) {
+    set_error("out is NULL");
+    return SPS_E_INVALID;
+  }
+  *out = nullptr;
+  SPS_TRY(validate_desc(desc));
+  if (sps_fd_device_count() < 1) {
+    set_error("no HIP device: this library runs the step on an MI355X only");
+    return SPS_E_HIP;
+  }
+  sps_fd_plan* p = new (std::nothrow) sps_fd_plan();
+  if (!p) {
+    set_error("host allocation failed");
+    return SPS_E_NOMEM;
+  }
+  p->d = *desc;
+  p->low = lower(*desc);
+  const int vec = desc->dtype == SPS_F32 ? 4 : 2;
+  const bool fusable =
+      p->low.fusable && (desc->dim == 3 || desc->n[0] % vec == 0);
+  if (desc->path == SPS_PATH_FUSED && !fusable) {
+    set_error("fused path unavailable: %s",
+              p->low.fusable ? "x extent not a multiple of the vector width"
+                             : p->low.why_not);
+    delete p;
+    return SPS_E_UNSUPPORTED;
+  }
+  p->path = (desc->path == SPS_PATH_REFERENCE || !fusable)
+                ? SPS_PATH_REFERENCE
+                : SPS_PATH_FUSED;
+  make_ref_params(*desc, p->rp32);
+  make_ref_params(*desc, p->rp64);
+  make_fused_coef(*desc, p->low, p->fc32);
+  make_fused_coef(*desc, p->low, p->fc64);
+  p->choose_default_shape();
+  p->nsites = sites_of(*desc);
+  p->esize = desc->dtype == SPS_F32 ? 4 : 8;
+  int rc = p->allocate();
+  if (rc != SPS_OK) {
+    sps_fd_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return SPS_OK;
+}
+
+int sps_fd_destroy(sps_fd_plan* p) {
+  if (!p) return SPS_OK;
+  p->drop_graphs();
+  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->own_stream && p->own_stream != p->stream)
+    hipStreamSynchronize(p->own_stream);
+  for (void* b : p->buf)
+    if (b) hipFree(b);
+  if (p->work) hipFree(p->work);
+  if (p->partials) hipFree(p->partials);
+  if (p->sums) hipFree(p->sums);
+  if (p->own_stream) hipStreamDestroy(p->own_stream);
+  delete p;
+  return SPS_OK;
+}
+
+int sps_fd_upload(sps_fd_plan* p, const void* host, int layout) {
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  const size_t bytes = p->nsites * p->esize;
+  if (layout == SPS_LAYOUT_INTERIOR) {
+    SPS_HIP_TRY(hipMemcpyAsync(p->buf[p->cur], host, bytes,
+                               hipMemcpyHostToDevice, p->stream));
+  } else if (layout == SPS_LAYOUT_PADDED) {
+    std::vector<char> tmp(bytes);
+    padded_to_interior(p->d, (const char*)host, tmp.data(), p->esize);
+    SPS_HIP_TRY(hipMemcpyAsync(p->buf[p->cur], tmp.data(), bytes,
+                               hipMemcpyHostToDevice, p->stream));
+    SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  } else {
+    set_error("unknown layout %d", layout);
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  p->has_data = true;
+  return SPS_OK;
+}
+
+static int download_buffer(sps_fd_plan* p, const void* dev, void* host,
+                           int layout, bool fill_ghosts) {
+  const size_t bytes = p->nsites * p->esize;
+  if (layout == SPS_LAYOUT_INTERIOR) {
+    SPS_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost,
+                               p->stream));
+    SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+    return SPS_OK;
+  }
+  if (layout != SPS_LAYOUT_PADDED) {
+    set_error("unknown layout %d", layout);
+    return SPS_E_INVALID;
+  }
+  std::vector<char> tmp(bytes);
+  SPS_HIP_TRY(hipMemcpyAsync(tmp.data(), dev, bytes, hipMemcpyDeviceToHost,
+                             p->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  interior_to_padded(p->d, tmp.data(), (char*)host, p->esize, fill_ghosts);
+  return SPS_OK;
+}
+
+int sps_fd_download(sps_fd_plan* p, void* host, int layout) {
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  return download_buffer(p, p->buf[p->cur], host, layout, true);
+}
+
+int sps_fd_step(sps_fd_plan* p, int nsteps) {
+  if (!p || nsteps < 0) {
+    set_error("plan is NULL or nsteps < 0");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  return p->step(nsteps);
+}
+
+static int run_ref_mode(sps_fd_plan* p, int mode, int op, int axis,
+                        int component) {
+  if (!p->work) SPS_HIP_TRY(hipMalloc(&p->work, p->nsites * p->esize));
+  if (p->d.dtype == SPS_F32) {
+    RefParams<float> rp = p->rp32;
+    rp.mode = mode;
+    rp.apply_op = op;
+    rp.apply_axis = axis;
+    rp.apply_component = component;
+    return launch_ref<float>((const float*)p->buf[p->cur], (float*)p->work,
+                             rp, p->d.dim, p->stream);
+  }
+  RefParams<double> rp = p->rp64;
+  rp.mode = mode;
+  rp.apply_op = op;
+  rp.apply_axis = axis;
+  rp.apply_component = component;
+  return launch_ref<double>((const double*)p->buf[p->cur], (double*)p->work,
+                            rp, p->d.dim, p->stream);
+}
+
+int sps_fd_equation(sps_fd_plan* p, void* host, int layout) {
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  SPS_TRY(run_ref_mode(p, 1, 0, 0, 0));
+  return download_buffer(p, p->work, host, layout, false);
+}
+
+int sps_fd_apply(sps_fd_plan* p, int op, int axis, int component, void* host,
+                 int layout) {
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  if (op != SPS_OP_LAP && op != SPS_OP_BILAP && op != SPS_OP_GRADLAP) {
+    set_error("apply: op must be LAP, BILAP or GRADLAP");
+    return SPS_E_INVALID;
+  }
+  if (op == SPS_OP_GRADLAP) {
+    const int ncomp = (p->d.dim == 2 && p->d.gradlap_points == 6) ? 2 : 1;
+    if (axis < 0 || axis >= p->d.dim || component < 0 || component >= ncomp) {
+      set_error("apply: gradlaplacian axis %d / component %d invalid", axis,
+                component);
+      return SPS_E_INVALID;
+    }
+  }
+  SPS_TRY(run_ref_mode(p, 2, op, axis, component));
+  return download_buffer(p, p->work, host, layout, false);
+}
+
+int sps_fd_reduce(sps_fd_plan* p, double* mass, double* energy) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  int rc = p->d.dtype == SPS_F32
+               ? launch_reduce<float>((const float*)p->buf[p->cur], p->d.dim,
+                                      p->d.n, p->d.h, p->partials, p->sums,
+                                      p->stream)
+               : launch_reduce<double>((const double*)p->buf[p->cur],
+                                       p->d.dim, p->d.n, p->d.h, p->partials,
+                                       p->sums, p->stream);
+  if (rc != SPS_OK) return rc;
+  double h[2];
+  SPS_HIP_TRY(hipMemcpyAsync(h, p->sums, sizeof(h), hipMemcpyDeviceToHost,
+                             p->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  if (mass) *mass = h[0];
+  if (energy) *energy = h[1];
+  return SPS_OK;
+}
+
+int sps_fd_sync(sps_fd_plan* p) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  return SPS_OK;
+}
+
+int sps_fd_set_stream(sps_fd_plan* p, void* hip_stream) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  // order the switch after the work already on the old stream
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  p->stream = hip_stream ? (hipStream_t)hip_stream : p->own_stream;
+  return SPS_OK;
+}
+
+int sps_fd_device_field(sps_fd_plan* p, void** dptr) {
+  if (!p || !dptr) {
+    set_error("plan or out pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  *dptr = p->buf[p->cur];
+  return SPS_OK;
+}
+
+int sps_fd_query(const sps_fd_plan* p, int* path, int* spl) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (path) *path = p->path;
+  if (spl) *spl = p->steps_per_launch();
+  return SPS_OK;
+}
+
+int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (steps_per_launch < 0 || steps_per_launch > 4 ||
+      (rows_per_tile != 0 &&
+       (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4))) {
+    set_error("tune: steps_per_launch in 0..4, rows_per_tile 0 or a "
+              "multiple of 4 in 4..512");
+    return SPS_E_INVALID;
+  }
+  if (steps_per_launch) p->K = steps_per_launch;
+  if (rows_per_tile) p->TY = rows_per_tile;
+  p->drop_graphs();
+  return SPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// fused2d.hip - single-field fused 2D step (K steps per launch).
+#include "fused.hpp"
+#include "fused_tile.hpp"
+
+namespace sps {
+namespace {
+
+// Halo layout: VEC = 16 bytes per lane, HL halo lanes per strip side.
+template <typename T, int K, int R, bool HASA, int POLY>
+__global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
+                                                    T* __restrict__ out, int X,
+                                                    int Y, int TY, int nstrips,
+                                                    int nitems,
+                                                    FusedCoef<T> c) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int HL = (K * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  const int lane = threadIdx.x & 63;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int item = b * 4 + (threadIdx.x >> 6);
+  if (item >= nitems) return;
+  const int strip = item % nstrips;
+  const int chunk = item / nstrips;
+  const int cb = strip * W + (lane - HL) * VEC;
+  int col = cb % X;
+  if (col < 0) col += X;
+  const bool store_lane = lane >= HL && lane < 64 - HL && cb < X;
+  const int y0 = chunk * TY;
+  const int ty = min(TY, Y - y0);
+  stream_tile<T, VEC, K, R, HASA, POLY, false>(in, out, X, Y, y0, ty, col,
+                                               store_lane, c);
+}
+
+template <typename T, int K, int R, bool HASA, int POLY>
+int launch_halo(const T* in, T* out, int X, int Y, int TY,
+                const FusedCoef<T>& c, hipStream_t s) {
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int HL = (K * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  const int nstrips = (X + W - 1) / W;
+  const int nchunks = (Y + TY - 1) / TY;
+  const int nitems = nstrips * nchunks;
+  const int nblocks = (nitems + 3) / 4;
+  fused2d_halo<T, K, R, HASA, POLY>
+      <<<nblocks, 256, 0, s>>>(in, out, X, Y, TY, nstrips, nitems, c);
+  return SPS_OK;
+}
+
+template <typename T, int K, int R, bool HASA, int POLY>
+int dispatch_layout(const T* in, T* out, int X, int Y, int TY,
+                    const FusedCoef<T>& c, hipStream_t s) {
+  return launch_halo<T, K, R, HASA, POLY>(in, out, X, Y, TY, c, s);
+}
+
+template <typename T, int K, int R>
+int dispatch_poly(const T* in, T* out, int X, int Y, int TY,
+                  const FusedCoef<T>& c, hipStream_t s) {
+  const bool hasa = c.dega >= 0;
+  const int deg = c.dega > c.degb ? c.dega : c.degb;
+  if (deg <= 3) {
+    return hasa ? dispatch_layout<T, K, R, true, 3>(in, out, X, Y, TY, c, s)
+                : dispatch_layout<T, K, R, false, 3>(in, out, X, Y, TY, c, s);
+  }
+  return hasa ? dispatch_layout<T, K, R, true, 6>(in, out, X, Y, TY, c, s)
+              : dispatch_layout<T, K, R, false, 6>(in, out, X, Y, TY, c, s);
+}
+
+template <typename T, int K>
+int dispatch_radius(const T* in, T* out, int X, int Y, int TY,
+                    const FusedCoef<T>& c, hipStream_t s) {
+  return c.radius == 1 ? dispatch_poly<T, K, 1>(in, out, X, Y, TY, c, s)
+                       : dispatch_poly<T, K, 2>(in, out, X, Y, TY, c, s);
+}
+
+}  // namespace
+
+template <typename T>
+int launch_fused2d(const T* in, T* out, int X, int Y, int K, int TY,
+                   const FusedCoef<T>& c, hipStream_t s) {
+  if (X % (16 / (int)sizeof(T)) != 0 || TY < 1) {
+    set_error("fused 2D step: X=%d must be a multiple of %d", X,
+              16 / (int)sizeof(T));
+    return SPS_E_UNSUPPORTED;
+  }
+  int rc;
+  switch (K) {
+    case 1: rc = dispatch_radius<T, 1>(in, out, X, Y, TY, c, s); break;
+    case 2: rc = dispatch_radius<T, 2>(in, out, X, Y, TY, c, s); break;
+    case 3: rc = dispatch_radius<T, 3>(in, out, X, Y, TY, c, s); break;
+    case 4: rc = dispatch_radius<T, 4>(in, out, X, Y, TY, c, s); break;
+    default:
+      set_error("fused 2D step: steps per launch %d not in 1..4", K);
+      return SPS_E_INVALID;
+  }
+  if (rc != SPS_OK) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("fused 2D launch failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+template int launch_fused2d<float>(const float*, float*, int, int, int, int,
+                                   const FusedCoef<float>&, hipStream_t);
+template int launch_fused2d<double>(const double*, double*, int, int, int, int,
+                                    const FusedCoef<double>&, hipStream_t);
+
+}  // namespace sps

@@ -1,0 +1,148 @@
+// lanes.hpp - wave64 neighbour exchange through DPP, small vector loads, FMA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sps {
+
+// DPP controls of the GFX9 wave-wide shifts and rotates.
+//   lane i receives from lane i-1 (lane 0 receives 0):   wave_shr:1
+//   lane i receives from lane i+1 (lane 63 receives 0):  wave_shl:1
+//   lane i receives from lane i-1, lane 0 from lane 63:  wave_ror:1
+//   lane i receives from lane i+1, lane 63 from lane 0:  wave_rol:1
+constexpr int kDppWaveShl1 = 0x130;
+constexpr int kDppWaveRol1 = 0x134;
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveRor1 = 0x13C;
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                         0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
+                                             0xF, false);
+  return __builtin_bit_cast(
+      double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Value held by the previous / next lane; ROT wraps around the wave.
+template <bool ROT, typename T>
+__device__ __forceinline__ T from_prev_lane(T v) {
+  return dpp<ROT ? kDppWaveRor1 : kDppWaveShr1>(v);
+}
+template <bool ROT, typename T>
+__device__ __forceinline__ T from_next_lane(T v) {
+  return dpp<ROT ? kDppWaveRol1 : kDppWaveShl1>(v);
+}
+
+__device__ __forceinline__ float fmad(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fmad(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+// Horner evaluation of sum_k c[k] x^k with a compile-time degree bound.
+template <int DEG, typename T>
+__device__ __forceinline__ T horner(const T* c, T x) {
+  if constexpr (DEG < 0) {
+    return T(0);
+  } else {
+    T r = c[DEG];
+#pragma unroll
+    for (int k = DEG - 1; k >= 0; --k) r = fmad(r, x, c[k]);
+    return r;
+  }
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+// Load / store VEC consecutive values; 16-byte vector accesses where the
+// width allows (callers guarantee the alignment).
+template <int VEC>
+__device__ __forceinline__ void load_vec(const float* p, float (&v)[VEC]) {
+  if constexpr (VEC % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 4; ++q) {
+      f32x4 t = *reinterpret_cast<const f32x4*>(p + 4 * q);
+      v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+    }
+  } else if constexpr (VEC % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 2; ++q) {
+      f32x2 t = *reinterpret_cast<const f32x2*>(p + 2 * q);
+      v[2 * q] = t.x; v[2 * q + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) v[q] = p[q];
+  }
+}
+
+template <int VEC>
+__device__ __forceinline__ void load_vec(const double* p, double (&v)[VEC]) {
+  if constexpr (VEC % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 2; ++q) {
+      f64x2 t = *reinterpret_cast<const f64x2*>(p + 2 * q);
+      v[2 * q] = t.x; v[2 * q + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) v[q] = p[q];
+  }
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_vec(float* p, const float (&v)[VEC]) {
+  if constexpr (VEC % 4 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 4; ++q) {
+      f32x4 t = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+      *reinterpret_cast<f32x4*>(p + 4 * q) = t;
+    }
+  } else if constexpr (VEC % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 2; ++q) {
+      f32x2 t = {v[2 * q], v[2 * q + 1]};
+      *reinterpret_cast<f32x2*>(p + 2 * q) = t;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) p[q] = v[q];
+  }
+}
+
+template <int VEC>
+__device__ __forceinline__ void store_vec(double* p, const double (&v)[VEC]) {
+  if constexpr (VEC % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < VEC / 2; ++q) {
+      f64x2 t = {v[2 * q], v[2 * q + 1]};
+      *reinterpret_cast<f64x2*>(p + 2 * q) = t;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < VEC; ++q) p[q] = v[q];
+  }
+}
+
+// Spread consecutive workgroup ids (which the dispatcher deals round-robin to
+// the 8 XCDs) so that each XCD receives a contiguous range of tiles and the
+// halo rows neighbouring tiles share stay in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  if ((nb & 7) != 0) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+}  // namespace sps

@@ -1,0 +1,80 @@
+// plan.hpp - the single-field plan behind sps_fd_plan*.
+#pragma once
+
+#include "common.hpp"
+#include "fused.hpp"
+
+namespace sps {
+void set_error(const char* fmt, ...);
+const char* last_error();
+int validate_desc(const sps_fd_desc* d);
+Lowered lower(const sps_fd_desc& d);
+template <typename T>
+void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c);
+void interior_to_padded(const sps_fd_desc& d, const char* src, char* dst,
+                        size_t es, bool fill_ghosts);
+void padded_to_interior(const sps_fd_desc& d, const char* src, char* dst,
+                        size_t es);
+
+// Launches replayed per captured graph (even, so buffer parity is kept).
+constexpr int kGraphLaunches = 16;
+
+struct GraphSlot {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+}  // namespace sps
+
+struct sps_fd_plan {
+  sps_fd_desc d{};
+  hipStream_t stream = nullptr;      // stream work is enqueued on
+  hipStream_t own_stream = nullptr;  // the plan's own stream
+  size_t nsites = 0, esize = 0;
+  void* buf[2] = {nullptr, nullptr};  // ping-pong fields (unpadded)
+  int cur = 0;                        // buf[cur] holds the current field
+  void* work = nullptr;               // dframe / derivative output
+  double* partials = nullptr;
+  double* sums = nullptr;
+  bool has_data = false;
+  int path = SPS_PATH_REFERENCE;
+  sps::Lowered low;
+  sps::RefParams<float> rp32;
+  sps::RefParams<double> rp64;
+  sps::FusedCoef<float> fc32;
+  sps::FusedCoef<double> fc64;
+  int K = 1;    // fused 2D: steps per launch
+  int TY = 32;  // fused 2D: rows per chunk; fused 3D: planes per chunk
+  sps::GraphSlot graphs[2];  // indexed by the buffer parity at capture
+
+  int launch_one(int k);
+  int steps_per_launch() const;
+  int step(int nsteps);
+  void drop_graphs();
+
+  void choose_default_shape() {
+    if (d.dim == 3) {
+      K = 1;
+      TY = 32;
+    } else {
+      K = d.dtype == SPS_F32 ? 2 : 1;
+      TY = 32;
+    }
+  }
+
+  int allocate() {
+    SPS_HIP_TRY(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+    stream = own_stream;
+    for (int i = 0; i < 2; ++i) {
+      hipError_t e = hipMalloc(&buf[i], nsites * esize);
+      if (e != hipSuccess) {
+        buf[i] = nullptr;
+        sps::set_error("hipMalloc of %zu bytes failed: %s", nsites * esize,
+                       hipGetErrorString(e));
+        return SPS_E_NOMEM;
+      }
+    }
+    SPS_HIP_TRY(hipMalloc(&partials, 2 * sps::kReduceBlocks * sizeof(double)));
+    SPS_HIP_TRY(hipMalloc(&sums, 2 * sizeof(double)));
+    return SPS_OK;
+  }
+};

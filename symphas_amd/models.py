@@ -1,0 +1,119 @@
+"""Problem descriptors for the SymPhas models on the solver path.
+
+Each builder returns the canonical term list that SolverFT::form_expr_one
+produces for the reference's EVOLUTION expression (examples/solvers/
+solverft.h:112-119): derivatives distributed over sums
+(sym/inc/expressionapply.h:1476-1488), one term per derivative, a derivative
+of a non-linear operand taking that operand as a polynomial in psi.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Sequence
+
+from . import abi
+
+
+@dataclass(frozen=True)
+class Stencil2d2h:
+    """Stencil2d2h<L, G, B> (sol/inc/stencilincludes.h:315-356).
+
+    The reference's default Stencil2d2h<> is <9, 6, 13> under the default
+    build option AVAILABLE_STENCILS_BASIC_ONLY (sol/inc/stencilincludes.h:
+    894-900, sol/CMakeLists.txt:73,100-106)."""
+    L: int = 9
+    G: int = 6
+    B: int = 13
+    dim: int = 2
+
+
+@dataclass(frozen=True)
+class Stencil3d2h:
+    """Stencil3d2h<L, G, B> (sol/inc/stencilincludes.h:368-409); default
+    <15, 10, 25> (:784-789, :913-917)."""
+    L: int = 15
+    G: int = 10
+    B: int = 25
+    dim: int = 3
+
+
+@dataclass(frozen=True)
+class TermSpec:
+    op: int
+    coeff: float
+    poly: Sequence[float]  # poly[k] multiplies psi^k
+    axis: int = 0
+    component: int = 0
+
+
+PSI = (0.0, 1.0)
+PSI3 = (0.0, 0.0, 0.0, 1.0)
+
+
+def make_desc(shape: Sequence[int], stencil, h: float, dt: float,
+              terms: Sequence[TermSpec], dtype: int = abi.F64,
+              path: int = abi.PATH_AUTO) -> abi.Desc:
+    """shape is the interior extent (nx, ny[, nz]), x fastest."""
+    if len(shape) != stencil.dim:
+        raise ValueError(f"shape {tuple(shape)} does not match a {stencil.dim}D stencil")
+    if len(terms) > abi.MAX_TERMS:
+        raise ValueError(f"at most {abi.MAX_TERMS} terms")
+    d = abi.Desc()
+    d.dim = stencil.dim
+    d.dtype = dtype
+    for a in range(3):
+        d.n[a] = int(shape[a]) if a < len(shape) else 1
+    d.lap_points, d.gradlap_points, d.bilap_points = stencil.L, stencil.G, stencil.B
+    d.nterms = len(terms)
+    d.path = path
+    d.h = float(h)
+    d.dt = float(dt)
+    for i, t in enumerate(terms):
+        if len(t.poly) > abi.MAX_DEGREE + 1:
+            raise ValueError("operand degree above 6")
+        tt = d.terms[i]
+        tt.op, tt.axis, tt.component = t.op, t.axis, t.component
+        tt.coeff = float(t.coeff)
+        for k, c in enumerate(t.poly):
+            tt.poly[k] = float(c)
+    return d
+
+
+def allen_cahn_terms(mobility: float = 1.0, kappa: float = 1.0):
+    """dpsi = M*(kappa*lap(psi) + psi - psi^3).
+
+    With M = kappa = 1 this is the README model
+    dop(1) = lap(op(1)) - (power(op(1), 2) - 1_n) * op(1) (README.md:83-85):
+    a laplacian term and a pointwise term."""
+    return [TermSpec(abi.OP_LAP, mobility * kappa, PSI),
+            TermSpec(abi.OP_ID, 1.0, (0.0, mobility, 0.0, -mobility))]
+
+
+def model_a_terms(c1: float = 1.0, c2: float = 1.0):
+    """MA: dpsi = lap(psi) + (c1 - c2*psi*psi)*psi
+    (examples/models/modeldefinitions.h:48)."""
+    return [TermSpec(abi.OP_LAP, 1.0, PSI),
+            TermSpec(abi.OP_ID, 1.0, (0.0, c1, 0.0, -c2))]
+
+
+def model_b_terms(c1: float = 1.0, c2: float = 1.0, mobility: float = 1.0,
+                  kappa: float = 1.0):
+    """MB: dpsi = -bilap(psi) - lap((c1 - c2*psi*psi)*psi)
+    (examples/models/modeldefinitions.h:53-55), i.e.
+    -bilap(psi) - c1*lap(psi) + c2*lap(psi^3) after distribution; with a
+    mobility M and gradient coefficient kappa,
+    dpsi = M*lap(c2*psi^3 - c1*psi - kappa*lap(psi))."""
+    return [TermSpec(abi.OP_BILAP, -mobility * kappa, PSI),
+            TermSpec(abi.OP_LAP, -mobility * c1, PSI),
+            TermSpec(abi.OP_LAP, mobility * c2, PSI3)]
+
+
+def allen_cahn(shape, h=0.5, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+               path=abi.PATH_AUTO, mobility=1.0, kappa=1.0) -> abi.Desc:
+    return make_desc(shape, stencil, h, dt, allen_cahn_terms(mobility, kappa), dtype, path)
+
+
+def model_b(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+            path=abi.PATH_AUTO, mobility=1.0, kappa=1.0, c1=1.0, c2=1.0) -> abi.Desc:
+    return make_desc(shape, stencil, h, dt, model_b_terms(c1, c2, mobility, kappa),
+                     dtype, path)

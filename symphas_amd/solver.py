@@ -1,0 +1,115 @@
+"""Host-side handle of the HIP finite-difference step (SolverFT semantics).
+
+`FDPlan` owns one periodic field on the device and advances it with the
+hand-written gfx950 kernels through the C ABI in include/sps_fd.h.  Arrays
+cross the boundary as numpy arrays of the plan's dtype, shaped (ny, nx) or
+(nz, ny, nx) for the interior layout and with 2*3 more cells per axis for the
+SymPhas padded layout (datatypes/inc/gridinfo.h:46).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_int, c_void_p
+
+import numpy as np
+
+from . import abi
+
+
+def _np_dtype(d: abi.Desc):
+    return np.float32 if d.dtype == abi.F32 else np.float64
+
+
+def interior_shape(d: abi.Desc):
+    return tuple(int(d.n[a]) for a in reversed(range(d.dim)))
+
+
+def padded_shape(d: abi.Desc):
+    return tuple(int(d.n[a]) + 2 * abi.BOUNDARY_DEPTH for a in reversed(range(d.dim)))
+
+
+class FDPlan:
+    def __init__(self, desc: abi.Desc):
+        self.lib = abi.load()
+        self.desc = desc
+        self.dtype = _np_dtype(desc)
+        h = c_void_p()
+        abi.check(self.lib.sps_fd_create(byref(desc), byref(h)))
+        self._h = h
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.sps_fd_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- data ---------------------------------------------------------------
+    def _shape(self, layout):
+        return interior_shape(self.desc) if layout == abi.LAYOUT_INTERIOR else padded_shape(self.desc)
+
+    def upload(self, field: np.ndarray, layout: int = abi.LAYOUT_INTERIOR):
+        arr = np.ascontiguousarray(field, dtype=self.dtype)
+        if arr.shape != self._shape(layout):
+            raise ValueError(f"field shape {arr.shape} != {self._shape(layout)}")
+        abi.check(self.lib.sps_fd_upload(self._h, arr.ctypes.data_as(c_void_p), layout))
+
+    def download(self, layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        out = np.empty(self._shape(layout), dtype=self.dtype)
+        abi.check(self.lib.sps_fd_download(self._h, out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def step(self, nsteps: int = 1):
+        abi.check(self.lib.sps_fd_step(self._h, int(nsteps)))
+
+    def equation(self, layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        out = np.empty(self._shape(layout), dtype=self.dtype)
+        abi.check(self.lib.sps_fd_equation(self._h, out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def apply(self, op: int, axis: int = 0, component: int = 0,
+              layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        out = np.empty(self._shape(layout), dtype=self.dtype)
+        abi.check(self.lib.sps_fd_apply(self._h, op, axis, component,
+                                        out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def reduce(self):
+        m, e = c_double(), c_double()
+        abi.check(self.lib.sps_fd_reduce(self._h, byref(m), byref(e)))
+        return m.value, e.value
+
+    def sync(self):
+        abi.check(self.lib.sps_fd_sync(self._h))
+
+    def set_stream(self, stream_handle: int | None):
+        """Enqueue later work on a caller-owned hipStream_t (None: own stream)."""
+        abi.check(self.lib.sps_fd_set_stream(self._h, c_void_p(stream_handle or 0)))
+
+    def device_ptr(self) -> int:
+        p = c_void_p()
+        abi.check(self.lib.sps_fd_device_field(self._h, byref(p)))
+        return p.value
+
+    def query(self):
+        path, spl = c_int(), c_int()
+        abi.check(self.lib.sps_fd_query(self._h, byref(path), byref(spl)))
+        return path.value, spl.value
+
+    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0):
+        abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile))
+
+
+def device_count() -> int:
+    return int(abi.load().sps_fd_device_count())
