@@ -113,7 +113,7 @@ def main():
     d32 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
     plan = FDPlan(d32)
     if args.steps_per_launch or args.rows_per_tile:
-        plan.tune(args.steps_per_launch, args.rows_per_tile)
+        plan.tune(args.steps_per_launch, args.rows_per_tile, 0)
     plan.set_stream(stream.cuda_stream)
     plan.upload(psi0)
     m0, e0 = plan.reduce()

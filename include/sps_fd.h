@@ -196,9 +196,14 @@ int sps_fd_device_field(sps_fd_plan* plan, void** dptr);
 /* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 
-/* Override the fused path's launch shape (0 keeps the current value):
- * steps_per_launch in 1..4, rows_per_tile a multiple of 4 in 4..512. */
-int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile);
+/* Override the fused path's launch shape (0 keeps the current value).
+ * 2D: steps_per_launch in 1..4, rows_per_tile = rows per streamed chunk, a
+ *     multiple of 4 in 4..512; planes_per_chunk must be 0.
+ * 3D: steps_per_launch must be 0 or 1; rows_per_tile = y rows per wave of the
+ *     register-streaming kernel (8 or 16; 4 selects the LDS plane-ring
+ *     kernel); planes_per_chunk = z planes per streamed chunk, 1..1024. */
+int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile,
+                int planes_per_chunk);
 
 /* ---------------------------------------------------------------- batched */
 
@@ -219,6 +224,12 @@ int sps_fd_batch_step(sps_fd_batch* batch, int nsteps);
 /* mass and energy are arrays of count doubles (either may be NULL). */
 int sps_fd_batch_reduce(sps_fd_batch* batch, double* mass, double* energy);
 int sps_fd_batch_sync(sps_fd_batch* batch);
+/* Launch shape of the batched kernel (0 keeps the current value):
+ * steps_per_launch in 1..4, rows_per_item = rows of one work item, 4..512. */
+int sps_fd_batch_tune(sps_fd_batch* batch, int steps_per_launch,
+                      int rows_per_item);
+/* Caller-owned hipStream_t for later work (NULL: the batch's own stream). */
+int sps_fd_batch_set_stream(sps_fd_batch* batch, void* hip_stream);
 
 #ifdef __cplusplus
 }

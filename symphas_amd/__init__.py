@@ -6,6 +6,6 @@ model descriptors, seeded inputs).  The C++ drop-in solver adapter lives in
 symphas_amd/host.
 """
 from . import abi, models, inputs  # noqa: F401
-from .solver import FDPlan, device_count  # noqa: F401
+from .solver import FDBatch, FDPlan, device_count  # noqa: F401
 
-__all__ = ["abi", "models", "inputs", "FDPlan", "device_count"]
+__all__ = ["abi", "models", "inputs", "FDPlan", "FDBatch", "device_count"]

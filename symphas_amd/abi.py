@@ -77,7 +77,7 @@ def load() -> ctypes.CDLL:
         "sps_fd_device_field": (c_int, [P, POINTER(P)]),
         "sps_fd_set_stream": (c_int, [P, P]),
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
-        "sps_fd_tune": (c_int, [P, c_int, c_int]),
+        "sps_fd_tune": (c_int, [P, c_int, c_int, c_int]),
         "sps_fd_batch_create": (c_int, [POINTER(Desc), c_int, POINTER(P)]),
         "sps_fd_batch_destroy": (c_int, [P]),
         "sps_fd_batch_upload": (c_int, [P, c_int, P, c_int]),
@@ -85,6 +85,8 @@ def load() -> ctypes.CDLL:
         "sps_fd_batch_step": (c_int, [P, c_int]),
         "sps_fd_batch_reduce": (c_int, [P, POINTER(c_double), POINTER(c_double)]),
         "sps_fd_batch_sync": (c_int, [P]),
+        "sps_fd_batch_tune": (c_int, [P, c_int, c_int]),
+        "sps_fd_batch_set_stream": (c_int, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -290,9 +290,9 @@ int sps_fd_plan::launch_one(int k) {
   } else {
     rc = d.dtype == SPS_F32
              ? launch_fused3d<float>((const float*)in, (float*)out, d.n[0],
-                                     d.n[1], d.n[2], TY, fc32, stream)
+                                     d.n[1], d.n[2], TY, TZ, fc32, stream)
              : launch_fused3d<double>((const double*)in, (double*)out, d.n[0],
-                                      d.n[1], d.n[2], TY, fc64, stream);
+                                      d.n[1], d.n[2], TY, TZ, fc64, stream);
   }
   if (rc == SPS_OK) cur = 1 - cur;
   return rc;
@@ -627,20 +627,34 @@ int sps_fd_query(const sps_fd_plan* p, int* path, int* spl) {
   return SPS_OK;
 }
 
-int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile) {
+int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile,
+                int planes_per_chunk) {
   if (!p) {
     set_error("plan is NULL");
     return SPS_E_INVALID;
   }
-  if (steps_per_launch < 0 || steps_per_launch > 4 ||
-      (rows_per_tile != 0 &&
-       (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4))) {
-    set_error("tune: steps_per_launch in 0..4, rows_per_tile 0 or a "
-              "multiple of 4 in 4..512");
-    return SPS_E_INVALID;
+  if (p->d.dim == 2) {
+    if (steps_per_launch < 0 || steps_per_launch > 4 || planes_per_chunk != 0 ||
+        (rows_per_tile != 0 &&
+         (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4))) {
+      set_error("tune (2D): steps_per_launch in 0..4, rows_per_tile 0 or a "
+                "multiple of 4 in 4..512, planes_per_chunk 0");
+      return SPS_E_INVALID;
+    }
+    if (steps_per_launch) p->K = steps_per_launch;
+    if (rows_per_tile) p->TY = rows_per_tile;
+  } else {
+    if (steps_per_launch < 0 || steps_per_launch > 1 ||
+        (rows_per_tile != 0 && rows_per_tile != 4 && rows_per_tile != 8 &&
+         rows_per_tile != 16) ||
+        planes_per_chunk < 0 || planes_per_chunk > 1024) {
+      set_error("tune (3D): steps_per_launch 0 or 1, rows_per_tile 0, 4, 8 "
+                "or 16, planes_per_chunk in 0..1024");
+      return SPS_E_INVALID;
+    }
+    if (rows_per_tile) p->TY = rows_per_tile;
+    if (planes_per_chunk) p->TZ = planes_per_chunk;
   }
-  if (steps_per_launch) p->K = steps_per_launch;
-  if (rows_per_tile) p->TY = rows_per_tile;
   p->drop_graphs();
   return SPS_OK;
 }

@@ -1,34 +1,502 @@
-// batch.hip - batched API entry points (filled in by the persistent kernel).
+// batch.hip - many independent 2D fields in one persistent launch per step
+// group (the batched counterpart of find_solution over an array of models,
+// modules-io.h:368-474, which steps the models one after another).
+//
+// Every instance is stored unpadded in one device pool with two buffers
+// (ping-pong shared by all instances).  A work item is a chunk of rows of one
+// instance; items are sorted largest first and handed out through a device
+// counter, so waves that draw small instances take more items.  Each wave
+// holds whole rows (width 64*v, v in {2,3,4,6,8}): the x wrap is the DPP
+// rotate and every lane stores.  Per-instance coefficients (mobility,
+// gradient coefficient, dt, h, model) live in a descriptor table read with
+// scalar loads.  The counter of launch parity p is used by that launch, which
+// clears the other parity's counter for the next launch; kernels on one stream
+// run in order, so the cleared counter is never in use.  Captured once in a
+// hipGraph per parity and replayed.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
 #include "common.hpp"
+#include "fused_tile.hpp"
 #include "plan.hpp"
+
+namespace sps {
+namespace {
+
+struct BatchInst {
+  float* buf[2];
+  int X, Y, vec, kind;  // kind: 0 radius 1 with A, 1 radius 2 without A,
+                        //       2 radius 2 with A, 3 radius 1 without A
+  double h, divh;
+  FusedCoef<float> c;
+};
+
+struct BatchItem {
+  int inst, y0, ty, pad;
+};
+
+template <int VEC, int K>
+__device__ __forceinline__ void run_item(const BatchInst& I, int parity,
+                                         int y0, int ty, int lane) {
+  const float* in = I.buf[parity];
+  float* out = I.buf[1 - parity];
+  const FusedCoef<float> c = I.c;
+  switch (I.kind) {
+    case 0:
+      stream_tile<float, VEC, K, 1, true, 3, true>(in, out, 64 * VEC, I.Y, y0,
+                                                  ty, lane * VEC, true, c);
+      break;
+    case 1:
+      stream_tile<float, VEC, K, 2, false, 3, true>(in, out, 64 * VEC, I.Y,
+                                                   y0, ty, lane * VEC, true, c);
+      break;
+    case 2:
+      stream_tile<float, VEC, K, 2, true, 3, true>(in, out, 64 * VEC, I.Y, y0,
+                                                  ty, lane * VEC, true, c);
+      break;
+    default:
+      stream_tile<float, VEC, K, 1, false, 3, true>(in, out, 64 * VEC, I.Y,
+                                                   y0, ty, lane * VEC, true, c);
+      break;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void batch_kernel(
+    const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items,
+    int nitems, int parity, unsigned* __restrict__ counters) {
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[1 - parity] = 0u;
+  unsigned* ctr = counters + parity;
+  while (true) {
+    unsigned it = 0;
+    if (lane == 0) it = atomicAdd(ctr, 1u);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
+    if (it >= (unsigned)nitems) break;  // every wave reaches this
+    const BatchItem item = items[it];
+    const BatchInst& I = insts[item.inst];
+    switch (I.vec) {
+      case 2: run_item<2, K>(I, parity, item.y0, item.ty, lane); break;
+      case 3: run_item<3, K>(I, parity, item.y0, item.ty, lane); break;
+      case 4: run_item<4, K>(I, parity, item.y0, item.ty, lane); break;
+      case 6: run_item<6, K>(I, parity, item.y0, item.ty, lane); break;
+      default: run_item<8, K>(I, parity, item.y0, item.ty, lane); break;
+    }
+  }
+}
+
+// FP64 sums per instance: one workgroup per instance.
+__global__ __launch_bounds__(256) void batch_reduce(
+    const BatchInst* __restrict__ insts, int parity, double* __restrict__ out) {
+  const BatchInst& I = insts[blockIdx.x];
+  const float* f = I.buf[parity];
+  const int X = I.X, Y = I.Y;
+  const double divh = I.divh;
+  double m = 0.0, e = 0.0;
+  for (int y = 0; y < Y; ++y) {
+    const float* row = f + (int64_t)y * X;
+    const float* rowy = f + (int64_t)(y + 1 == Y ? 0 : y + 1) * X;
+    for (int x = threadIdx.x; x < X; x += 256) {
+      const double p = row[x];
+      const double p2 = p * p;
+      const double gx = ((double)row[x + 1 == X ? 0 : x + 1] - p) * divh;
+      const double gy = ((double)rowy[x] - p) * divh;
+      m += p;
+      e += -0.5 * p2 + 0.25 * (p2 * p2) + 0.5 * (gx * gx + gy * gy);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    m += __shfl_xor(m, o, 64);
+    e += __shfl_xor(e, o, 64);
+  }
+  __shared__ double sh[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = m;
+    sh[1][threadIdx.x >> 6] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
+    out[2 * blockIdx.x + 1] = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) *
+                              (I.h * I.h);
+  }
+}
+
+int kind_of(const FusedCoef<float>& c) {
+  const bool hasa = c.dega >= 0;
+  if (c.radius == 1) return hasa ? 0 : 3;
+  return hasa ? 2 : 1;
+}
+
+}  // namespace
+}  // namespace sps
+
+using namespace sps;
+
+struct sps_fd_batch {
+  int count = 0;
+  std::vector<sps_fd_desc> descs;
+  std::vector<size_t> offset;  // element offset of each instance in a buffer
+  size_t total = 0;            // elements per buffer
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  float* pool = nullptr;  // 2 * total floats
+  BatchInst* d_insts = nullptr;
+  BatchItem* d_items = nullptr;
+  unsigned* d_counters = nullptr;
+  double* d_red = nullptr;
+  int nitems = 0;
+  int parity = 0;
+  int K = 4, TY = 32;  // best of a 1..4 x 16..128 sweep on 256 instances
+  int grid[4] = {1, 1, 1, 1};  // persistent grid per steps-per-launch
+  bool has_data = false;
+  GraphSlot graphs[2];
+  std::vector<BatchInst> h_insts;
+
+  void drop_graphs() {
+    for (auto& g : graphs) {
+      if (g.exec) hipGraphExecDestroy(g.exec);
+      if (g.graph) hipGraphDestroy(g.graph);
+      g = GraphSlot{};
+    }
+  }
+
+  int build_items() {
+    std::vector<BatchItem> items;
+    for (int i = 0; i < count; ++i) {
+      const int Y = descs[i].n[1];
+      for (int y0 = 0; y0 < Y; y0 += TY)
+        items.push_back(BatchItem{i, y0, std::min(TY, Y - y0), 0});
+    }
+    // largest first: cost ~ row width x streamed rows
+    std::stable_sort(items.begin(), items.end(),
+                     [&](const BatchItem& a, const BatchItem& b) {
+                       const double ca = (double)descs[a.inst].n[0] * a.ty;
+                       const double cb = (double)descs[b.inst].n[0] * b.ty;
+                       return ca > cb;
+                     });
+    if (d_items) hipFree(d_items);
+    d_items = nullptr;
+    nitems = (int)items.size();
+    SPS_HIP_TRY(hipMalloc(&d_items, sizeof(BatchItem) * items.size()));
+    SPS_HIP_TRY(hipMemcpy(d_items, items.data(), sizeof(BatchItem) * items.size(),
+                          hipMemcpyHostToDevice));
+    return SPS_OK;
+  }
+
+  int launch_one(int k) {
+    switch (k) {
+      case 1: batch_kernel<1><<<grid[0], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
+      case 2: batch_kernel<2><<<grid[1], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
+      case 3: batch_kernel<3><<<grid[2], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
+      default: batch_kernel<4><<<grid[3], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error("batch launch failed: %s", hipGetErrorString(e));
+      return SPS_E_HIP;
+    }
+    parity = 1 - parity;
+    return SPS_OK;
+  }
+
+  int step(int nsteps) {
+    int launches = nsteps / K;
+    const int rem = nsteps % K;
+    if (launches >= 2 * kGraphLaunches) {
+      GraphSlot& g = graphs[parity];
+      if (!g.exec) {
+        const int start = parity;
+        SPS_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+        int rc = SPS_OK;
+        for (int i = 0; i < kGraphLaunches && rc == SPS_OK; ++i) rc = launch_one(K);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &graph);
+        parity = start;
+        if (rc != SPS_OK) {
+          if (graph) hipGraphDestroy(graph);
+          return rc;
+        }
+        if (e != hipSuccess) {
+          set_error("batch graph capture failed: %s", hipGetErrorString(e));
+          return SPS_E_HIP;
+        }
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          hipGraphDestroy(graph);
+          set_error("batch graph instantiate failed: %s", hipGetErrorString(e));
+          return SPS_E_HIP;
+        }
+        g.graph = graph;
+        g.exec = exec;
+      }
+      while (launches >= kGraphLaunches) {
+        SPS_HIP_TRY(hipGraphLaunch(g.exec, stream));
+        launches -= kGraphLaunches;
+      }
+    }
+    for (int i = 0; i < launches; ++i) SPS_TRY(launch_one(K));
+    if (rem) SPS_TRY(launch_one(rem));
+    return SPS_OK;
+  }
+};
+
+static int batch_width_ok(int n) {
+  return n == 128 || n == 192 || n == 256 || n == 384 || n == 512;
+}
 
 extern "C" {
 
-int sps_fd_batch_create(const sps_fd_desc*, int, sps_fd_batch** out) {
-  if (out) *out = nullptr;
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+int sps_fd_batch_create(const sps_fd_desc* descs, int count,
+                        sps_fd_batch** out) {
+  if (!out || !descs || count < 1) {
+    set_error("batch: descriptors, count >= 1 and out are required");
+    return SPS_E_INVALID;
+  }
+  *out = nullptr;
+  for (int i = 0; i < count; ++i) {
+    int rc = validate_desc(&descs[i]);
+    if (rc != SPS_OK) return rc;
+    const sps_fd_desc& d = descs[i];
+    if (d.dim != 2 || d.dtype != SPS_F32) {
+      set_error("batch instance %d: the batched kernel advances 2D FP32 fields",
+                i);
+      return SPS_E_UNSUPPORTED;
+    }
+    if (!batch_width_ok(d.n[0])) {
+      set_error("batch instance %d: x extent %d not in {128,192,256,384,512}",
+                i, d.n[0]);
+      return SPS_E_UNSUPPORTED;
+    }
+    Lowered lw = lower(d);
+    if (!lw.fusable) {
+      set_error("batch instance %d: %s", i, lw.why_not);
+      return SPS_E_UNSUPPORTED;
+    }
+    for (int k = 4; k <= kMaxDeg; ++k)
+      if (lw.a[k] != 0.0 || lw.b[k] != 0.0) {
+        set_error("batch instance %d: polynomial degree above 3", i);
+        return SPS_E_UNSUPPORTED;
+      }
+  }
+  if (sps_fd_device_count() < 1) {
+    set_error("no HIP device: this library runs the step on an MI355X only");
+    return SPS_E_HIP;
+  }
+  sps_fd_batch* b = new (std::nothrow) sps_fd_batch();
+  if (!b) {
+    set_error("host allocation failed");
+    return SPS_E_NOMEM;
+  }
+  b->count = count;
+  b->descs.assign(descs, descs + count);
+  size_t off = 0;
+  for (int i = 0; i < count; ++i) {
+    b->offset.push_back(off);
+    const size_t n = (size_t)descs[i].n[0] * descs[i].n[1];
+    off += (n + 63) & ~(size_t)63;  // 256-byte aligned instances
+  }
+  b->total = off;
+  auto fail = [&](int rc) {
+    sps_fd_batch_destroy(b);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    set_error("stream creation failed");
+    return fail(SPS_E_HIP);
+  }
+  b->stream = b->own_stream;
+  if (hipMalloc(&b->pool, 2 * b->total * sizeof(float)) != hipSuccess) {
+    set_error("hipMalloc of the batch pool (%zu bytes) failed",
+              2 * b->total * sizeof(float));
+    return fail(SPS_E_NOMEM);
+  }
+  b->h_insts.resize(count);
+  for (int i = 0; i < count; ++i) {
+    BatchInst& I = b->h_insts[i];
+    I.buf[0] = b->pool + b->offset[i];
+    I.buf[1] = b->pool + b->total + b->offset[i];
+    I.X = descs[i].n[0];
+    I.Y = descs[i].n[1];
+    I.vec = I.X / 64;
+    I.h = descs[i].h;
+    I.divh = 1.0 / descs[i].h;
+    make_fused_coef(descs[i], lower(descs[i]), I.c);
+    I.kind = kind_of(I.c);
+  }
+  if (hipMalloc(&b->d_insts, sizeof(BatchInst) * count) != hipSuccess ||
+      hipMemcpy(b->d_insts, b->h_insts.data(), sizeof(BatchInst) * count,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) != hipSuccess ||
+      hipMemset(b->d_counters, 0, 2 * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&b->d_red, 2 * sizeof(double) * count) != hipSuccess) {
+    set_error("batch device set-up failed");
+    return fail(SPS_E_HIP);
+  }
+  int rc = b->build_items();
+  if (rc != SPS_OK) return fail(rc);
+  // persistent grid: as many workgroups as fit on the device at once
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess) {
+    set_error("device query failed");
+    return fail(SPS_E_HIP);
+  }
+  const void* kernels[4] = {(const void*)batch_kernel<1>, (const void*)batch_kernel<2>,
+                            (const void*)batch_kernel<3>, (const void*)batch_kernel<4>};
+  for (int k = 0; k < 4; ++k) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[k], 256,
+                                                     0) != hipSuccess)
+      per_cu = 1;
+    b->grid[k] = std::max(1, cus * std::max(1, per_cu));
+  }
+  *out = b;
+  return SPS_OK;
 }
-int sps_fd_batch_destroy(sps_fd_batch*) { return SPS_OK; }
-int sps_fd_batch_upload(sps_fd_batch*, int, const void*, int) {
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+
+int sps_fd_batch_destroy(sps_fd_batch* b) {
+  if (!b) return SPS_OK;
+  b->drop_graphs();
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->pool) hipFree(b->pool);
+  if (b->d_insts) hipFree(b->d_insts);
+  if (b->d_items) hipFree(b->d_items);
+  if (b->d_counters) hipFree(b->d_counters);
+  if (b->d_red) hipFree(b->d_red);
+  if (b->own_stream) hipStreamDestroy(b->own_stream);
+  delete b;
+  return SPS_OK;
 }
-int sps_fd_batch_download(sps_fd_batch*, int, void*, int) {
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+
+static int batch_index_ok(sps_fd_batch* b, int index, const void* host) {
+  if (!b || !host) {
+    set_error("batch or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (index < 0 || index >= b->count) {
+    set_error("batch index %d out of range [0, %d)", index, b->count);
+    return SPS_E_INVALID;
+  }
+  return SPS_OK;
 }
-int sps_fd_batch_step(sps_fd_batch*, int) {
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+
+int sps_fd_batch_upload(sps_fd_batch* b, int index, const void* host,
+                        int layout) {
+  SPS_TRY(batch_index_ok(b, index, host));
+  const sps_fd_desc& d = b->descs[index];
+  const size_t n = (size_t)d.n[0] * d.n[1];
+  std::vector<float> tmp;
+  const void* src = host;
+  if (layout == SPS_LAYOUT_PADDED) {
+    tmp.resize(n);
+    padded_to_interior(d, (const char*)host, (char*)tmp.data(), sizeof(float));
+    src = tmp.data();
+  } else if (layout != SPS_LAYOUT_INTERIOR) {
+    set_error("unknown layout %d", layout);
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipMemcpyAsync(b->h_insts[index].buf[b->parity], src,
+                             n * sizeof(float), hipMemcpyHostToDevice,
+                             b->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->has_data = true;
+  return SPS_OK;
 }
-int sps_fd_batch_reduce(sps_fd_batch*, double*, double*) {
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+
+int sps_fd_batch_download(sps_fd_batch* b, int index, void* host, int layout) {
+  SPS_TRY(batch_index_ok(b, index, host));
+  const sps_fd_desc& d = b->descs[index];
+  const size_t n = (size_t)d.n[0] * d.n[1];
+  std::vector<float> tmp(n);
+  SPS_HIP_TRY(hipMemcpyAsync(tmp.data(), b->h_insts[index].buf[b->parity],
+                             n * sizeof(float), hipMemcpyDeviceToHost,
+                             b->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (layout == SPS_LAYOUT_INTERIOR) {
+    std::memcpy(host, tmp.data(), n * sizeof(float));
+  } else if (layout == SPS_LAYOUT_PADDED) {
+    interior_to_padded(d, (const char*)tmp.data(), (char*)host, sizeof(float),
+                       true);
+  } else {
+    set_error("unknown layout %d", layout);
+    return SPS_E_INVALID;
+  }
+  return SPS_OK;
 }
-int sps_fd_batch_sync(sps_fd_batch*) {
-  sps::set_error("batched stepping not built yet");
-  return SPS_E_UNSUPPORTED;
+
+int sps_fd_batch_step(sps_fd_batch* b, int nsteps) {
+  if (!b || nsteps < 0) {
+    set_error("batch is NULL or nsteps < 0");
+    return SPS_E_INVALID;
+  }
+  if (!b->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  return b->step(nsteps);
+}
+
+int sps_fd_batch_reduce(sps_fd_batch* b, double* mass, double* energy) {
+  if (!b) {
+    set_error("batch is NULL");
+    return SPS_E_INVALID;
+  }
+  batch_reduce<<<b->count, 256, 0, b->stream>>>(b->d_insts, b->parity,
+                                                 b->d_red);
+  SPS_HIP_TRY(hipGetLastError());
+  std::vector<double> h(2 * (size_t)b->count);
+  SPS_HIP_TRY(hipMemcpyAsync(h.data(), b->d_red, h.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int i = 0; i < b->count; ++i) {
+    if (mass) mass[i] = h[2 * i];
+    if (energy) energy[i] = h[2 * i + 1];
+  }
+  return SPS_OK;
+}
+
+int sps_fd_batch_sync(sps_fd_batch* b) {
+  if (!b) {
+    set_error("batch is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  return SPS_OK;
+}
+
+int sps_fd_batch_tune(sps_fd_batch* b, int steps_per_launch,
+                      int rows_per_item) {
+  if (!b || steps_per_launch < 0 || steps_per_launch > 4 ||
+      (rows_per_item != 0 && (rows_per_item < 4 || rows_per_item > 512))) {
+    set_error("batch tune: steps_per_launch in 0..4, rows_per_item 0 or "
+              "4..512");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (steps_per_launch) b->K = steps_per_launch;
+  if (rows_per_item && rows_per_item != b->TY) {
+    b->TY = rows_per_item;
+    SPS_TRY(b->build_items());
+  }
+  b->drop_graphs();
+  return SPS_OK;
+}
+
+int sps_fd_batch_set_stream(sps_fd_batch* b, void* hip_stream) {
+  if (!b) {
+    set_error("batch is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->stream = hip_stream ? (hipStream_t)hip_stream : b->own_stream;
+  return SPS_OK;
 }
 
 }  // extern "C"

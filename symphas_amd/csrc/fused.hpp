@@ -11,9 +11,11 @@ template <typename T>
 int launch_fused2d(const T* in, T* out, int X, int Y, int K, int TY,
                    const FusedCoef<T>& c, hipStream_t s);
 
-// 3D: one step per launch; TZ = planes per streamed chunk.
+// 3D: one step per launch; TZ = planes per streamed chunk; rows = y rows per
+// wave of the register-streaming kernel (8 or 16), any other value selects
+// the LDS plane-ring kernel.
 template <typename T>
-int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int TZ,
+int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int rows, int TZ,
                    const FusedCoef<T>& c, hipStream_t s);
 
 // FP64 mass and free energy of an unpadded periodic field.  partials holds

@@ -11,6 +11,8 @@
 #include "fused.hpp"
 #include "lanes.hpp"
 
+#include <type_traits>
+
 namespace sps {
 namespace {
 
@@ -131,6 +133,139 @@ __global__ __launch_bounds__(256) void fused3d_kernel(
   }
 }
 
+// v2: one wave owns a 64-lane x TYW-row column block and streams the planes
+// of one z-chunk through register windows; x neighbours come from adjacent
+// lanes (DPP), y and z neighbours are registers.  No LDS, no barriers.
+template <typename T, int TYW, int R, bool HASA, int POLY>
+__global__ __launch_bounds__(256) void fused3d_stream(
+    const T* __restrict__ in, T* __restrict__ out, int X, int Y, int Z, int TZ,
+    int nstrips, int nrowblk, int nitems, FusedCoef<T> c) {
+  constexpr int HL = R;  // halo lanes (one column per lane)
+  constexpr int W = 64 - 2 * HL;
+  constexpr int NP = TYW + 2 * R;  // psi rows held per plane
+  constexpr int NM = TYW + 2;      // mu rows per plane (radius 2 only)
+  const int lane = threadIdx.x & 63;
+  const int item = __builtin_amdgcn_readfirstlane(
+      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (item >= nitems) return;
+  const int strip = item % nstrips;
+  const int rb = (item / nstrips) % nrowblk;
+  const int zc = item / (nstrips * nrowblk);
+  const int x = strip * W + lane - HL;
+  int col = x % X;
+  if (col < 0) col += X;
+  const bool store_lane = lane >= HL && lane < 64 - HL && x < X;
+  const int y0 = rb * TYW;
+  const int z0 = zc * TZ;
+  const int tz = min(TZ, Z - z0);
+  const int64_t XY = (int64_t)X * Y;
+  // wave-uniform row offsets (scalar registers); the lane's column is added
+  // in the address only
+  int rowoff[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) {
+    int y = (y0 - R + r) % Y;
+    if (y < 0) y += Y;
+    rowoff[r] = __builtin_amdgcn_readfirstlane(y * X);
+  }
+  const int NI = tz + 2 * R;
+  int zl = (z0 - R) % Z;
+  if (zl < 0) zl += Z;
+
+  T P[3][NP];
+  T M[3][R == 2 ? NM : 1];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+#pragma unroll
+    for (int r = 0; r < NP; ++r) P[s][r] = T(0);
+#pragma unroll
+    for (int r = 0; r < (R == 2 ? NM : 1); ++r) M[s][r] = T(0);
+  }
+
+  // the next plane is loaded one phase ahead into NX, so each wave's loads
+  // overlap the arithmetic of the plane before
+  T NX[NP];
+  auto fetch = [&]() {
+    const T* src = in + (int64_t)zl * XY;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) NX[r] = (src + rowoff[r])[col];
+    zl = zl + 1 == Z ? 0 : zl + 1;
+  };
+  fetch();
+  auto phase = [&](auto PHc, int i) {
+    constexpr int PH = decltype(PHc)::value;
+    constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;
+    {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) P[S2][r] = NX[r];
+      if (i + 1 < NI) fetch();
+    }
+    if constexpr (R == 2) {
+      // mu of plane i-1, rows y0-1 .. y0+TYW (psi rows 1 .. TYW+2)
+#pragma unroll
+      for (int r = 1; r <= TYW + 2; ++r) {
+        const T ctr = P[S1][r];
+        const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+        const T d = fmad(T(-6), ctr,
+                         ((xl + xr) + (P[S1][r - 1] + P[S1][r + 1])) +
+                             (P[S0][r] + P[S2][r]));
+        M[S2][r - 1] = fmad(c.kmu, d, horner<POLY>(c.b, ctr));
+      }
+      if (i >= 4) {
+        const int zo = z0 + i - 4;
+        T* dst = out + (int64_t)zo * XY;
+#pragma unroll
+        for (int m = 1; m <= TYW; ++m) {
+          const T ctr = M[S1][m];
+          const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+          const T d = fmad(T(-6), ctr,
+                           ((xl + xr) + (M[S1][m - 1] + M[S1][m + 1])) +
+                               (M[S0][m] + M[S2][m]));
+          T base = P[S0][m + 1];
+          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
+          const T v = fmad(c.kd, d, base);
+          if (store_lane && y0 + m - 1 < Y) (dst + rowoff[m + 1])[col] = v;
+        }
+      }
+    } else {
+      if (i >= 2) {
+        const int zo = z0 + i - 2;
+        T* dst = out + (int64_t)zo * XY;
+#pragma unroll
+        for (int r = 1; r <= TYW; ++r) {
+          const T ctr = P[S1][r];
+          const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+          const T d = fmad(T(-6), ctr,
+                           ((xl + xr) + (P[S1][r - 1] + P[S1][r + 1])) +
+                               (P[S0][r] + P[S2][r]));
+          T base = ctr;
+          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
+          const T v = fmad(c.k1, d, base);
+          if (store_lane && y0 + r - 1 < Y) (dst + rowoff[r])[col] = v;
+        }
+      }
+    }
+  };
+  for (int i = 0; i < NI; i += 3) {
+    phase(std::integral_constant<int, 0>{}, i);
+    if (i + 1 < NI) phase(std::integral_constant<int, 1>{}, i + 1);
+    if (i + 2 < NI) phase(std::integral_constant<int, 2>{}, i + 2);
+  }
+}
+
+template <typename T, int TYW, int R, bool HASA, int POLY>
+int launch3s(const T* in, T* out, int X, int Y, int Z, int TZ,
+             const FusedCoef<T>& c, hipStream_t s) {
+  constexpr int W = 64 - 2 * R;
+  const int nstrips = (X + W - 1) / W;
+  const int nrowblk = (Y + TYW - 1) / TYW;
+  const int nzc = (Z + TZ - 1) / TZ;
+  const int nitems = nstrips * nrowblk * nzc;
+  fused3d_stream<T, TYW, R, HASA, POLY><<<(nitems + 3) / 4, 256, 0, s>>>(
+      in, out, X, Y, Z, TZ, nstrips, nrowblk, nitems, c);
+  return SPS_OK;
+}
+
 template <typename T, int TY, int R, bool HASA, int POLY>
 int launch3(const T* in, T* out, int X, int Y, int Z, int TZ,
             const FusedCoef<T>& c, hipStream_t s) {
@@ -142,30 +277,41 @@ int launch3(const T* in, T* out, int X, int Y, int Z, int TZ,
   return SPS_OK;
 }
 
+template <typename T, int TY, int R, bool HASA, int POLY>
+int launch_variant(int variant, const T* in, T* out, int X, int Y, int Z,
+                   int TZ, const FusedCoef<T>& c, hipStream_t s) {
+  switch (variant) {
+    case 8: return launch3s<T, 8, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+    case 16: return launch3s<T, 16, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+    default: return launch3<T, TY, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+  }
+}
+
 template <typename T, int TY, int R>
-int dispatch3(const T* in, T* out, int X, int Y, int Z, int TZ,
+int dispatch3(int variant, const T* in, T* out, int X, int Y, int Z, int TZ,
               const FusedCoef<T>& c, hipStream_t s) {
   const bool hasa = c.dega >= 0;
   const int deg = c.dega > c.degb ? c.dega : c.degb;
   if (deg <= 3)
-    return hasa ? launch3<T, TY, R, true, 3>(in, out, X, Y, Z, TZ, c, s)
-                : launch3<T, TY, R, false, 3>(in, out, X, Y, Z, TZ, c, s);
-  return hasa ? launch3<T, TY, R, true, 6>(in, out, X, Y, Z, TZ, c, s)
-              : launch3<T, TY, R, false, 6>(in, out, X, Y, Z, TZ, c, s);
+    return hasa ? launch_variant<T, TY, R, true, 3>(variant, in, out, X, Y, Z, TZ, c, s)
+                : launch_variant<T, TY, R, false, 3>(variant, in, out, X, Y, Z, TZ, c, s);
+  return hasa ? launch_variant<T, TY, R, true, 6>(variant, in, out, X, Y, Z, TZ, c, s)
+              : launch_variant<T, TY, R, false, 6>(variant, in, out, X, Y, Z, TZ, c, s);
 }
 
 }  // namespace
 
 template <typename T>
-int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int TZ,
+int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int rows, int TZ,
                    const FusedCoef<T>& c, hipStream_t s) {
   if (TZ < 1) {
     set_error("fused 3D step: planes per chunk must be >= 1");
     return SPS_E_INVALID;
   }
   constexpr int TY = sizeof(T) == 4 ? 16 : 8;
-  int rc = c.radius == 1 ? dispatch3<T, TY, 1>(in, out, X, Y, Z, TZ, c, s)
-                         : dispatch3<T, TY, 2>(in, out, X, Y, Z, TZ, c, s);
+  int rc = c.radius == 1
+               ? dispatch3<T, TY, 1>(rows, in, out, X, Y, Z, TZ, c, s)
+               : dispatch3<T, TY, 2>(rows, in, out, X, Y, Z, TZ, c, s);
   if (rc != SPS_OK) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -176,8 +322,8 @@ int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int TZ,
 }
 
 template int launch_fused3d<float>(const float*, float*, int, int, int, int,
-                                   const FusedCoef<float>&, hipStream_t);
+                                   int, const FusedCoef<float>&, hipStream_t);
 template int launch_fused3d<double>(const double*, double*, int, int, int, int,
-                                    const FusedCoef<double>&, hipStream_t);
+                                    int, const FusedCoef<double>&, hipStream_t);
 
 }  // namespace sps

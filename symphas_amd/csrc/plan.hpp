@@ -43,7 +43,8 @@ struct sps_fd_plan {
   sps::FusedCoef<float> fc32;
   sps::FusedCoef<double> fc64;
   int K = 1;    // fused 2D: steps per launch
-  int TY = 32;  // fused 2D: rows per chunk; fused 3D: planes per chunk
+  int TY = 32;  // fused 2D: rows per chunk; fused 3D: y rows per wave
+  int TZ = 64;  // fused 3D: planes per chunk
   sps::GraphSlot graphs[2];  // indexed by the buffer parity at capture
 
   int launch_one(int k);
@@ -51,13 +52,20 @@ struct sps_fd_plan {
   int step(int nsteps);
   void drop_graphs();
 
+  // Launch shape before any tuning: the best of a sweep of steps per launch
+  // {1..4} x rows per chunk {16..256} on Model B 4096^2 (FP32: 3 x 32,
+  // FP64: 4 x 64) and planes per chunk {8..128} on 512^3.
   void choose_default_shape() {
     if (d.dim == 3) {
       K = 1;
+      TY = 8;
+      TZ = 64;
+    } else if (d.dtype == SPS_F32) {
+      K = 3;
       TY = 32;
     } else {
-      K = d.dtype == SPS_F32 ? 2 : 1;
-      TY = 32;
+      K = 4;
+      TY = 64;
     }
   }
 

@@ -9,6 +9,8 @@ of a non-linear operand taking that operand as a polynomial in psi.
 from __future__ import annotations
 
 from dataclasses import dataclass
+
+import numpy as np
 from typing import Sequence
 
 from . import abi
@@ -117,3 +119,44 @@ def model_b(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
             path=abi.PATH_AUTO, mobility=1.0, kappa=1.0, c1=1.0, c2=1.0) -> abi.Desc:
     return make_desc(shape, stencil, h, dt, model_b_terms(c1, c2, mobility, kappa),
                      dtype, path)
+
+
+def batched_instances(count: int = 256, seed: int = 7,
+                      sides=(128, 192, 256, 384, 512), h: float = 1.0,
+                      dt: float = 0.01, coeff_range=(0.5, 2.0), dtype=abi.F32,
+                      stability_margin: float = 0.95):
+    """The batched workload: count independent periodic 2D instances, the
+    first half Allen-Cahn dpsi = M*(kappa*lap psi + psi - psi^3), the second
+    half Model B dpsi = M*lap(psi^3 - psi - kappa*lap psi), square sides drawn
+    uniformly from `sides`, M and kappa uniform in coeff_range.
+
+    Explicit Euler on -M*kappa*bilap(psi) with the 13-point bilaplacian is
+    stable only while dt*M*kappa*64/h^4 <= 2 (the largest eigenvalue of
+    lap5∘lap5 is 64/h^4), so Model B draws are redrawn until M*kappa is
+    within stability_margin of that bound; with dt = 0.01, h = 1 and
+    coefficients in [0.5, 2] about 5% of raw Model B draws exceed it.
+
+    Returns a list of (descriptor, field_seed) pairs; draws come from the
+    SplitMix64 stream of `seed`, so the workload is reproducible."""
+    from . import inputs
+    stream = iter(((inputs.splitmix64(seed, 64 * count) >> 11).astype(np.float64)
+                   * (1.0 / 9007199254740992.0)).tolist())
+    lo, hi = coeff_range
+    bound = stability_margin * 2.0 * h ** 4 / (64.0 * dt)
+    out = []
+    for i in range(count):
+        side = sides[min(int(next(stream) * len(sides)), len(sides) - 1)]
+        model_b_inst = i >= count // 2
+        while True:
+            m = lo + (hi - lo) * next(stream)
+            k = lo + (hi - lo) * next(stream)
+            if not model_b_inst or m * k <= bound:
+                break
+        if model_b_inst:
+            d = model_b((side, side), h=h, dt=dt, dtype=dtype, path=abi.PATH_FUSED,
+                        mobility=m, kappa=k)
+        else:
+            d = allen_cahn((side, side), h=h, dt=dt, dtype=dtype, path=abi.PATH_FUSED,
+                           mobility=m, kappa=k, stencil=Stencil2d2h(5, 6, 13))
+        out.append((d, seed * 1000003 + i))
+    return out

@@ -107,9 +107,73 @@ class FDPlan:
         abi.check(self.lib.sps_fd_query(self._h, byref(path), byref(spl)))
         return path.value, spl.value
 
-    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0):
-        abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile))
+    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0):
+        abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk))
 
 
 def device_count() -> int:
     return int(abi.load().sps_fd_device_count())
+
+
+class FDBatch:
+    """Many independent 2D FP32 fields advanced together by one persistent
+    launch per step group (sps_fd_batch_* in include/sps_fd.h)."""
+
+    def __init__(self, descs):
+        self.lib = abi.load()
+        self.descs = list(descs)
+        arr = (abi.Desc * len(self.descs))(*self.descs)
+        h = c_void_p()
+        abi.check(self.lib.sps_fd_batch_create(arr, len(self.descs), byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.sps_fd_batch_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, index: int, field: np.ndarray, layout: int = abi.LAYOUT_INTERIOR):
+        d = self.descs[index]
+        shape = interior_shape(d) if layout == abi.LAYOUT_INTERIOR else padded_shape(d)
+        arr = np.ascontiguousarray(field, dtype=np.float32)
+        if arr.shape != shape:
+            raise ValueError(f"field shape {arr.shape} != {shape}")
+        abi.check(self.lib.sps_fd_batch_upload(self._h, index, arr.ctypes.data_as(c_void_p), layout))
+
+    def download(self, index: int, layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        d = self.descs[index]
+        shape = interior_shape(d) if layout == abi.LAYOUT_INTERIOR else padded_shape(d)
+        out = np.empty(shape, dtype=np.float32)
+        abi.check(self.lib.sps_fd_batch_download(self._h, index, out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def step(self, nsteps: int = 1):
+        abi.check(self.lib.sps_fd_batch_step(self._h, int(nsteps)))
+
+    def reduce(self):
+        n = len(self.descs)
+        m = (c_double * n)()
+        e = (c_double * n)()
+        abi.check(self.lib.sps_fd_batch_reduce(self._h, m, e))
+        return np.array(m[:]), np.array(e[:])
+
+    def sync(self):
+        abi.check(self.lib.sps_fd_batch_sync(self._h))
+
+    def tune(self, steps_per_launch: int = 0, rows_per_item: int = 0):
+        abi.check(self.lib.sps_fd_batch_tune(self._h, steps_per_launch, rows_per_item))
+
+    def set_stream(self, stream_handle):
+        abi.check(self.lib.sps_fd_batch_set_stream(self._h, c_void_p(stream_handle or 0)))

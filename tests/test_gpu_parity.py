@@ -34,10 +34,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def gpu_run(desc, psi, nsteps, K=0, TY=0, chunks=None):
+def gpu_run(desc, psi, nsteps, K=0, TY=0, chunks=None, TZ=0):
     with FDPlan(desc) as p:
-        if K or TY:
-            p.tune(K, TY)
+        if K or TY or TZ:
+            p.tune(K, TY, TZ)
         p.upload(psi.astype(p.dtype))
         if chunks:
             for c in chunks:
@@ -231,10 +231,11 @@ def test_3d_chunking_is_bitwise(gpu_lib):
     desc = models.model_b((70, 33, 21), h=1.0, dt=0.005, stencil=Stencil3d2h(7, 10, 25),
                           dtype=F32, path=FUSED)
     psi = field(desc, seed=12)
-    say("3D z-chunking")
-    base = gpu_run(desc, psi, 10, TY=32)
-    for TZ in (4, 8, 20, 64):
-        assert np.array_equal(gpu_run(desc, psi, 10, TY=TZ), base), TZ
+    say("3D z-chunking and kernel variants")
+    base = gpu_run(desc, psi, 10, TY=8, TZ=32)
+    for rows in (4, 8, 16):
+        for TZ in (1, 4, 8, 20, 64):
+            assert np.array_equal(gpu_run(desc, psi, 10, TY=rows, TZ=TZ), base), (rows, TZ)
 
 
 def test_graph_replay_matches_eager(gpu_lib):
@@ -350,6 +351,8 @@ def test_state_and_argument_errors(gpu_lib):
         assert ei.value.status == abi.E_STATE
         with pytest.raises(abi.SpsError) as ei:
             p.tune(5, 0)
+        with pytest.raises(abi.SpsError) as ei:
+            p.tune(0, 0, 8)
         assert ei.value.status == abi.E_INVALID
         p.upload(np.zeros((32, 32), np.float32))
         lib = abi.load()
