@@ -181,9 +181,11 @@ int sps_fd_reduce(sps_fd_plan* plan, double* mass, double* energy);
 /* Wait for all work enqueued on the plan. */
 int sps_fd_sync(sps_fd_plan* plan);
 
-/* Enqueue the plan's later work on a caller-owned hipStream_t (NULL restores
- * the plan's own stream).  The caller keeps the stream alive while the plan
- * uses it.  No counterpart in the reference, whose CUDA path has no streams
+/* Enqueue the plan's later work on a caller-owned hipStream_t; NULL is the
+ * HIP null stream, as in every HIP call (step sequences are then launched
+ * one by one instead of through a captured graph, since the null stream
+ * cannot be captured).  The caller keeps the stream alive while the plan uses
+ * it.  No counterpart in the reference, whose CUDA path has no streams
  * (sym/inc/expressioncuda.cuh:1989-2031 synchronises the device after every
  * evaluation); it lets a host share one stream between the solver and its own
  * work, events and captures. */
@@ -192,6 +194,13 @@ int sps_fd_set_stream(sps_fd_plan* plan, void* hip_stream);
 /* Device pointer of the current field (interior layout, dtype of the plan),
  * valid until the next sps_fd_step on the plan. */
 int sps_fd_device_field(sps_fd_plan* plan, void** dptr);
+
+/* Copy the current field to (direction 0) or from (direction 1) caller-owned
+ * device memory of n sites in the interior layout, on the plan's stream.
+ * No counterpart in the reference (its fields live in host memory); it lets
+ * device-resident consumers such as the surrogate's training read solver
+ * states without a host round trip. */
+int sps_fd_copy_device(sps_fd_plan* plan, void* dev, int direction);
 
 /* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
@@ -228,7 +237,13 @@ int sps_fd_batch_sync(sps_fd_batch* batch);
  * steps_per_launch in 1..4, rows_per_item = rows of one work item, 4..512. */
 int sps_fd_batch_tune(sps_fd_batch* batch, int steps_per_launch,
                       int rows_per_item);
-/* Caller-owned hipStream_t for later work (NULL: the batch's own stream). */
+/* Copy instance `index` to (direction 0) or from (direction 1) caller-owned
+ * device memory, interior layout, on the batch's stream (no reference
+ * counterpart, as sps_fd_copy_device). */
+int sps_fd_batch_copy_device(sps_fd_batch* batch, int index, void* dev,
+                             int direction);
+/* Caller-owned hipStream_t for later work (NULL: the HIP null stream, as
+ * sps_fd_set_stream). */
 int sps_fd_batch_set_stream(sps_fd_batch* batch, void* hip_stream);
 
 #ifdef __cplusplus

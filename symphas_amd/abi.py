@@ -76,6 +76,8 @@ def load() -> ctypes.CDLL:
         "sps_fd_sync": (c_int, [P]),
         "sps_fd_device_field": (c_int, [P, POINTER(P)]),
         "sps_fd_set_stream": (c_int, [P, P]),
+        "sps_fd_copy_device": (c_int, [P, P, c_int]),
+        "sps_fd_batch_copy_device": (c_int, [P, c_int, P, c_int]),
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int]),
         "sps_fd_batch_create": (c_int, [POINTER(Desc), c_int, POINTER(P)]),
@@ -88,6 +90,18 @@ def load() -> ctypes.CDLL:
         "sps_fd_batch_tune": (c_int, [P, c_int, c_int]),
         "sps_fd_batch_set_stream": (c_int, [P, P]),
     }
+    I, F = c_int, ctypes.c_float
+    sig.update({
+        "sps_nn_conv3x3": (c_int, [P, I, I, I, I, P, P, I, I, I, P, P, P, P]),
+        "sps_nn_conv3x3_wgrad": (c_int, [P, P, I, I, I, I, I, P, P]),
+        "sps_nn_conv3x3_in1": (c_int, [P, I, I, I, P, P, I, I, P, P, P]),
+        "sps_nn_conv3x3_in1_wgrad": (c_int, [P, P, I, I, I, I, P, P]),
+        "sps_nn_conv3x3_out1": (c_int, [P, I, I, I, I, P, P, P, P]),
+        "sps_nn_conv3x3_out1_bwd": (c_int, [P, P, I, I, I, I, P, P, P, P]),
+        "sps_nn_silu_bwd": (c_int, [P, P, P, c_size_t, P]),
+        "sps_nn_pool2": (c_int, [P, I, I, I, I, I, P, P]),
+        "sps_nn_unpool2": (c_int, [P, I, I, I, I, F, P, P]),
+    })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res

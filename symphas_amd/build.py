@@ -26,6 +26,7 @@ SOURCES = {
     "fused3d.hip": [],
     "reduce.hip": [],
     "batch.hip": [],
+    "nn.hip": [],
 }
 
 COMMON = [

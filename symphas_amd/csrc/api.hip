@@ -314,7 +314,8 @@ int sps_fd_plan::step(int nsteps) {
   const int spl = steps_per_launch();
   int launches = nsteps / spl;
   const int rem = nsteps % spl;
-  if (launches >= 2 * kGraphLaunches) {
+  // the null stream cannot be captured: launch one by one there
+  if (launches >= 2 * kGraphLaunches && stream != nullptr) {
     GraphSlot& g = graphs[cur];
     if (!g.exec) {
       const int start = cur;
@@ -604,7 +605,28 @@ int sps_fd_set_stream(sps_fd_plan* p, void* hip_stream) {
   }
   // order the switch after the work already on the old stream
   SPS_HIP_TRY(hipStreamSynchronize(p->stream));
-  p->stream = hip_stream ? (hipStream_t)hip_stream : p->own_stream;
+  p->stream = (hipStream_t)hip_stream;
+  return SPS_OK;
+}
+
+int sps_fd_copy_device(sps_fd_plan* p, void* dev, int direction) {
+  if (!p || !dev || (direction != 0 && direction != 1)) {
+    set_error("copy_device: plan, device pointer and direction 0/1 required");
+    return SPS_E_INVALID;
+  }
+  const size_t bytes = p->nsites * p->esize;
+  if (direction == 0) {
+    if (!p->has_data) {
+      set_error("no field uploaded");
+      return SPS_E_STATE;
+    }
+    SPS_HIP_TRY(hipMemcpyAsync(dev, p->buf[p->cur], bytes,
+                               hipMemcpyDeviceToDevice, p->stream));
+  } else {
+    SPS_HIP_TRY(hipMemcpyAsync(p->buf[p->cur], dev, bytes,
+                               hipMemcpyDeviceToDevice, p->stream));
+    p->has_data = true;
+  }
   return SPS_OK;
 }
 

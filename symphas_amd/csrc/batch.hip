@@ -206,7 +206,8 @@ struct sps_fd_batch {
   int step(int nsteps) {
     int launches = nsteps / K;
     const int rem = nsteps % K;
-    if (launches >= 2 * kGraphLaunches) {
+    // the null stream cannot be captured: launch one by one there
+    if (launches >= 2 * kGraphLaunches && stream != nullptr) {
       GraphSlot& g = graphs[parity];
       if (!g.exec) {
         const int start = parity;
@@ -489,13 +490,34 @@ int sps_fd_batch_tune(sps_fd_batch* b, int steps_per_launch,
   return SPS_OK;
 }
 
+int sps_fd_batch_copy_device(sps_fd_batch* b, int index, void* dev,
+                             int direction) {
+  SPS_TRY(batch_index_ok(b, index, dev));
+  if (direction != 0 && direction != 1) {
+    set_error("batch copy_device: direction must be 0 or 1");
+    return SPS_E_INVALID;
+  }
+  const sps_fd_desc& d = b->descs[index];
+  const size_t bytes = (size_t)d.n[0] * d.n[1] * sizeof(float);
+  float* buf = b->h_insts[index].buf[b->parity];
+  if (direction == 0)
+    SPS_HIP_TRY(hipMemcpyAsync(dev, buf, bytes, hipMemcpyDeviceToDevice,
+                               b->stream));
+  else {
+    SPS_HIP_TRY(hipMemcpyAsync(buf, dev, bytes, hipMemcpyDeviceToDevice,
+                               b->stream));
+    b->has_data = true;
+  }
+  return SPS_OK;
+}
+
 int sps_fd_batch_set_stream(sps_fd_batch* b, void* hip_stream) {
   if (!b) {
     set_error("batch is NULL");
     return SPS_E_INVALID;
   }
   SPS_HIP_TRY(hipStreamSynchronize(b->stream));
-  b->stream = hip_stream ? (hipStream_t)hip_stream : b->own_stream;
+  b->stream = (hipStream_t)hip_stream;
   return SPS_OK;
 }
 

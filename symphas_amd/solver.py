@@ -94,8 +94,15 @@ class FDPlan:
         abi.check(self.lib.sps_fd_sync(self._h))
 
     def set_stream(self, stream_handle: int | None):
-        """Enqueue later work on a caller-owned hipStream_t (None: own stream)."""
+        """Enqueue later work on a caller-owned hipStream_t (0/None: the null
+        stream, e.g. torch's default stream)."""
         abi.check(self.lib.sps_fd_set_stream(self._h, c_void_p(stream_handle or 0)))
+
+    def copy_to_device(self, dev_ptr: int):
+        abi.check(self.lib.sps_fd_copy_device(self._h, c_void_p(dev_ptr), 0))
+
+    def copy_from_device(self, dev_ptr: int):
+        abi.check(self.lib.sps_fd_copy_device(self._h, c_void_p(dev_ptr), 1))
 
     def device_ptr(self) -> int:
         p = c_void_p()
@@ -177,3 +184,9 @@ class FDBatch:
 
     def set_stream(self, stream_handle):
         abi.check(self.lib.sps_fd_batch_set_stream(self._h, c_void_p(stream_handle or 0)))
+
+    def copy_to_device(self, index: int, dev_ptr: int):
+        abi.check(self.lib.sps_fd_batch_copy_device(self._h, index, c_void_p(dev_ptr), 0))
+
+    def copy_from_device(self, index: int, dev_ptr: int):
+        abi.check(self.lib.sps_fd_batch_copy_device(self._h, index, c_void_p(dev_ptr), 1))

@@ -1,0 +1,220 @@
+"""BF16 MFMA convolution kernels against the FP64 oracle (oracle/conv_oracle.py)
+on BF16-rounded inputs.  BF16 outputs must be within one BF16 ulp of the FP64
+result (the kernels accumulate in FP32 and round once); FP32 weight gradients
+within the FP32 accumulation tolerance."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import conv_oracle as co
+from symphas_amd import nn
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def bf(a):
+    return torch.from_numpy(co.bf16_round(a).astype(np.float32)).to(DEV).to(torch.bfloat16)
+
+
+def np64(t):
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+def within_ulp(got, want, ulps=1.0, floor=1e-30):
+    ulp = np.maximum(np.abs(want), floor) * 2.0 ** -7
+    return np.abs(got - want) <= ulps * ulp + 1e-6
+
+
+def rnd(shape, seed, scale=1.0):
+    return np.random.default_rng(seed).standard_normal(shape) * scale
+
+
+@pytest.mark.parametrize("N,H,W,C,K", [(2, 16, 16, 32, 64), (1, 8, 32, 64, 32),
+                                       (4, 32, 32, 128, 256), (1, 16, 64, 256, 128)])
+def test_conv_forward_plain(gpu_lib, N, H, W, C, K):
+    print(f"[case] conv fwd {N}x{H}x{W} {C}->{K}", flush=True)
+    x = co.bf16_round(rnd((N, H, W, C), 1))
+    w = co.bf16_round(rnd((9, K, C), 2, 1.0 / np.sqrt(9 * C)))
+    b = rnd(K, 3, 0.1).astype(np.float32)
+    xt, wt = bf(x), bf(w)
+    bt = torch.from_numpy(b).to(DEV)
+    y, pre = nn.conv3x3_raw(xt, wt, bt, nn.ACT_SILU, want_pre=True)
+    torch.cuda.synchronize()
+    ref = co.conv3x3(x, w, b.astype(np.float64))
+    assert within_ulp(np64(pre), ref).all()
+    assert within_ulp(np64(y), co.silu(co.bf16_round(ref)), ulps=2).mean() > 0.999
+    y0, _ = nn.conv3x3_raw(xt, wt, None, nn.ACT_NONE)
+    assert within_ulp(np64(y0), co.conv3x3(x, w)).all()
+
+
+def test_conv_forward_upsample_add(gpu_lib):
+    print("[case] conv fwd upsample-add", flush=True)
+    N, H, W, C, K = 2, 8, 16, 64, 32
+    x = co.bf16_round(rnd((N, H, W, C), 4))
+    w = co.bf16_round(rnd((9, K, C), 5, 1.0 / np.sqrt(9 * C)))
+    b = rnd(K, 6, 0.1).astype(np.float32)
+    skip = co.bf16_round(rnd((N, 2 * H, 2 * W, K), 7))
+    y, pre = nn.conv3x3_raw(bf(x), bf(w), torch.from_numpy(b).to(DEV), nn.ACT_SILU,
+                            nn.EPI_UPADD, bf(skip), want_pre=True)
+    # the kernel adds the FP32 accumulator to the skip before rounding
+    ref = co.upsample2(co.conv3x3(x, w, b.astype(np.float64))) + skip
+    assert within_ulp(np64(pre), ref, ulps=1.01).all()
+
+
+@pytest.mark.parametrize("N,H,W,C,K", [(2, 16, 32, 32, 64), (1, 32, 32, 128, 64)])
+def test_conv_backward_data(gpu_lib, N, H, W, C, K):
+    print(f"[case] conv bwd data {C}->{K}", flush=True)
+    w = co.bf16_round(rnd((9, K, C), 8, 0.2))
+    dy = co.bf16_round(rnd((N, H, W, K), 9))
+    wb = nn.flip_weights(torch.from_numpy(w).float()).to(DEV).to(torch.bfloat16)
+    dx, _ = nn.conv3x3_raw(bf(dy), wb, None, nn.ACT_NONE)
+    assert within_ulp(np64(dx), co.conv3x3_bwd_data(dy, w)).all()
+
+
+@pytest.mark.parametrize("N,H,W,C,K", [(2, 8, 32, 32, 64), (1, 16, 64, 64, 32),
+                                       (3, 4, 32, 64, 64), (1, 8, 32, 32, 32),
+                                       (2, 8, 8, 64, 64), (1, 4, 16, 32, 64)])
+def test_conv_backward_weight(gpu_lib, N, H, W, C, K):
+    print(f"[case] conv wgrad {C}->{K}", flush=True)
+    x = co.bf16_round(rnd((N, H, W, C), 10))
+    dy = co.bf16_round(rnd((N, H, W, K), 11))
+    dw = nn.conv3x3_wgrad_raw(bf(x), bf(dy), K)
+    ref = co.conv3x3_bwd_weight(x, dy)
+    got = dw.cpu().numpy().astype(np.float64)
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max() * np.sqrt(N * H * W)
+
+
+def test_thin_layers(gpu_lib):
+    print("[case] thin layers", flush=True)
+    N, H, W, K = 2, 16, 16, 32
+    lib = nn._lib()
+    x = rnd((N, H, W), 12).astype(np.float32)
+    w = rnd((9, K), 13, 0.3).astype(np.float32)
+    b = rnd(K, 14, 0.1).astype(np.float32)
+    xt, wt, bt = (torch.from_numpy(a).to(DEV) for a in (x, w, b))
+    y = torch.empty((N, H, W, K), device=DEV, dtype=torch.bfloat16)
+    pre = torch.empty_like(y)
+    nn._check(lib.sps_nn_conv3x3_in1(nn._ptr(xt), N, H, W, nn._ptr(wt), nn._ptr(bt), K, 1,
+                                     nn._ptr(y), nn._ptr(pre), nn._stream()))
+    ref = co.conv3x3(x[..., None].astype(np.float64), w[:, :, None].astype(np.float64), b)
+    assert within_ulp(np64(pre), ref).all()
+    dpre = co.bf16_round(rnd((N, H, W, K), 15))
+    dw = torch.zeros((9, K), device=DEV)
+    nn._check(lib.sps_nn_conv3x3_in1_wgrad(nn._ptr(xt), nn._ptr(bf(dpre)), N, H, W, K,
+                                           nn._ptr(dw), nn._stream()))
+    refw = co.conv3x3_bwd_weight(x[..., None].astype(np.float64), dpre)[:, :, 0]
+    assert np.allclose(dw.cpu().numpy(), refw, rtol=1e-4, atol=1e-4)
+    # last layer
+    h = co.bf16_round(rnd((N, H, W, K), 16))
+    wo = rnd((9, K), 17, 0.2).astype(np.float32)
+    bo = np.array([0.3], np.float32)
+    out = torch.empty((N, H, W), device=DEV)
+    wot, bot = torch.from_numpy(wo).to(DEV), torch.from_numpy(bo).to(DEV)
+    nn._check(lib.sps_nn_conv3x3_out1(nn._ptr(bf(h)), N, H, W, K, nn._ptr(wot), nn._ptr(bot),
+                                      nn._ptr(out), nn._stream()))
+    refo = co.conv3x3(h, wo[:, None, :].astype(np.float64), bo.astype(np.float64))[..., 0]
+    assert np.allclose(out.cpu().numpy(), refo, rtol=1e-5, atol=1e-4)
+    dy = rnd((N, H, W), 18).astype(np.float32)
+    dx = torch.empty((N, H, W, K), device=DEV, dtype=torch.bfloat16)
+    dwo = torch.zeros((9, K), device=DEV)
+    nn._check(lib.sps_nn_conv3x3_out1_bwd(nn._ptr(bf(h)), nn._ptr(torch.from_numpy(dy).to(DEV)),
+                                          N, H, W, K, nn._ptr(wot), nn._ptr(dx), nn._ptr(dwo),
+                                          nn._stream()))
+    refdx = co.conv3x3_bwd_data(dy[..., None].astype(np.float64), wo[:, None, :].astype(np.float64))
+    assert within_ulp(np64(dx), refdx, ulps=1.01).all()
+    refdw = co.conv3x3_bwd_weight(h, dy[..., None].astype(np.float64))[:, 0, :]
+    assert np.allclose(dwo.cpu().numpy(), refdw, rtol=1e-4, atol=1e-4)
+
+
+def test_elementwise(gpu_lib):
+    print("[case] elementwise", flush=True)
+    a = co.bf16_round(rnd((2, 8, 16, 32), 19, 3.0))
+    d = co.bf16_round(rnd((2, 8, 16, 32), 20))
+    dx = nn.silu_bwd_raw(bf(d), bf(a))
+    assert within_ulp(np64(dx), d * co.silu_grad(a), ulps=2).all()
+    p = nn.pool2_raw(bf(a), 0)
+    assert within_ulp(np64(p), co.pool2(a), ulps=1.01).all()
+    s = nn.pool2_raw(bf(a), 1)
+    assert within_ulp(np64(s), co.pool2(a, "sum"), ulps=1.01).all()
+    u = nn.unpool2_raw(bf(a), 0.25)
+    assert np.array_equal(np64(u), co.upsample2(a) * 0.25)
+
+
+def _unet_cpu_fp64(net, psi):
+    """FP64 CPU restatement of UNet.forward with torch's conv2d (circular
+    padding) and autograd: same weights rounded to BF16, activations not
+    rounded.  Returns (output, parameters in the order of net.parameters())."""
+    import torch.nn.functional as F
+
+    def cw(w):  # [9][K][C] -> [K][C][3][3]
+        w = w.detach().cpu().to(torch.bfloat16).double()
+        return w.reshape(3, 3, w.shape[1], w.shape[2]).permute(2, 3, 0, 1).contiguous()
+
+    def conv(x, w, b):
+        return F.conv2d(F.pad(x, (1, 1, 1, 1), mode="circular"), w, b)
+
+    params = {}
+    for name, p in net.named_parameters():
+        if name in ("w_in", "w_out"):
+            t = p.detach().cpu().double()
+        elif "_w." in name:
+            t = cw(p)
+        else:
+            t = p.detach().cpu().double()
+        params[name] = t.requires_grad_(True)
+    L = len(net.ch)
+    x = psi.cpu().double()[:, None]
+    wi = params["w_in"].t().reshape(-1, 1, 3, 3)  # [9][K] -> [K][1][3][3]
+    x = F.silu(conv(x, wi, params["b_in"]))
+    skips = []
+    for l in range(L):
+        if l > 0:
+            x = F.avg_pool2d(x, 2)
+        for j in (0, 1):
+            x = F.silu(conv(x, params[f"enc_w.{2 * l + j}"], params[f"enc_b.{2 * l + j}"]))
+        skips.append(x)
+    for i, l in enumerate(reversed(range(L - 1))):
+        y = conv(x, params[f"dec_w.{2 * i}"], params[f"dec_b.{2 * i}"])
+        x = F.silu(F.interpolate(y, scale_factor=2, mode="nearest") + skips[l])
+        x = F.silu(conv(x, params[f"dec_w.{2 * i + 1}"], params[f"dec_b.{2 * i + 1}"]))
+    wo = params["w_out"].t().reshape(1, -1, 3, 3)
+    d = conv(x, wo, None)[:, 0]
+    d = d - d.mean(dim=(1, 2), keepdim=True)
+    return psi.cpu().double() + net.out_scale * d, params
+
+
+def test_unet_forward_and_gradient_vs_cpu(gpu_lib):
+    """Whole network: forward and the gradient of a loss against an FP64 CPU
+    autograd restatement (torch conv2d, circular padding).  The GPU keeps
+    activations and activation gradients in BF16, so agreement is at the
+    percent level."""
+    print("[case] unet forward/gradient", flush=True)
+    torch.manual_seed(0)
+    net = nn.UNet(seed=1).to(DEV)
+    with torch.no_grad():
+        net.w_out.copy_(torch.randn_like(net.w_out) * 0.1)
+    psi = torch.from_numpy(rnd((2, 64, 64), 21, 0.5).astype(np.float32)).to(DEV)
+    target = torch.from_numpy(rnd((2, 64, 64), 22, 0.5).astype(np.float32)).to(DEV)
+    out = net(psi)
+    loss = ((out - target) ** 2).mean()
+    net.zero_grad()
+    loss.backward()
+    ref_out, params = _unet_cpu_fp64(net, psi)
+    ref_loss = ((ref_out - target.cpu().double()) ** 2).mean()
+    ref_loss.backward()
+    inc = (out - psi).detach().cpu().double()
+    ref_inc = (ref_out - psi.cpu().double()).detach()
+    err = float((inc - ref_inc).norm() / ref_inc.norm())
+    print(f"unet increment rel L2 vs FP64 CPU: {err:.2e}")
+    assert err < 2e-2
+    for name, p in net.named_parameters():
+        g = p.grad.detach().cpu().double()
+        r = params[name].grad
+        if "_w." in name:  # back to [9][K][C]
+            r = r.permute(2, 3, 0, 1).reshape(9, r.shape[0], r.shape[1])
+        elif name in ("w_in", "w_out"):
+            r = r.reshape(g.shape)
+        rel = float((g - r).norm() / max(float(r.norm()), 1e-30))
+        print(f"grad {name:10s} rel L2 {rel:.2e}")
+        assert rel < 5e-2, name
