@@ -82,6 +82,7 @@ def extra_cfg3(torch, stream, steps=200):
     with FDPlan(d) as p:
         p.set_stream(stream.cuda_stream)
         p.upload(psi)
+        shape = p.autotune(use_cache=False)
         m0, _ = p.reduce()
         p.step(10)
         p.upload(psi)
@@ -92,6 +93,7 @@ def extra_cfg3(torch, stream, steps=200):
     g = n ** 3 * steps / (ms / 1e3) / 1e9
     return {"workload": "Model B 3D 512^3 FP32, Stencil3d2h<7,10,25>, dt=0.005",
             "steps": steps, "Gsite_updates_per_s": g,
+            "launch_shape": {"rows_per_wave": shape[1], "planes_per_chunk": shape[2]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
             "mass_drift_rel": abs(m1 - m0) / float(np.abs(psi.astype(np.float64)).sum())}
 
@@ -111,6 +113,7 @@ def extra_cfg4(torch, stream, steps=480):
         bt.set_stream(stream.cuda_stream)
         for i, f in enumerate(fields):
             bt.upload(i, f)
+        bshape = bt.autotune()
         bt.step(48)
         bt.sync()
         a, b = _events(torch)
@@ -138,6 +141,7 @@ def extra_cfg4(torch, stream, steps=480):
                         "{128,192,256,384,512}, M, kappa ~ U[0.5,2] (Model B draws kept "
                         "within the explicit-Euler bound), FP32",
             "sites": sites, "steps": steps, "Gsite_updates_per_s": g,
+            "launch_shape": {"steps_per_launch": bshape[0], "rows_per_item": bshape[1]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
             "sequential_plans_Gsite_updates_per_s": seq,
             "speedup_vs_sequential": g / seq}
@@ -264,10 +268,13 @@ def main():
     psi0 = inputs.uniform((N, N), seed=SEED, dtype=np.float32)
     d32 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
     plan = FDPlan(d32)
-    if args.steps_per_launch or args.rows_per_tile:
-        plan.tune(args.steps_per_launch, args.rows_per_tile, 0)
     plan.set_stream(stream.cuda_stream)
     plan.upload(psi0)
+    if args.steps_per_launch or args.rows_per_tile:
+        plan.tune(args.steps_per_launch, args.rows_per_tile, 0)
+        tuned = "fixed by flags"
+    else:
+        tuned = "autotuned: %d steps per launch, %d rows per chunk" % plan.autotune(use_cache=False)[:2]
     m0, e0 = plan.reduce()
     path, spl = plan.query()
 
@@ -359,7 +366,7 @@ def main():
         "config": {"workload": "Model B 2D 4096x4096 periodic, h=1, dt=0.01, "
                                "Stencil2d2h<5,6,13>, FP32 fields, FP64 sums",
                    "sites": sites, "path": "fused" if path == abi.PATH_FUSED else "reference",
-                   "steps_per_launch": spl},
+                   "steps_per_launch": spl, "launch_shape": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,

@@ -214,6 +214,19 @@ int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile,
                 int planes_per_chunk);
 
+/* Pick the fused path's launch shape by timing candidates on the device:
+ * 2D steps_per_launch {1,2,3,4} x rows_per_tile {8,16,32,64}; 3D rows
+ * {4 (LDS kernel), 8, 16} x planes {8,16,32,64}.  The field is saved before
+ * and restored after the trial launches, so the plan's state is unchanged.
+ * When the environment variable SPS_FD_TUNE_CACHE names a file, a choice
+ * recorded there for the same (dim, shape, dtype, lowered form, GPU) is used
+ * without timing (unless use_cache is 0) and new choices are appended.
+ * The chosen shape is applied to the plan and returned through the optional
+ * pointers.  Plans on the reference path have nothing to tune (SPS_OK).
+ * No counterpart in the reference (its CPU loops have no launch shape). */
+int sps_fd_autotune(sps_fd_plan* plan, int use_cache, int* steps_per_launch,
+                    int* rows_per_tile, int* planes_per_chunk);
+
 /* ---------------------------------------------------------------- batched */
 
 /* Many independent 2D fields of uneven sizes and coefficients, advanced by one
@@ -242,6 +255,11 @@ int sps_fd_batch_tune(sps_fd_batch* batch, int steps_per_launch,
  * counterpart, as sps_fd_copy_device). */
 int sps_fd_batch_copy_device(sps_fd_batch* batch, int index, void* dev,
                              int direction);
+/* Time steps_per_launch {1..4} x rows_per_item {16,32,64,128} on the
+ * device (fields saved and restored) and apply the fastest; returns the
+ * choice through the optional pointers. */
+int sps_fd_batch_autotune(sps_fd_batch* batch, int* steps_per_launch,
+                          int* rows_per_item);
 /* Caller-owned hipStream_t for later work (NULL: the HIP null stream, as
  * sps_fd_set_stream). */
 int sps_fd_batch_set_stream(sps_fd_batch* batch, void* hip_stream);

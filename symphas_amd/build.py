@@ -27,6 +27,7 @@ SOURCES = {
     "reduce.hip": [],
     "batch.hip": [],
     "nn.hip": [],
+    "tune.hip": [],
 }
 
 COMMON = [

@@ -511,6 +511,77 @@ int sps_fd_batch_copy_device(sps_fd_batch* b, int index, void* dev,
   return SPS_OK;
 }
 
+int sps_fd_batch_autotune(sps_fd_batch* b, int* out_k, int* out_rows) {
+  if (!b) {
+    set_error("batch is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!b->has_data) {
+    set_error("autotune needs uploaded fields");
+    return SPS_E_STATE;
+  }
+  const size_t bytes = b->total * sizeof(float);
+  float* saved = nullptr;
+  SPS_HIP_TRY(hipMalloc(&saved, bytes));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SPS_OK;
+  const int oldK = b->K, oldTY = b->TY;
+  int bestK = oldK, bestTY = oldTY;
+  double best = 1e300;
+  float* cur = b->pool + (size_t)b->parity * b->total;
+  if (hipMemcpyAsync(saved, cur, bytes, hipMemcpyDeviceToDevice, b->stream) !=
+          hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = SPS_E_HIP;
+  for (int rows : {16, 32, 64, 128}) {
+    if (rc != SPS_OK) break;
+    b->TY = rows;
+    if ((rc = b->build_items()) != SPS_OK) break;
+    for (int k = 1; k <= 4 && rc == SPS_OK; ++k) {
+      double ms_best = 1e300;
+      for (int rep = 0; rep < 3 && rc == SPS_OK; ++rep) {
+        if ((rc = b->launch_one(k)) != SPS_OK) break;
+        if (hipEventRecord(e0, b->stream) != hipSuccess) rc = SPS_E_HIP;
+        for (int i = 0; i < 12 / k && rc == SPS_OK; ++i) rc = b->launch_one(k);
+        if (rc != SPS_OK || hipEventRecord(e1, b->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess) {
+          if (rc == SPS_OK) rc = SPS_E_HIP;
+          break;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) ms_best = std::min(ms_best, (double)ms);
+      }
+      if (rc == SPS_OK && ms_best < best) {
+        best = ms_best;
+        bestK = k;
+        bestTY = rows;
+      }
+    }
+  }
+  cur = b->pool + (size_t)b->parity * b->total;
+  if (hipMemcpyAsync(cur, saved, bytes, hipMemcpyDeviceToDevice, b->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->stream) != hipSuccess) {
+    if (rc == SPS_OK) rc = SPS_E_HIP;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(saved);
+  b->K = rc == SPS_OK ? bestK : oldK;
+  b->TY = rc == SPS_OK ? bestTY : oldTY;
+  int rc2 = b->build_items();
+  b->drop_graphs();
+  if (rc != SPS_OK) {
+    set_error("batch autotune failed");
+    return rc;
+  }
+  if (rc2 != SPS_OK) return rc2;
+  if (out_k) *out_k = b->K;
+  if (out_rows) *out_rows = b->TY;
+  return SPS_OK;
+}
+
 int sps_fd_batch_set_stream(sps_fd_batch* b, void* hip_stream) {
   if (!b) {
     set_error("batch is NULL");

@@ -1,0 +1,184 @@
+// tune.hip - launch-shape autotuner of the fused single-field step.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "plan.hpp"
+
+using namespace sps;
+
+namespace {
+
+std::string device_tag() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return "unknown";
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s/%dcu", prop.gcnArchName,
+                prop.multiProcessorCount);
+  for (char* c = buf; *c; ++c)
+    if (*c == ' ') *c = '_';
+  return buf;
+}
+
+std::string plan_key(const sps_fd_plan* p) {
+  const FusedCoef<double>& c = p->fc64;
+  char buf[512];
+  std::snprintf(buf, sizeof(buf),
+                "dim=%d n=%dx%dx%d dtype=%s radius=%d a=%d deg=%d dev=%s",
+                p->d.dim, p->d.n[0], p->d.n[1], p->d.dim == 3 ? p->d.n[2] : 1,
+                p->d.dtype == SPS_F32 ? "f32" : "f64", c.radius,
+                c.dega >= 0 ? 1 : 0,
+                (c.dega > c.degb ? c.dega : c.degb) <= 3 ? 3 : 6,
+                device_tag().c_str());
+  return buf;
+}
+
+bool cache_lookup(const std::string& key, int* k, int* r, int* z) {
+  const char* path = std::getenv("SPS_FD_TUNE_CACHE");
+  if (!path || !*path) return false;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return false;
+  char line[1024];
+  bool found = false;
+  while (std::fgets(line, sizeof(line), f)) {
+    char* bar = std::strstr(line, " | ");
+    if (!bar) continue;
+    if (std::string(line, bar - line) != key) continue;
+    int a, b, c;
+    if (std::sscanf(bar + 3, "%d %d %d", &a, &b, &c) == 3) {
+      *k = a;
+      *r = b;
+      *z = c;
+      found = true;  // the last entry for a key wins
+    }
+  }
+  std::fclose(f);
+  return found;
+}
+
+void cache_store(const std::string& key, int k, int r, int z, double us) {
+  const char* path = std::getenv("SPS_FD_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%s | %d %d %d | %.3f us/step\n", key.c_str(), k, r, z, us);
+  std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
+                               int* out_rows, int* out_planes) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  auto report = [&]() {
+    if (out_k) *out_k = p->K;
+    if (out_rows) *out_rows = p->TY;
+    if (out_planes) *out_planes = p->d.dim == 3 ? p->TZ : 0;
+    return SPS_OK;
+  };
+  if (p->path != SPS_PATH_FUSED) return report();
+  if (!p->has_data) {
+    set_error("autotune needs an uploaded field");
+    return SPS_E_STATE;
+  }
+  const std::string key = plan_key(p);
+  int ck, cr, cz;
+  if (use_cache && cache_lookup(key, &ck, &cr, &cz)) {
+    p->K = ck;
+    p->TY = cr;
+    if (p->d.dim == 3) p->TZ = cz;
+    p->drop_graphs();
+    return report();
+  }
+  struct Cand {
+    int k, rows, planes;
+  };
+  std::vector<Cand> cands;
+  if (p->d.dim == 2) {
+    for (int k = 1; k <= 4; ++k)
+      for (int r : {8, 16, 32, 64}) cands.push_back({k, r, 0});
+  } else {
+    for (int r : {8, 16})
+      for (int z : {8, 16, 32, 64}) cands.push_back({1, r, z});
+    cands.push_back({1, 4, 64});
+  }
+  const size_t bytes = p->nsites * p->esize;
+  void* saved = nullptr;
+  SPS_HIP_TRY(hipMalloc(&saved, bytes));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SPS_OK;
+  const int oldK = p->K, oldTY = p->TY, oldTZ = p->TZ;
+  Cand best{oldK, oldTY, oldTZ};
+  double best_us = 1e300;
+  do {
+    if (hipMemcpyAsync(saved, p->buf[p->cur], bytes, hipMemcpyDeviceToDevice,
+                       p->stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      set_error("autotune set-up failed");
+      rc = SPS_E_HIP;
+      break;
+    }
+    for (const Cand& c : cands) {
+      p->K = c.k;
+      p->TY = c.rows;
+      if (p->d.dim == 3) p->TZ = c.planes;
+      const int spl = p->steps_per_launch();
+      // about 12 steps (a multiple of every K) per trial, twice
+      const int launches = p->d.dim == 3 ? 4 : 12 / spl;
+      double us = 1e300;
+      for (int rep = 0; rep < 3 && rc == SPS_OK; ++rep) {
+        if ((rc = p->launch_one(spl)) != SPS_OK) break;  // warm
+        if (hipEventRecord(e0, p->stream) != hipSuccess) rc = SPS_E_HIP;
+        for (int i = 0; i < launches && rc == SPS_OK; ++i)
+          rc = p->launch_one(spl);
+        if (rc != SPS_OK || hipEventRecord(e1, p->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess) {
+          if (rc == SPS_OK) rc = SPS_E_HIP;
+          break;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (rep > 0) us = std::min(us, 1e3 * ms / (launches * spl));
+      }
+      if (rc != SPS_OK) break;
+      if (us < best_us) {
+        best_us = us;
+        best = c;
+      }
+    }
+    // restore the field the caller had
+    if (hipMemcpyAsync(p->buf[p->cur], saved, bytes, hipMemcpyDeviceToDevice,
+                       p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess) {
+      if (rc == SPS_OK) {
+        set_error("autotune restore failed");
+        rc = SPS_E_HIP;
+      }
+    }
+  } while (false);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipFree(saved);
+  if (rc != SPS_OK) {
+    p->K = oldK;
+    p->TY = oldTY;
+    p->TZ = oldTZ;
+    p->drop_graphs();
+    return rc;
+  }
+  p->K = best.k;
+  p->TY = best.rows;
+  if (p->d.dim == 3) p->TZ = best.planes;
+  p->drop_graphs();
+  cache_store(key, p->K, p->TY, p->d.dim == 3 ? p->TZ : 0, best_us);
+  return report();
+}

@@ -117,6 +117,13 @@ class FDPlan:
     def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0):
         abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk))
 
+    def autotune(self, use_cache: bool = True):
+        """Time candidate launch shapes on the device and keep the fastest;
+        returns (steps_per_launch, rows_per_tile, planes_per_chunk)."""
+        k, r, z = c_int(), c_int(), c_int()
+        abi.check(self.lib.sps_fd_autotune(self._h, int(use_cache), byref(k), byref(r), byref(z)))
+        return k.value, r.value, z.value
+
 
 def device_count() -> int:
     return int(abi.load().sps_fd_device_count())
@@ -181,6 +188,11 @@ class FDBatch:
 
     def tune(self, steps_per_launch: int = 0, rows_per_item: int = 0):
         abi.check(self.lib.sps_fd_batch_tune(self._h, steps_per_launch, rows_per_item))
+
+    def autotune(self):
+        k, r = c_int(), c_int()
+        abi.check(self.lib.sps_fd_batch_autotune(self._h, byref(k), byref(r)))
+        return k.value, r.value
 
     def set_stream(self, stream_handle):
         abi.check(self.lib.sps_fd_batch_set_stream(self._h, c_void_p(stream_handle or 0)))

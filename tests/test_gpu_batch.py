@@ -92,3 +92,19 @@ def test_batch_rejects_unsupported(gpu_lib):
     d = models.model_b((128, 128), dtype=abi.F64)
     with pytest.raises(abi.SpsError):
         FDBatch([d])
+
+
+def test_batch_autotune_keeps_fields(gpu_lib):
+    descs, fields = _mixed(count=6, seed=13)
+    print("[case] batch autotune", flush=True)
+    with FDBatch(descs) as b:
+        for i, f in enumerate(fields):
+            b.upload(i, f)
+        k, rows = b.autotune()
+        assert k in (1, 2, 3, 4) and rows in (16, 32, 64, 128)
+        for i, f in enumerate(fields):
+            assert np.array_equal(b.download(i), f)
+        b.step(24)
+        got = [b.download(i) for i in range(len(descs))]
+    for d, f, g in zip(descs, fields, got):
+        assert np.array_equal(g, _single(d, f, 24))

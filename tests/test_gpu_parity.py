@@ -372,3 +372,47 @@ def test_cfg2_full_size_two_steps(gpu_lib, orc):
     got = gpu_run(desc32, psi, 2).astype(np.float64)
     want = oracle_run(orc, desc64, psi.astype(np.float64), 2)
     assert rel_l2(got, want) <= 1e-6
+
+
+# --------------------------------------------------------------- autotuner
+def test_autotune_keeps_field_and_caches(gpu_lib, tmp_path, monkeypatch):
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("SPS_FD_TUNE_CACHE", str(cache))
+    desc = models.model_b((512, 384), h=1.0, dt=0.01, dtype=F32, path=FUSED)
+    psi = field(desc, seed=20).astype(np.float32)
+    say("autotune 2D")
+    base = gpu_run(desc, psi, 24, K=1, TY=32)
+    with FDPlan(desc) as p:
+        p.upload(psi)
+        choice = p.autotune(use_cache=False)
+        assert np.array_equal(p.download(), psi)
+        p.step(24)
+        assert np.array_equal(p.download(), base)
+    assert choice[0] in (1, 2, 3, 4) and choice[1] in (8, 16, 32, 64)
+    text = cache.read_text()
+    assert "dim=2 n=512x384x1 dtype=f32" in text
+    with FDPlan(desc) as p:
+        p.upload(psi)
+        assert p.autotune(use_cache=True) == choice
+
+
+def test_autotune_3d(gpu_lib):
+    desc = models.model_b((96, 64, 40), h=1.0, dt=0.005, stencil=Stencil3d2h(7, 10, 25),
+                          dtype=F32, path=FUSED)
+    psi = field(desc, seed=21).astype(np.float32)
+    say("autotune 3D")
+    base = gpu_run(desc, psi, 6, TY=8, TZ=32)
+    with FDPlan(desc) as p:
+        p.upload(psi)
+        k, rows, planes = p.autotune(use_cache=False)
+        assert k == 1 and rows in (4, 8, 16) and planes in (8, 16, 32, 64)
+        assert np.array_equal(p.download(), psi)
+        p.step(6)
+        assert np.array_equal(p.download(), base)
+
+
+def test_autotune_reference_path_is_noop(gpu_lib):
+    desc = models.model_b((30, 20), dtype=F64, path=REF)
+    with FDPlan(desc) as p:
+        p.upload(field(desc, seed=22))
+        assert p.autotune()[0] == 1

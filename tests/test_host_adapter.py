@@ -1,0 +1,100 @@
+"""The C++ drop-in adapter (symphas_amd/host/solverft_mi355x.hpp): its
+EVOLUTION lowering produces the same term lists as symphas_amd/models.py
+(the reference's MODEL definitions), it rejects what the step cannot run, and
+models driven through it on the GPU match the CPU oracle."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from symphas_amd import abi, inputs, models
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+@pytest.fixture(scope="session")
+def driver(tmp_path_factory):
+    out = tmp_path_factory.mktemp("adapter") / "adapter_driver"
+    lib = ROOT / "symphas_amd" / "lib"
+    cmd = [shutil.which("g++") or "g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror",
+           str(ROOT / "tests" / "host" / "adapter_driver.cpp"), f"-L{lib}", "-lsps_fd",
+           f"-Wl,-rpath,{lib}", "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    return out
+
+
+def _parse(text):
+    out = {}
+    for line in text.strip().splitlines():
+        parts = [p.strip() for p in line.split("|")]
+        name, rest = parts[0], parts[1:]
+        if rest and rest[0].startswith("error"):
+            out[name] = rest[0]
+            continue
+        terms = []
+        for t in rest:
+            kv = dict(x.split("=", 1) for x in t.split())
+            terms.append((int(kv["op"]), int(kv["axis"]), float(kv["coeff"]),
+                          [float(v) for v in kv["poly"].split(",")]))
+        out[name] = terms
+    return out
+
+
+def _expected(desc):
+    return [(desc.terms[i].op, desc.terms[i].axis, desc.terms[i].coeff,
+             list(desc.terms[i].poly)) for i in range(desc.nterms)]
+
+
+def test_lowering_matches_model_definitions(driver):
+    res = subprocess.run([str(driver), "lower"], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    got = _parse(res.stdout)
+    assert got["allen_cahn"] == _expected(models.allen_cahn((8, 8)))
+    assert got["model_b"] == _expected(models.model_b((8, 8)))
+    assert got["model_a"] == _expected(models.make_desc((8, 8), models.Stencil2d2h(), 1, 0.01,
+                                                        models.model_a_terms()))
+    assert got["model_b_coeffs"] == _expected(models.model_b((8, 8), c1=0.7, c2=1.3))
+    assert got["lap_lap"] == [(abi.OP_BILAP, 0, 1.0, [0, 1, 0, 0, 0, 0, 0]),
+                              (abi.OP_LAP, 0, 0.5, [0, 0, 1, 0, 0, 0, 0])]
+    assert got["gradlap_y"] == [(abi.OP_GRADLAP, 1, 2.0, [0, 0, 1, 0, 0, 0, 0])]
+    for name in ("derivative_product", "derivative_times_field", "power7", "second_field"):
+        assert got[name].startswith("error -2"), name
+
+
+def _run(driver, case, tmp_path):
+    out = tmp_path / f"{case}.bin"
+    print(f"[case] adapter {case}", flush=True)
+    res = subprocess.run([str(driver), "run", case, str(out)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    return np.fromfile(out, dtype=np.float64)
+
+
+@pytest.mark.gpu
+def test_adapter_allen_cahn_bitwise(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "allen_cahn", tmp_path).reshape(64, 64)
+    d = models.allen_cahn((64, 64), h=0.5, dt=0.01, dtype=abi.F64, path=abi.PATH_REFERENCE)
+    want = orc.unpad(d, orc.run(d, inputs.uniform((64, 64), seed=1), 1000, 8))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_adapter_model_b_single_steps_bitwise(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "model_b_steps", tmp_path).reshape(40, 48)
+    d = models.model_b((48, 40), h=1.0, dt=0.01, stencil=models.Stencil2d2h(9, 6, 13),
+                       dtype=abi.F64, path=abi.PATH_REFERENCE)
+    want = orc.unpad(d, orc.run(d, inputs.uniform((40, 48), seed=2), 50, 8))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_adapter_model_b_3d_fp32(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "model_b_3d_fp32", tmp_path).reshape(16, 24, 32)
+    d = models.model_b((32, 24, 16), h=1.0, dt=0.005, stencil=models.Stencil3d2h(7, 10, 25),
+                       dtype=abi.F64, path=abi.PATH_REFERENCE)
+    psi = inputs.uniform((16, 24, 32), seed=3, lo=-0.05, hi=0.05).astype(np.float32)
+    want = orc.unpad(d, orc.run(d, psi.astype(np.float64), 40, 8))
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
