@@ -43,7 +43,38 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other configurations (3D, batched, surrogate)")
     ap.add_argument("--surrogate-iters", type=int, default=300)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device 5 times (the known-size "
+                         "transfer that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_probe():
+    import torch
+    torch.cuda.set_device(0)
+    n = 64 * 1024 * 1024  # 256 MiB of FP32
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": 4 * n, "repeats": 5}), flush=True)
+
+
+def profiled_traffic(spl, rows):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled
+    by the copy calibration) for the same kernel and launch shape, from the
+    committed profile summary; None when the shapes differ."""
+    path = ROOT / "profiles" / "fused2d_modelb_4096_f32.md"
+    try:
+        for line in path.read_text().splitlines():
+            if line.startswith("machine-readable:"):
+                kv = dict(x.strip().split(" = ") for x in line.split(":", 1)[1].split(";"))
+                if int(kv["steps_per_launch"]) == spl and int(kv["rows_per_tile"]) == rows:
+                    return float(kv["traffic_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
+    return None
 
 
 def _events(torch):
@@ -248,6 +279,9 @@ def cpu_baseline(args):
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
@@ -274,7 +308,9 @@ def main():
         plan.tune(args.steps_per_launch, args.rows_per_tile, 0)
         tuned = "fixed by flags"
     else:
-        tuned = "autotuned: %d steps per launch, %d rows per chunk" % plan.autotune(use_cache=False)[:2]
+        shape = plan.autotune(use_cache=False)
+        tuned = "autotuned: %d steps per launch, %d rows per chunk" % shape[:2]
+    k_rows = plan.query()[1], (args.rows_per_tile or (shape[1] if not args.steps_per_launch else 0))
     m0, e0 = plan.reduce()
     path, spl = plan.query()
 
@@ -369,7 +405,7 @@ def main():
                    "steps_per_launch": spl, "launch_shape": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(*k_rows),
                      "note": ("algorithmic 8 B per site-update x sites x steps per launch / "
                               "average launch time; with several steps fused per launch the "
                               "fraction can exceed 1")},

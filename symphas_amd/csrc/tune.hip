@@ -80,7 +80,7 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
     return SPS_E_INVALID;
   }
   auto report = [&]() {
-    if (out_k) *out_k = p->K;
+    if (out_k) *out_k = p->steps_per_launch();
     if (out_rows) *out_rows = p->TY;
     if (out_planes) *out_planes = p->d.dim == 3 ? p->TZ : 0;
     return SPS_OK;
