@@ -43,6 +43,12 @@ def parse():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the other configurations (3D, batched, surrogate)")
     ap.add_argument("--surrogate-iters", type=int, default=300)
+    ap.add_argument("--values-per-lane", type=int, default=0)
+    ap.add_argument("--shape-sweep", action="store_true",
+                    help="only run every launch shape the 2D autotuner tries (steps per "
+                         "launch 1..4 x [4 values per lane x rows 8..64, 2 values per lane "
+                         "x rows 16..128], 6 launches each, in that order) on the 4096^2 "
+                         "Model B field, for per-shape profiles")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the known-size "
                          "transfer that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -61,16 +67,38 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": 4 * n, "repeats": 5}), flush=True)
 
 
-def profiled_traffic(spl, rows):
+def shape_sweep():
+    import numpy as np
+    import torch
+    from symphas_amd import abi, inputs, models
+    from symphas_amd.solver import FDPlan
+    torch.cuda.set_device(0)
+    psi0 = inputs.uniform((N, N), seed=SEED, dtype=np.float32)
+    d32 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
+    order = []
+    with FDPlan(d32) as plan:
+        plan.upload(psi0)
+        for k in (1, 2, 3, 4):
+            for vec, rows_set in ((4, (8, 16, 32, 64)), (2, (16, 32, 64, 128))):
+                for rows in rows_set:
+                    plan.tune(k, rows, 0, vec)
+                    plan.step(6 * k)  # six launches, no graph
+                    plan.sync()
+                    order.append([k, rows, vec])
+    print(json.dumps({"shape_sweep": order, "launches_per_shape": 6}), flush=True)
+
+
+def profiled_traffic(spl, rows, vec):
     """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled
     by the copy calibration) for the same kernel and launch shape, from the
-    committed profile summary; None when the shapes differ."""
+    committed profile summary; None for a shape it does not list."""
     path = ROOT / "profiles" / "fused2d_modelb_4096_f32.md"
     try:
         for line in path.read_text().splitlines():
             if line.startswith("machine-readable:"):
                 kv = dict(x.strip().split(" = ") for x in line.split(":", 1)[1].split(";"))
-                if int(kv["steps_per_launch"]) == spl and int(kv["rows_per_tile"]) == rows:
+                if (int(kv["steps_per_launch"]) == spl and int(kv["rows_per_tile"]) == rows
+                        and int(kv["values_per_lane"]) == vec):
                     return float(kv["traffic_bytes_per_launch"])
     except (OSError, ValueError, KeyError):
         return None
@@ -124,7 +152,8 @@ def extra_cfg3(torch, stream, steps=200):
     g = n ** 3 * steps / (ms / 1e3) / 1e9
     return {"workload": "Model B 3D 512^3 FP32, Stencil3d2h<7,10,25>, dt=0.005",
             "steps": steps, "Gsite_updates_per_s": g,
-            "launch_shape": {"rows_per_wave": shape[1], "planes_per_chunk": shape[2]},
+            "launch_shape": {"steps_per_launch": shape[0], "rows_per_wave": shape[1],
+                             "planes_per_chunk": shape[2]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
             "mass_drift_rel": abs(m1 - m0) / float(np.abs(psi.astype(np.float64)).sum())}
 
@@ -282,6 +311,9 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
+    if args.shape_sweep:
+        shape_sweep()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
@@ -304,13 +336,14 @@ def main():
     plan = FDPlan(d32)
     plan.set_stream(stream.cuda_stream)
     plan.upload(psi0)
-    if args.steps_per_launch or args.rows_per_tile:
-        plan.tune(args.steps_per_launch, args.rows_per_tile, 0)
+    if args.steps_per_launch or args.rows_per_tile or args.values_per_lane:
+        plan.tune(args.steps_per_launch, args.rows_per_tile, 0, args.values_per_lane)
+        shape = (args.steps_per_launch, args.rows_per_tile, 0, args.values_per_lane)
         tuned = "fixed by flags"
     else:
         shape = plan.autotune(use_cache=False)
-        tuned = "autotuned: %d steps per launch, %d rows per chunk" % shape[:2]
-    k_rows = plan.query()[1], (args.rows_per_tile or (shape[1] if not args.steps_per_launch else 0))
+        tuned = ("autotuned: %d steps per launch, %d rows per chunk, %d values per lane"
+                 % (shape[0], shape[1], shape[3]))
     m0, e0 = plan.reduce()
     path, spl = plan.query()
 
@@ -405,7 +438,7 @@ def main():
                    "steps_per_launch": spl, "launch_shape": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(*k_rows),
+                     "traffic": profiled_traffic(shape[0], shape[1], shape[3]),
                      "note": ("algorithmic 8 B per site-update x sites x steps per launch / "
                               "average launch time; with several steps fused per launch the "
                               "fraction can exceed 1")},

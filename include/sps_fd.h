@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SPS_FD_ABI_VERSION 1
+#define SPS_FD_ABI_VERSION 2
 
 /* Ghost width of the SymPhas padded layout (datatypes/inc/gridinfo.h:46). */
 #define SPS_BOUNDARY_DEPTH 3
@@ -207,25 +207,31 @@ int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 
 /* Override the fused path's launch shape (0 keeps the current value).
  * 2D: steps_per_launch in 1..4, rows_per_tile = rows per streamed chunk, a
- *     multiple of 4 in 4..512; planes_per_chunk must be 0.
- * 3D: steps_per_launch must be 0 or 1; rows_per_tile = y rows per wave of the
- *     register-streaming kernel (8 or 16; 4 selects the LDS plane-ring
- *     kernel); planes_per_chunk = z planes per streamed chunk, 1..1024. */
+ *     multiple of 4 in 4..512; planes_per_chunk must be 0; values_per_lane =
+ *     consecutive x values per lane, 16 or 8 bytes' worth (FP32: 4 or 2,
+ *     FP64: 2 or 1), dividing n[0].
+ * 3D: (steps_per_launch, rows_per_tile) = steps fused per launch and y rows
+ *     per wave, one of (1, 4), (1, 8), (1, 16), (2, 4) and, FP32 only, (2, 8);
+ *     planes_per_chunk = z planes per streamed chunk, 1..1024;
+ *     values_per_lane must be 0 or 1. */
 int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile,
-                int planes_per_chunk);
+                int planes_per_chunk, int values_per_lane);
 
 /* Pick the fused path's launch shape by timing candidates on the device:
- * 2D steps_per_launch {1,2,3,4} x rows_per_tile {8,16,32,64}; 3D rows
- * {4 (LDS kernel), 8, 16} x planes {8,16,32,64}.  The field is saved before
- * and restored after the trial launches, so the plan's state is unchanged.
- * When the environment variable SPS_FD_TUNE_CACHE names a file, a choice
- * recorded there for the same (dim, shape, dtype, lowered form, GPU) is used
- * without timing (unless use_cache is 0) and new choices are appended.
- * The chosen shape is applied to the plan and returned through the optional
- * pointers.  Plans on the reference path have nothing to tune (SPS_OK).
+ * 2D steps_per_launch {1,2,3,4} x (16-byte lanes, rows_per_tile
+ * {8,16,32,64} | 8-byte lanes, rows {16,32,64,128}); 3D (steps, rows) as
+ * sps_fd_tune allows x planes {8, 16, 32} and n[2] split into 1..8 equal
+ * chunks.  The field is saved before and restored after the trial launches,
+ * so the plan's state is unchanged.  When the environment variable
+ * SPS_FD_TUNE_CACHE names a file, a choice recorded there for the same (dim,
+ * shape, dtype, lowered form, GPU) is used without timing (unless use_cache
+ * is 0) and new choices are appended.  The chosen shape is applied to the
+ * plan and returned through the optional pointers (values_per_lane is 1 in
+ * 3D).  Plans on the reference path have nothing to tune (SPS_OK).
  * No counterpart in the reference (its CPU loops have no launch shape). */
 int sps_fd_autotune(sps_fd_plan* plan, int use_cache, int* steps_per_launch,
-                    int* rows_per_tile, int* planes_per_chunk);
+                    int* rows_per_tile, int* planes_per_chunk,
+                    int* values_per_lane);
 
 /* ---------------------------------------------------------------- batched */
 

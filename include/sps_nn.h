@@ -52,7 +52,7 @@ int sps_nn_conv3x3_wgrad(const void* x, const void* dpre, int n, int h, int w,
 
 /* First layer, one FP32 input channel:
  *   pre[n,h,w,k] = bias[k] + sum_t x[n, shift_t(h,w)] * weight[t][k],
- *   y = act(pre) (BF16), weight [9][cout] FP32, cout <= 64. */
+ *   y = act(pre) (BF16), weight [9][cout] FP32, cout % 8 == 0, cout <= 64. */
 int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
                        const float* weight, const float* bias, int cout,
                        int act, void* y, void* pre, void* stream);
@@ -62,7 +62,7 @@ int sps_nn_conv3x3_in1_wgrad(const float* x, const void* dpre, int n, int h,
 
 /* Last layer, one FP32 output channel:
  *   y[n,h,w] = bias[0] + sum_{t,c} x[n, shift_t(h,w), c] * weight[t][c],
- *   weight [9][cin] FP32, cin <= 64. */
+ *   weight [9][cin] FP32, cin % 8 == 0, cin <= 64. */
 int sps_nn_conv3x3_out1(const void* x, int n, int h, int w, int cin,
                         const float* weight, const float* bias, float* y,
                         void* stream);

@@ -79,8 +79,9 @@ def load() -> ctypes.CDLL:
         "sps_fd_copy_device": (c_int, [P, P, c_int]),
         "sps_fd_batch_copy_device": (c_int, [P, c_int, P, c_int]),
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
-        "sps_fd_tune": (c_int, [P, c_int, c_int, c_int]),
-        "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
+        "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                                    POINTER(c_int)]),
         "sps_fd_batch_autotune": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
         "sps_fd_batch_create": (c_int, [POINTER(Desc), c_int, POINTER(P)]),
         "sps_fd_batch_destroy": (c_int, [P]),

@@ -284,22 +284,22 @@ int sps_fd_plan::launch_one(int k) {
   } else if (d.dim == 2) {
     rc = d.dtype == SPS_F32
              ? launch_fused2d<float>((const float*)in, (float*)out, d.n[0],
-                                     d.n[1], k, TY, fc32, stream)
+                                     d.n[1], k, TY, VEC, fc32, stream)
              : launch_fused2d<double>((const double*)in, (double*)out, d.n[0],
-                                      d.n[1], k, TY, fc64, stream);
+                                      d.n[1], k, TY, VEC, fc64, stream);
   } else {
     rc = d.dtype == SPS_F32
              ? launch_fused3d<float>((const float*)in, (float*)out, d.n[0],
-                                     d.n[1], d.n[2], TY, TZ, fc32, stream)
+                                     d.n[1], d.n[2], k, TY, TZ, fc32, stream)
              : launch_fused3d<double>((const double*)in, (double*)out, d.n[0],
-                                      d.n[1], d.n[2], TY, TZ, fc64, stream);
+                                      d.n[1], d.n[2], k, TY, TZ, fc64, stream);
   }
   if (rc == SPS_OK) cur = 1 - cur;
   return rc;
 }
 
 int sps_fd_plan::steps_per_launch() const {
-  return (path == SPS_PATH_FUSED && d.dim == 2) ? K : 1;
+  return path == SPS_PATH_FUSED ? K : 1;
 }
 
 void sps_fd_plan::drop_graphs() {
@@ -389,7 +389,7 @@ int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
   }
   p->d = *desc;
   p->low = lower(*desc);
-  const int vec = desc->dtype == SPS_F32 ? 4 : 2;
+  const int vec = desc->dtype == SPS_F32 ? 2 : 1;  // narrowest lane width
   const bool fusable =
       p->low.fusable && (desc->dim == 3 || desc->n[0] % vec == 0);
   if (desc->path == SPS_PATH_FUSED && !fusable) {
@@ -406,9 +406,9 @@ int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
   make_ref_params(*desc, p->rp64);
   make_fused_coef(*desc, p->low, p->fc32);
   make_fused_coef(*desc, p->low, p->fc64);
-  p->choose_default_shape();
   p->nsites = sites_of(*desc);
   p->esize = desc->dtype == SPS_F32 ? 4 : 8;
+  p->choose_default_shape();
   int rc = p->allocate();
   if (rc != SPS_OK) {
     sps_fd_destroy(p);
@@ -650,31 +650,39 @@ int sps_fd_query(const sps_fd_plan* p, int* path, int* spl) {
 }
 
 int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile,
-                int planes_per_chunk) {
+                int planes_per_chunk, int values_per_lane) {
   if (!p) {
     set_error("plan is NULL");
     return SPS_E_INVALID;
   }
   if (p->d.dim == 2) {
+    const int wide = 16 / (int)p->esize, narrow = 8 / (int)p->esize;
+    const int vec = values_per_lane ? values_per_lane : p->VEC;
     if (steps_per_launch < 0 || steps_per_launch > 4 || planes_per_chunk != 0 ||
         (rows_per_tile != 0 &&
-         (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4))) {
+         (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4)) ||
+        (vec != wide && vec != narrow) || p->d.n[0] % vec != 0) {
       set_error("tune (2D): steps_per_launch in 0..4, rows_per_tile 0 or a "
-                "multiple of 4 in 4..512, planes_per_chunk 0");
+                "multiple of 4 in 4..512, planes_per_chunk 0, values_per_lane "
+                "0, %d or %d dividing the x extent", wide, narrow);
       return SPS_E_INVALID;
     }
     if (steps_per_launch) p->K = steps_per_launch;
     if (rows_per_tile) p->TY = rows_per_tile;
+    p->VEC = vec;
   } else {
-    if (steps_per_launch < 0 || steps_per_launch > 1 ||
-        (rows_per_tile != 0 && rows_per_tile != 4 && rows_per_tile != 8 &&
-         rows_per_tile != 16) ||
-        planes_per_chunk < 0 || planes_per_chunk > 1024) {
-      set_error("tune (3D): steps_per_launch 0 or 1, rows_per_tile 0, 4, 8 "
-                "or 16, planes_per_chunk in 0..1024");
+    const int k = steps_per_launch ? steps_per_launch : p->K;
+    const int rows = rows_per_tile ? rows_per_tile : p->TY;
+    if (steps_per_launch < 0 || rows_per_tile < 0 || planes_per_chunk < 0 ||
+        planes_per_chunk > 1024 || (values_per_lane != 0 && values_per_lane != 1) ||
+        !sps::fused3d_shape_ok((int)p->esize, k, rows)) {
+      set_error("tune (3D): (steps_per_launch, rows_per_tile) in (1, 4|8|16), "
+                "(2, 4) or, FP32 only, (2, 8); planes_per_chunk in 0..1024; "
+                "values_per_lane 0 or 1");
       return SPS_E_INVALID;
     }
-    if (rows_per_tile) p->TY = rows_per_tile;
+    p->K = k;
+    p->TY = rows;
     if (planes_per_chunk) p->TZ = planes_per_chunk;
   }
   p->drop_graphs();

@@ -5,14 +5,15 @@
 namespace sps {
 namespace {
 
-// Halo layout: VEC = 16 bytes per lane, HL halo lanes per strip side.
-template <typename T, int K, int R, bool HASA, int POLY>
+// Halo layout: VEC values per lane (16 or 8 bytes), HL halo lanes per strip
+// side.  Halving VEC doubles the strips, so the same number of waves covers
+// chunks twice as tall, whose pipeline fill (2*K*R rows) costs less.
+template <typename T, int VEC, int K, int R, bool HASA, int POLY>
 __global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
                                                     T* __restrict__ out, int X,
                                                     int Y, int TY, int nstrips,
                                                     int nitems,
                                                     FusedCoef<T> c) {
-  constexpr int VEC = 16 / sizeof(T);
   constexpr int HL = (K * R + VEC - 1) / VEC;
   constexpr int W = (64 - 2 * HL) * VEC;
   const int lane = threadIdx.x & 63;
@@ -31,63 +32,66 @@ __global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
                                                store_lane, c);
 }
 
-template <typename T, int K, int R, bool HASA, int POLY>
+template <typename T, int VEC, int K, int R, bool HASA, int POLY>
 int launch_halo(const T* in, T* out, int X, int Y, int TY,
                 const FusedCoef<T>& c, hipStream_t s) {
-  constexpr int VEC = 16 / sizeof(T);
   constexpr int HL = (K * R + VEC - 1) / VEC;
   constexpr int W = (64 - 2 * HL) * VEC;
   const int nstrips = (X + W - 1) / W;
   const int nchunks = (Y + TY - 1) / TY;
   const int nitems = nstrips * nchunks;
   const int nblocks = (nitems + 3) / 4;
-  fused2d_halo<T, K, R, HASA, POLY>
+  fused2d_halo<T, VEC, K, R, HASA, POLY>
       <<<nblocks, 256, 0, s>>>(in, out, X, Y, TY, nstrips, nitems, c);
   return SPS_OK;
 }
 
 template <typename T, int K, int R, bool HASA, int POLY>
-int dispatch_layout(const T* in, T* out, int X, int Y, int TY,
+int dispatch_layout(const T* in, T* out, int X, int Y, int TY, int vec,
                     const FusedCoef<T>& c, hipStream_t s) {
-  return launch_halo<T, K, R, HASA, POLY>(in, out, X, Y, TY, c, s);
+  constexpr int WIDE = 16 / sizeof(T), NARROW = 8 / sizeof(T);
+  return vec == NARROW
+             ? launch_halo<T, NARROW, K, R, HASA, POLY>(in, out, X, Y, TY, c, s)
+             : launch_halo<T, WIDE, K, R, HASA, POLY>(in, out, X, Y, TY, c, s);
 }
 
 template <typename T, int K, int R>
-int dispatch_poly(const T* in, T* out, int X, int Y, int TY,
+int dispatch_poly(const T* in, T* out, int X, int Y, int TY, int vec,
                   const FusedCoef<T>& c, hipStream_t s) {
   const bool hasa = c.dega >= 0;
   const int deg = c.dega > c.degb ? c.dega : c.degb;
   if (deg <= 3) {
-    return hasa ? dispatch_layout<T, K, R, true, 3>(in, out, X, Y, TY, c, s)
-                : dispatch_layout<T, K, R, false, 3>(in, out, X, Y, TY, c, s);
+    return hasa ? dispatch_layout<T, K, R, true, 3>(in, out, X, Y, TY, vec, c, s)
+                : dispatch_layout<T, K, R, false, 3>(in, out, X, Y, TY, vec, c, s);
   }
-  return hasa ? dispatch_layout<T, K, R, true, 6>(in, out, X, Y, TY, c, s)
-              : dispatch_layout<T, K, R, false, 6>(in, out, X, Y, TY, c, s);
+  return hasa ? dispatch_layout<T, K, R, true, 6>(in, out, X, Y, TY, vec, c, s)
+              : dispatch_layout<T, K, R, false, 6>(in, out, X, Y, TY, vec, c, s);
 }
 
 template <typename T, int K>
-int dispatch_radius(const T* in, T* out, int X, int Y, int TY,
+int dispatch_radius(const T* in, T* out, int X, int Y, int TY, int vec,
                     const FusedCoef<T>& c, hipStream_t s) {
-  return c.radius == 1 ? dispatch_poly<T, K, 1>(in, out, X, Y, TY, c, s)
-                       : dispatch_poly<T, K, 2>(in, out, X, Y, TY, c, s);
+  return c.radius == 1 ? dispatch_poly<T, K, 1>(in, out, X, Y, TY, vec, c, s)
+                       : dispatch_poly<T, K, 2>(in, out, X, Y, TY, vec, c, s);
 }
 
 }  // namespace
 
 template <typename T>
-int launch_fused2d(const T* in, T* out, int X, int Y, int K, int TY,
+int launch_fused2d(const T* in, T* out, int X, int Y, int K, int TY, int vec,
                    const FusedCoef<T>& c, hipStream_t s) {
-  if (X % (16 / (int)sizeof(T)) != 0 || TY < 1) {
-    set_error("fused 2D step: X=%d must be a multiple of %d", X,
-              16 / (int)sizeof(T));
+  const int wide = 16 / (int)sizeof(T), narrow = 8 / (int)sizeof(T);
+  if ((vec != wide && vec != narrow) || X % vec != 0 || TY < 1) {
+    set_error("fused 2D step: %d values per lane (%d or %d) must divide X=%d",
+              vec, wide, narrow, X);
     return SPS_E_UNSUPPORTED;
   }
   int rc;
   switch (K) {
-    case 1: rc = dispatch_radius<T, 1>(in, out, X, Y, TY, c, s); break;
-    case 2: rc = dispatch_radius<T, 2>(in, out, X, Y, TY, c, s); break;
-    case 3: rc = dispatch_radius<T, 3>(in, out, X, Y, TY, c, s); break;
-    case 4: rc = dispatch_radius<T, 4>(in, out, X, Y, TY, c, s); break;
+    case 1: rc = dispatch_radius<T, 1>(in, out, X, Y, TY, vec, c, s); break;
+    case 2: rc = dispatch_radius<T, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 3: rc = dispatch_radius<T, 3>(in, out, X, Y, TY, vec, c, s); break;
+    case 4: rc = dispatch_radius<T, 4>(in, out, X, Y, TY, vec, c, s); break;
     default:
       set_error("fused 2D step: steps per launch %d not in 1..4", K);
       return SPS_E_INVALID;
@@ -102,8 +106,8 @@ int launch_fused2d(const T* in, T* out, int X, int Y, int K, int TY,
 }
 
 template int launch_fused2d<float>(const float*, float*, int, int, int, int,
-                                   const FusedCoef<float>&, hipStream_t);
+                                   int, const FusedCoef<float>&, hipStream_t);
 template int launch_fused2d<double>(const double*, double*, int, int, int, int,
-                                    const FusedCoef<double>&, hipStream_t);
+                                    int, const FusedCoef<double>&, hipStream_t);
 
 }  // namespace sps

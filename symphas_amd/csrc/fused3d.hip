@@ -1,13 +1,25 @@
-// fused3d.hip - fused 3D step: 2.5-D streaming along z.
+// fused3d.hip - fused 3D steps: register streaming along z.
 //
-// A workgroup owns a 64 x TY tile of the (x, y) plane and streams the planes
-// of one z-chunk.  Planes of psi (tile + 2-cell halo) and of mu (tile +
-// 1-cell halo) live in rings of four LDS slots, so each plane needs two
-// barriers: psi(i) loaded -> mu(i-1) computed -> psi'(i-2) written.  With the
-// 7-point laplacian, lap7∘lap7 is the 25-point bilaplacian of Stencil3d2h
-// <7,*,25> exactly, so the canonical form
+// One wave owns a column block of 64 lanes (one x column per lane) by TYW
+// rows and streams the planes of one z-chunk.  K steps are fused per launch
+// as a chain of radius-one stages, each holding a ring of three planes in
+// registers: x neighbours come from adjacent lanes (DPP), y and z neighbours
+// are registers, and there is no LDS and no barrier.
+//
+// With the 7-point laplacian, lap7∘lap7 is the 25-point bilaplacian of
+// Stencil3d2h<7,*,25> exactly, so a radius-two step is the canonical form
 //   mu = B(psi) + kmu*D7(psi),  psi' = psi + Adt(psi) + kd*D7(mu)
-// is the reference's -bilap psi - lap psi + lap psi^3 up to rounding.
+// (two stages: mu, then update), which is the reference's
+// -bilap psi - lap psi + lap psi^3 up to rounding; a radius-one step is one
+// stage psi' = psi + Adt(psi) + k1*D7(psi).  Every stage is evaluated with
+// the same expression whatever K, so K fused steps are bitwise K launches.
+//
+// Plane pipeline: while input plane i is loaded, stage s produces plane i - s
+// of its output from planes i-s-1 .. i-s+1 of its input.  Input plane 0 lies
+// R*K planes below the chunk and the NS stages total radius R*K, so the last
+// stage stores chunk plane i - 2*NS.  Rows shrink by one per stage on each
+// side (TYW + 2*R*K rows are loaded) and so do the valid lanes (R*K halo
+// lanes per side are loaded but not stored).
 #include "fused.hpp"
 #include "lanes.hpp"
 
@@ -16,134 +28,32 @@
 namespace sps {
 namespace {
 
-constexpr int kTX = 64;
-
-template <typename T, int TY, int R, bool HASA, int POLY>
-__global__ __launch_bounds__(256) void fused3d_kernel(
-    const T* __restrict__ in, T* __restrict__ out, int X, int Y, int Z, int TZ,
-    int nbx, int nby, FusedCoef<T> c) {
-  constexpr int PX = kTX + 4, PY = TY + 4;
-  constexpr int MX = kTX + 2, MY = TY + 2;
-  constexpr int NLOAD = (PX * PY + 255) / 256;
-  constexpr int NMU = (MX * MY + 255) / 256;
-  __shared__ T sp[4][PY][PX];
-  __shared__ T sm[R == 2 ? 4 : 1][R == 2 ? MY : 1][R == 2 ? MX : 1];
-
-  const int tid = threadIdx.x;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int bx = b % nbx;
-  const int by = (b / nbx) % nby;
-  const int bz = b / (nbx * nby);
-  const int x0 = bx * kTX, y0 = by * TY, z0 = bz * TZ;
-  const int tz = min(TZ, Z - z0);
-  const int64_t XY = (int64_t)X * Y;
-
-  // in-plane offsets of the loads this thread does for every plane
-  int64_t loff[NLOAD];
-#pragma unroll
-  for (int q = 0; q < NLOAD; ++q) {
-    const int e = tid + 256 * q;
-    const int ly = e / PX, lx = e % PX;
-    int gx = (x0 + lx - 2) % X;
-    if (gx < 0) gx += X;
-    int gy = (y0 + ly - 2) % Y;
-    if (gy < 0) gy += Y;
-    loff[q] = (int64_t)gy * X + gx;
-  }
-
-  const int NI = tz + 2 * R;
-  int zl = (z0 - R) % Z;
-  if (zl < 0) zl += Z;
-  for (int i = 0; i < NI; ++i) {
-    // A: psi plane of stream index i
-    {
-      T* dst = &sp[i & 3][0][0];
-      const T* src = in + (int64_t)zl * XY;
-#pragma unroll
-      for (int q = 0; q < NLOAD; ++q) {
-        const int e = tid + 256 * q;
-        if (e < PX * PY) dst[e] = src[loff[q]];
-      }
-      zl = zl + 1 == Z ? 0 : zl + 1;
-    }
-    __syncthreads();
-    if constexpr (R == 2) {
-      // B: mu of plane i-1 on the tile + 1 halo
-      if (i >= 2) {
-        const int sc = (i - 1) & 3, sd = (i - 2) & 3, su = i & 3;
-#pragma unroll
-        for (int q = 0; q < NMU; ++q) {
-          const int e = tid + 256 * q;
-          if (e < MX * MY) {
-            const int my = e / MX, mx = e % MX;
-            const int py = my + 1, px = mx + 1;
-            const T ctr = sp[sc][py][px];
-            const T d = fmad(T(-6), ctr,
-                             ((sp[sc][py][px - 1] + sp[sc][py][px + 1]) +
-                              (sp[sc][py - 1][px] + sp[sc][py + 1][px])) +
-                                 (sp[sd][py][px] + sp[su][py][px]));
-            sm[sc][my][mx] = fmad(c.kmu, d, horner<POLY>(c.b, ctr));
-          }
-        }
-      }
-      __syncthreads();
-      // C: psi' of plane i-2
-      if (i >= 4) {
-        const int zo = z0 + i - 4;
-        const int mc = (i - 2) & 3, md = (i - 3) & 3, mu = (i - 1) & 3;
-        const int tx = tid & 63;
-#pragma unroll
-        for (int m = 0; m < TY / 4; ++m) {
-          const int ty = (tid >> 6) + 4 * m;
-          const int my = ty + 1, mx = tx + 1;
-          const T d = fmad(T(-6), sm[mc][my][mx],
-                           ((sm[mc][my][mx - 1] + sm[mc][my][mx + 1]) +
-                            (sm[mc][my - 1][mx] + sm[mc][my + 1][mx])) +
-                               (sm[md][my][mx] + sm[mu][my][mx]));
-          T base = sp[mc][ty + 2][tx + 2];
-          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-          const T v = fmad(c.kd, d, base);
-          if (x0 + tx < X && y0 + ty < Y)
-            out[(int64_t)zo * XY + (int64_t)(y0 + ty) * X + (x0 + tx)] = v;
-        }
-      }
-    } else {
-      // C (radius one): psi' of plane i-1 straight from psi
-      if (i >= 2) {
-        const int zo = z0 + i - 2;
-        const int pc = (i - 1) & 3, pd = (i - 2) & 3, pu = i & 3;
-        const int tx = tid & 63;
-#pragma unroll
-        for (int m = 0; m < TY / 4; ++m) {
-          const int ty = (tid >> 6) + 4 * m;
-          const int py = ty + 2, px = tx + 2;
-          const T ctr = sp[pc][py][px];
-          const T d = fmad(T(-6), ctr,
-                           ((sp[pc][py][px - 1] + sp[pc][py][px + 1]) +
-                            (sp[pc][py - 1][px] + sp[pc][py + 1][px])) +
-                               (sp[pd][py][px] + sp[pu][py][px]));
-          T base = ctr;
-          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-          const T v = fmad(c.k1, d, base);
-          if (x0 + tx < X && y0 + ty < Y)
-            out[(int64_t)zo * XY + (int64_t)(y0 + ty) * X + (x0 + tx)] = v;
-        }
-      }
-    }
-  }
+// radius-one 7-point difference (unscaled) at output row r of a ring whose
+// plane S1 is centred on row r + 1
+template <int S0, int S1, int S2, typename T, int NI>
+__device__ __forceinline__ T d7(const T (&a)[3][NI], int r) {
+  const T ctr = a[S1][r + 1];
+  const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+  return fmad(T(-6), ctr,
+              ((xl + xr) + (a[S1][r] + a[S1][r + 2])) +
+                  (a[S0][r + 1] + a[S2][r + 1]));
 }
 
-// v2: one wave owns a 64-lane x TYW-row column block and streams the planes
-// of one z-chunk through register windows; x neighbours come from adjacent
-// lanes (DPP), y and z neighbours are registers.  No LDS, no barriers.
-template <typename T, int TYW, int R, bool HASA, int POLY>
+template <int N>
+struct RowsOf {
+  static constexpr int value = N > 0 ? N : 1;
+};
+
+template <typename T, int TYW, int R, int K, bool HASA, int POLY>
 __global__ __launch_bounds__(256) void fused3d_stream(
     const T* __restrict__ in, T* __restrict__ out, int X, int Y, int Z, int TZ,
     int nstrips, int nrowblk, int nitems, FusedCoef<T> c) {
-  constexpr int HL = R;  // halo lanes (one column per lane)
+  constexpr int RT = R * K;                 // radius of K steps
+  constexpr int NS = (R == 2 ? 2 : 1) * K;  // radius-one stages
+  static_assert(NS >= 1 && NS <= 4, "at most four stages");
+  constexpr int HL = RT;  // halo lanes per side
   constexpr int W = 64 - 2 * HL;
-  constexpr int NP = TYW + 2 * R;  // psi rows held per plane
-  constexpr int NM = TYW + 2;      // mu rows per plane (radius 2 only)
+  constexpr int NP = TYW + 2 * RT;  // rows of the loaded planes
   const int lane = threadIdx.x & 63;
   const int item = __builtin_amdgcn_readfirstlane(
       xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
@@ -164,23 +74,29 @@ __global__ __launch_bounds__(256) void fused3d_stream(
   int rowoff[NP];
 #pragma unroll
   for (int r = 0; r < NP; ++r) {
-    int y = (y0 - R + r) % Y;
+    int y = (y0 - RT + r) % Y;
     if (y < 0) y += Y;
     rowoff[r] = __builtin_amdgcn_readfirstlane(y * X);
   }
-  const int NI = tz + 2 * R;
-  int zl = (z0 - R) % Z;
+  const int NI = tz + 2 * RT;
+  int zl = (z0 - RT) % Z;
   if (zl < 0) zl += Z;
 
-  T P[3][NP];
-  T M[3][R == 2 ? NM : 1];
+  // rings: A0 = psi planes, A(s) = output planes of stage s < NS
+  T A0[3][NP];
+  T A1[3][RowsOf<NS >= 2 ? NP - 2 : 0>::value];
+  T A2[3][RowsOf<NS >= 3 ? NP - 4 : 0>::value];
+  T A3[3][RowsOf<NS >= 4 ? NP - 6 : 0>::value];
+  auto zero = [](auto& a) {
 #pragma unroll
-  for (int s = 0; s < 3; ++s) {
+    for (int q = 0; q < 3; ++q)
 #pragma unroll
-    for (int r = 0; r < NP; ++r) P[s][r] = T(0);
-#pragma unroll
-    for (int r = 0; r < (R == 2 ? NM : 1); ++r) M[s][r] = T(0);
-  }
+      for (int r = 0; r < (int)(sizeof(a[0]) / sizeof(T)); ++r) a[q][r] = T(0);
+  };
+  zero(A0);
+  zero(A1);
+  zero(A2);
+  zero(A3);
 
   // the next plane is loaded one phase ahead into NX, so each wave's loads
   // overlap the arithmetic of the plane before
@@ -192,59 +108,63 @@ __global__ __launch_bounds__(256) void fused3d_stream(
     zl = zl + 1 == Z ? 0 : zl + 1;
   };
   fetch();
-  auto phase = [&](auto PHc, int i) {
+
+  // stage s (1-based) reads ring `a` (its input) and, for an update stage of
+  // a radius-two step, ring `b` (psi of the step, two stages back)
+  auto stage = [&](auto sc, auto PHc, auto& a, auto& b, auto& o, int i) {
+    constexpr int S = decltype(sc)::value;
     constexpr int PH = decltype(PHc)::value;
     constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;
-    {
-#pragma unroll
-      for (int r = 0; r < NP; ++r) P[S2][r] = NX[r];
-      if (i + 1 < NI) fetch();
-    }
-    if constexpr (R == 2) {
-      // mu of plane i-1, rows y0-1 .. y0+TYW (psi rows 1 .. TYW+2)
-#pragma unroll
-      for (int r = 1; r <= TYW + 2; ++r) {
-        const T ctr = P[S1][r];
-        const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
-        const T d = fmad(T(-6), ctr,
-                         ((xl + xr) + (P[S1][r - 1] + P[S1][r + 1])) +
-                             (P[S0][r] + P[S2][r]));
-        M[S2][r - 1] = fmad(c.kmu, d, horner<POLY>(c.b, ctr));
-      }
-      if (i >= 4) {
-        const int zo = z0 + i - 4;
-        T* dst = out + (int64_t)zo * XY;
-#pragma unroll
-        for (int m = 1; m <= TYW; ++m) {
-          const T ctr = M[S1][m];
-          const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
-          const T d = fmad(T(-6), ctr,
-                           ((xl + xr) + (M[S1][m - 1] + M[S1][m + 1])) +
-                               (M[S0][m] + M[S2][m]));
-          T base = P[S0][m + 1];
-          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-          const T v = fmad(c.kd, d, base);
-          if (store_lane && y0 + m - 1 < Y) (dst + rowoff[m + 1])[col] = v;
-        }
-      }
+    constexpr int NO = NP - 2 * S;  // rows produced
+    constexpr bool MU = R == 2 && (S % 2 == 1);
+    constexpr bool LAST = S == NS;
+    if constexpr (LAST) {
+      if (i < 2 * NS) return;
     } else {
-      if (i >= 2) {
-        const int zo = z0 + i - 2;
-        T* dst = out + (int64_t)zo * XY;
+      // input planes are valid from i = 2*(S-1) on; earlier ones would only
+      // fill the ring with garbage that no valid output reads
+      if (i < 2 * (S - 1)) return;
+    }
+    const int zo = z0 + i - 2 * NS;
+    T* dst = out + (int64_t)zo * XY;
 #pragma unroll
-        for (int r = 1; r <= TYW; ++r) {
-          const T ctr = P[S1][r];
-          const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
-          const T d = fmad(T(-6), ctr,
-                           ((xl + xr) + (P[S1][r - 1] + P[S1][r + 1])) +
-                               (P[S0][r] + P[S2][r]));
-          T base = ctr;
-          if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-          const T v = fmad(c.k1, d, base);
-          if (store_lane && y0 + r - 1 < Y) (dst + rowoff[r])[col] = v;
-        }
+    for (int r = 0; r < NO; ++r) {
+      const T d = d7<S0, S1, S2>(a, r);
+      T v;
+      if constexpr (MU) {
+        v = fmad(c.kmu, d, horner<POLY>(c.b, a[S1][r + 1]));
+      } else {
+        T base;
+        if constexpr (R == 2)
+          base = b[S0][r + 2];
+        else
+          base = a[S1][r + 1];
+        if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
+        v = fmad(R == 2 ? c.kd : c.k1, d, base);
+      }
+      if constexpr (LAST) {
+        if (store_lane && y0 + r < Y) (dst + rowoff[r + RT])[col] = v;
+      } else {
+        o[S2][r] = v;
       }
     }
+  };
+
+  auto phase = [&](auto PHc, int i) {
+    constexpr int PH = decltype(PHc)::value;
+    constexpr int S2 = (PH + 2) % 3;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) A0[S2][r] = NX[r];
+    if (i + 1 < NI) fetch();
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    // ring arguments that a stage does not read are passed as A0
+    stage(I1{}, PHc, A0, A0, A1, i);
+    if constexpr (NS >= 2) stage(I2{}, PHc, A1, A0, A2, i);
+    if constexpr (NS >= 3) stage(I3{}, PHc, A2, A1, A3, i);
+    if constexpr (NS >= 4) stage(I4{}, PHc, A3, A2, A0, i);
   };
   for (int i = 0; i < NI; i += 3) {
     phase(std::integral_constant<int, 0>{}, i);
@@ -253,65 +173,73 @@ __global__ __launch_bounds__(256) void fused3d_stream(
   }
 }
 
-template <typename T, int TYW, int R, bool HASA, int POLY>
+template <typename T, int TYW, int R, int K, bool HASA, int POLY>
 int launch3s(const T* in, T* out, int X, int Y, int Z, int TZ,
              const FusedCoef<T>& c, hipStream_t s) {
-  constexpr int W = 64 - 2 * R;
+  constexpr int W = 64 - 2 * R * K;
   const int nstrips = (X + W - 1) / W;
   const int nrowblk = (Y + TYW - 1) / TYW;
   const int nzc = (Z + TZ - 1) / TZ;
-  const int nitems = nstrips * nrowblk * nzc;
-  fused3d_stream<T, TYW, R, HASA, POLY><<<(nitems + 3) / 4, 256, 0, s>>>(
-      in, out, X, Y, Z, TZ, nstrips, nrowblk, nitems, c);
-  return SPS_OK;
-}
-
-template <typename T, int TY, int R, bool HASA, int POLY>
-int launch3(const T* in, T* out, int X, int Y, int Z, int TZ,
-            const FusedCoef<T>& c, hipStream_t s) {
-  const int nbx = (X + kTX - 1) / kTX;
-  const int nby = (Y + TY - 1) / TY;
-  const int nbz = (Z + TZ - 1) / TZ;
-  fused3d_kernel<T, TY, R, HASA, POLY>
-      <<<nbx * nby * nbz, 256, 0, s>>>(in, out, X, Y, Z, TZ, nbx, nby, c);
-  return SPS_OK;
-}
-
-template <typename T, int TY, int R, bool HASA, int POLY>
-int launch_variant(int variant, const T* in, T* out, int X, int Y, int Z,
-                   int TZ, const FusedCoef<T>& c, hipStream_t s) {
-  switch (variant) {
-    case 8: return launch3s<T, 8, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
-    case 16: return launch3s<T, 16, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
-    default: return launch3<T, TY, R, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+  const int64_t nitems = (int64_t)nstrips * nrowblk * nzc;
+  if (nitems > ((int64_t)1 << 31) - 4) {
+    set_error("fused 3D step: too many work items; raise planes per chunk");
+    return SPS_E_INVALID;
   }
+  fused3d_stream<T, TYW, R, K, HASA, POLY><<<(unsigned)((nitems + 3) / 4), 256, 0, s>>>(
+      in, out, X, Y, Z, TZ, nstrips, nrowblk, (int)nitems, c);
+  return SPS_OK;
 }
 
-template <typename T, int TY, int R>
-int dispatch3(int variant, const T* in, T* out, int X, int Y, int Z, int TZ,
+template <typename T, int R, bool HASA, int POLY>
+int launch_variant(int K, int rows, const T* in, T* out, int X, int Y, int Z,
+                   int TZ, const FusedCoef<T>& c, hipStream_t s) {
+  if (K == 1) {
+    switch (rows) {
+      case 4: return launch3s<T, 4, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      case 8: return launch3s<T, 8, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      case 16: return launch3s<T, 16, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      default: break;
+    }
+  } else if (K == 2) {
+    if (rows == 4) return launch3s<T, 4, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+    if constexpr (sizeof(T) == 4) {
+      if (rows == 8) return launch3s<T, 8, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+    }
+  }
+  set_error("fused 3D step: unsupported launch shape (%d steps, %d rows)", K, rows);
+  return SPS_E_INVALID;
+}
+
+template <typename T, int R>
+int dispatch3(int K, int rows, const T* in, T* out, int X, int Y, int Z, int TZ,
               const FusedCoef<T>& c, hipStream_t s) {
   const bool hasa = c.dega >= 0;
   const int deg = c.dega > c.degb ? c.dega : c.degb;
   if (deg <= 3)
-    return hasa ? launch_variant<T, TY, R, true, 3>(variant, in, out, X, Y, Z, TZ, c, s)
-                : launch_variant<T, TY, R, false, 3>(variant, in, out, X, Y, Z, TZ, c, s);
-  return hasa ? launch_variant<T, TY, R, true, 6>(variant, in, out, X, Y, Z, TZ, c, s)
-              : launch_variant<T, TY, R, false, 6>(variant, in, out, X, Y, Z, TZ, c, s);
+    return hasa ? launch_variant<T, R, true, 3>(K, rows, in, out, X, Y, Z, TZ, c, s)
+                : launch_variant<T, R, false, 3>(K, rows, in, out, X, Y, Z, TZ, c, s);
+  return hasa ? launch_variant<T, R, true, 6>(K, rows, in, out, X, Y, Z, TZ, c, s)
+              : launch_variant<T, R, false, 6>(K, rows, in, out, X, Y, Z, TZ, c, s);
 }
 
 }  // namespace
 
+bool fused3d_shape_ok(int dtype_size, int K, int rows) {
+  if (K == 1) return rows == 4 || rows == 8 || rows == 16;
+  if (K == 2) return rows == 4 || (rows == 8 && dtype_size == 4);
+  return false;
+}
+
 template <typename T>
-int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int rows, int TZ,
-                   const FusedCoef<T>& c, hipStream_t s) {
+int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int K, int rows,
+                   int TZ, const FusedCoef<T>& c, hipStream_t s) {
   if (TZ < 1) {
     set_error("fused 3D step: planes per chunk must be >= 1");
     return SPS_E_INVALID;
   }
-  constexpr int TY = sizeof(T) == 4 ? 16 : 8;
   int rc = c.radius == 1
-               ? dispatch3<T, TY, 1>(rows, in, out, X, Y, Z, TZ, c, s)
-               : dispatch3<T, TY, 2>(rows, in, out, X, Y, Z, TZ, c, s);
+               ? dispatch3<T, 1>(K, rows, in, out, X, Y, Z, TZ, c, s)
+               : dispatch3<T, 2>(K, rows, in, out, X, Y, Z, TZ, c, s);
   if (rc != SPS_OK) return rc;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -322,8 +250,9 @@ int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int rows, int TZ,
 }
 
 template int launch_fused3d<float>(const float*, float*, int, int, int, int,
-                                   int, const FusedCoef<float>&, hipStream_t);
+                                   int, int, const FusedCoef<float>&, hipStream_t);
 template int launch_fused3d<double>(const double*, double*, int, int, int, int,
-                                    int, const FusedCoef<double>&, hipStream_t);
+                                    int, int, const FusedCoef<double>&,
+                                    hipStream_t);
 
 }  // namespace sps

@@ -122,6 +122,11 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
         for (int j = 0; j < VEC; ++j) v[j] = buf[q][j];
 #pragma unroll
         for (int s = 0; s < K; ++s) {
+          // stage s has valid input from stream row 2*R*s on (each stage
+          // delays by R rows and needs 2*R rows of fill); before that it
+          // would only turn garbage into garbage, so it is skipped.  Its
+          // valid outputs depend only on pushes of valid rows.
+          if (s > 0 && i + q < 2 * R * s) continue;
           if (q == 0) st[s].template push<0>(c, v);
           if (q == 1) st[s].template push<1>(c, v);
           if (q == 2) st[s].template push<2>(c, v);

@@ -2,16 +2,17 @@
 // (include/sps_nn.h).
 //
 // Forward (and backward-data, which is the forward kernel with flipped,
-// transposed weights) is an implicit GEMM: M = pixels, N = output channels,
+// transposed weights) is an implicit GEMM: M = output channels, N = pixels,
 // K = 9 taps x input channels, on v_mfma_f32_16x16x32_bf16.  Operand layout
 // of that instruction on gfx950 (wave64):
 //   A (16 x 32, M x K): lane l holds A[l % 16][8*(l/16) + 0..7]
 //   B (32 x 16, K x N): lane l holds B[8*(l/16) + 0..7][l % 16]
 //   D (16 x 16):        lane l holds D[4*(l/16) + 0..3][l % 16]
-// With NHWC activations and [tap][out][in] weights both A and B fragments
-// are single 16-byte loads, so the kernel streams straight from L1/L2; each
-// wave owns 64 pixels x 16*NT output channels and reuses every A fragment
-// NT times and every B fragment 4 times.
+// With NHWC activations and [tap][out][in] weights both A (weights) and B
+// (pixels) fragments are single 16-byte loads, so the kernel streams straight
+// from L1/L2; each wave owns 16*NT output channels x 64 pixels, reuses every
+// weight fragment 4 times and every pixel fragment NT times, and stores 4
+// consecutive output channels of a pixel per lane (8 bytes).
 //
 // The weight gradient reduces over pixels, which are strided in NHWC, so it
 // stages a 32-pixel row chunk of dpre and the three shifted input rows in LDS
@@ -45,6 +46,38 @@ __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
 }
 
+// 4 consecutive BF16 values as one 8-byte access (callers keep 8-byte
+// alignment: channel offsets are multiples of 4)
+__device__ __forceinline__ void store4(uint16_t* p, const float (&v)[4]) {
+  const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+  const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  *reinterpret_cast<uint2*>(p) = make_uint2(lo, hi);
+}
+__device__ __forceinline__ void load4(const uint16_t* p, float (&v)[4]) {
+  const uint2 q = *reinterpret_cast<const uint2*>(p);
+  v[0] = bf2f((uint16_t)(q.x & 0xffffu));
+  v[1] = bf2f((uint16_t)(q.x >> 16));
+  v[2] = bf2f((uint16_t)(q.y & 0xffffu));
+  v[3] = bf2f((uint16_t)(q.y >> 16));
+}
+// 8 consecutive BF16 values as one 16-byte access (16-byte aligned)
+__device__ __forceinline__ void store8(uint16_t* p, const float (&v)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    w[i] = (uint32_t)f2bf(v[2 * i]) | ((uint32_t)f2bf(v[2 * i + 1]) << 16);
+  *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ void load8(const uint16_t* p, float (&v)[8]) {
+  const uint4 q = *reinterpret_cast<const uint4*>(p);
+  const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    v[2 * i] = bf2f((uint16_t)(w[i] & 0xffffu));
+    v[2 * i + 1] = bf2f((uint16_t)(w[i] >> 16));
+  }
+}
+
 __device__ __forceinline__ int wrapi(int v, int n) {
   return v < 0 ? v + n : (v >= n ? v - n : v);
 }
@@ -55,6 +88,9 @@ __global__ __launch_bounds__(256) void conv_fwd(
     const uint16_t* __restrict__ wt, const float* __restrict__ bias, int K,
     int act, const uint16_t* __restrict__ skip, uint16_t* __restrict__ y,
     uint16_t* __restrict__ pre, int npix) {
+  // D = W (out channels x in channels) * X (in channels x pixels): rows of D
+  // are output channels, so lane l ends with 4 consecutive channels of one
+  // pixel and stores them as one 8-byte word.
   const int lane = threadIdx.x & 63;
   const int p0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
   if (p0 >= npix) return;
@@ -70,65 +106,79 @@ __global__ __launch_bounds__(256) void conv_fwd(
     ph[m] = r / W;
     pw[m] = r - ph[m] * W;
   }
-  f32x4v acc[4][NT];
+  f32x4v acc[NT][4];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int j = 0; j < NT; ++j)
 #pragma unroll
-    for (int j = 0; j < NT; ++j) acc[m][j] = f32x4v{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < 4; ++m) acc[j][m] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
   for (int tap = 0; tap < 9; ++tap) {
     const int dr = tap / 3 - 1, ds = tap % 3 - 1;
-    const uint16_t* arow[4];
+    const uint16_t* xrow[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int hh = wrapi(ph[m] + dr, H), ww = wrapi(pw[m] + ds, W);
-      arow[m] = x + ((int64_t)(pn[m] * H + hh) * W + ww) * C + cq;
+      xrow[m] = x + ((int64_t)(pn[m] * H + hh) * W + ww) * C + cq;
     }
-    const uint16_t* brow = wt + ((int64_t)tap * K + k0 + (lane & 15)) * C + cq;
+    const uint16_t* wrow = wt + ((int64_t)tap * K + k0 + (lane & 15)) * C + cq;
 #pragma unroll 2
     for (int c0 = 0; c0 < C; c0 += 32) {
-      bf16x8 a[4], b[NT];
+      bf16x8 a[NT], b[4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) a[m] = load_frag(arow[m] + c0);
+      for (int j = 0; j < NT; ++j) a[j] = load_frag(wrow + (int64_t)j * 16 * C + c0);
 #pragma unroll
-      for (int j = 0; j < NT; ++j) b[j] = load_frag(brow + (int64_t)j * 16 * C + c0);
+      for (int m = 0; m < 4; ++m) b[m] = load_frag(xrow[m] + c0);
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
+      for (int j = 0; j < NT; ++j)
 #pragma unroll
-        for (int j = 0; j < NT; ++j)
-          acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[j],
-                                                              acc[m][j], 0, 0, 0);
+        for (int m = 0; m < 4; ++m)
+          acc[j][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[j], b[m],
+                                                              acc[j][m], 0, 0, 0);
     }
   }
 
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int j = 0; j < NT; ++j) {
+    const int ch = k0 + 16 * j + 4 * (lane >> 4);  // 4 consecutive channels
+    float bv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (bias) {
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      const int ch = k0 + 16 * j + (lane & 15);
-      const float bv = bias ? bias[ch] : 0.f;
+      for (int i = 0; i < 4; ++i) bv[i] = bias[ch + i];
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int p = p0 + 16 * m + 4 * (lane >> 4) + i;
-        const float v = acc[m][j][i] + bv;
-        if constexpr (EPI == SPS_NN_EPI_PLAIN) {
-          const int64_t o = (int64_t)p * K + ch;
-          if (pre) pre[o] = f2bf(v);
-          y[o] = f2bf(act ? silu(v) : v);
-        } else {
-          const int n = p / HW, r = p - n * HW, hh = r / W, ww = r - hh * W;
+    for (int m = 0; m < 4; ++m) {
+      const int p = p0 + 16 * m + (lane & 15);
+      float v[4];
 #pragma unroll
-          for (int a2 = 0; a2 < 2; ++a2)
+      for (int i = 0; i < 4; ++i) v[i] = acc[j][m][i] + bv[i];
+      if constexpr (EPI == SPS_NN_EPI_PLAIN) {
+        const int64_t o = (int64_t)p * K + ch;
+        if (pre) store4(pre + o, v);
+        if (act) {
 #pragma unroll
-            for (int b2 = 0; b2 < 2; ++b2) {
-              const int64_t o =
-                  ((int64_t)(n * 2 * H + 2 * hh + a2) * (2 * W) + 2 * ww + b2) *
-                      K + ch;
-              const float u = v + bf2f(skip[o]);
-              if (pre) pre[o] = f2bf(u);
-              y[o] = f2bf(act ? silu(u) : u);
-            }
+          for (int i = 0; i < 4; ++i) v[i] = silu(v[i]);
         }
+        store4(y + o, v);
+      } else {
+        const int n = pn[m], hh = ph[m], ww = pw[m];
+#pragma unroll
+        for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+          for (int b2 = 0; b2 < 2; ++b2) {
+            const int64_t o =
+                ((int64_t)(n * 2 * H + 2 * hh + a2) * (2 * W) + 2 * ww + b2) * K +
+                ch;
+            float u[4];
+            load4(skip + o, u);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] += v[i];
+            if (pre) store4(pre + o, u);
+            if (act) {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) u[i] = silu(u[i]);
+            }
+            store4(y + o, u);
+          }
       }
     }
   }
@@ -248,6 +298,8 @@ __global__ __launch_bounds__(256) void conv_wgrad(
 }
 
 // ------------------------------------------------------------ thin layers
+// The first (1 -> K) and last (K -> 1) convolutions have no GEMM shape; they
+// run on the vector units with 16-byte accesses of 8 channels.  K % 8 == 0.
 
 __global__ __launch_bounds__(256) void in1_fwd(const float* __restrict__ x,
                                                int N, int H, int W,
@@ -265,34 +317,75 @@ __global__ __launch_bounds__(256) void in1_fwd(const float* __restrict__ x,
   for (int t = 0; t < 9; ++t)
     xv[t] = x[(int64_t)(n * H + wrapi(h + t / 3 - 1, H)) * W +
               wrapi(w + t % 3 - 1, W)];
-  for (int k = 0; k < K; ++k) {
-    float v = bias ? bias[k] : 0.f;
+  for (int k0 = 0; k0 < K; k0 += 8) {
+    float v[8];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) v = fmaf(xv[t], wt[t * K + k], v);
-    if (pre) pre[p * K + k] = f2bf(v);
-    y[p * K + k] = f2bf(act ? silu(v) : v);
+    for (int i = 0; i < 8; ++i) {
+      float a = bias ? bias[k0 + i] : 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) a = fmaf(xv[t], wt[t * K + k0 + i], a);
+      v[i] = a;
+    }
+    if (pre) store8(pre + p * K + k0, v);
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = silu(v[i]);
+    }
+    store8(y + p * K + k0, v);
   }
 }
 
-// threads = 9*K (tap, k); each block reduces a range of pixels
-__global__ void in1_wgrad(const float* __restrict__ x,
-                          const uint16_t* __restrict__ dy, int N, int H, int W,
-                          int K, float* __restrict__ dw, int pix_per_block) {
-  const int tk = threadIdx.x;
-  if (tk >= 9 * K) return;
-  const int t = tk / K, k = tk - t * K;
-  const int dr = t / 3 - 1, ds = t % 3 - 1;
+// dweight[t][k] += sum_p x[shift_t(p)] * dy[p][k]: 256-pixel chunks staged
+// in LDS, then thread (t, k) pairs reduce the chunk.
+__global__ __launch_bounds__(256) void in1_wgrad(const float* __restrict__ x,
+                                                 const uint16_t* __restrict__ dy,
+                                                 int N, int H, int W, int K,
+                                                 float* __restrict__ dw,
+                                                 int chunks_per_block) {
+  __shared__ float sx[9][256];
+  __shared__ float sd[256][65];
   const int64_t np = (int64_t)N * H * W;
-  const int64_t a = (int64_t)blockIdx.x * pix_per_block;
-  const int64_t b = min(np, a + pix_per_block);
   const int HW = H * W;
-  float acc = 0.f;
-  for (int64_t p = a; p < b; ++p) {
-    const int n = p / HW, r = p - (int64_t)n * HW, h = r / W, w = r - h * W;
-    const float xv = x[(int64_t)(n * H + wrapi(h + dr, H)) * W + wrapi(w + ds, W)];
-    acc = fmaf(xv, bf2f(dy[p * K + k]), acc);
+  float acc[3] = {0.f, 0.f, 0.f};  // up to 9*64 = 576 (t, k) pairs / 256
+  for (int c = 0; c < chunks_per_block; ++c) {
+    const int64_t base = ((int64_t)blockIdx.x * chunks_per_block + c) * 256;
+    if (base >= np) break;
+    __syncthreads();
+    const int64_t p = base + threadIdx.x;
+    if (p < np) {
+      const int n = p / HW, r = p - (int64_t)n * HW, h = r / W, w = r - h * W;
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        sx[t][threadIdx.x] = x[(int64_t)(n * H + wrapi(h + t / 3 - 1, H)) * W +
+                               wrapi(w + t % 3 - 1, W)];
+      for (int k0 = 0; k0 < K; k0 += 8) {
+        float v[8];
+        load8(dy + p * K + k0, v);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sd[threadIdx.x][k0 + i] = v[i];
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 9; ++t) sx[t][threadIdx.x] = 0.f;
+      for (int k = 0; k < K; ++k) sd[threadIdx.x][k] = 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int tk = threadIdx.x + 256 * q;
+      if (tk < 9 * K) {
+        const int t = tk / K, k = tk - t * K;
+        float a = acc[q];
+        for (int i = 0; i < 256; ++i) a = fmaf(sx[t][i], sd[i][k], a);
+        acc[q] = a;
+      }
+    }
   }
-  atomicAdd(&dw[tk], acc);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int tk = threadIdx.x + 256 * q;
+    if (tk < 9 * K) atomicAdd(&dw[tk], acc[q]);
+  }
 }
 
 __global__ __launch_bounds__(256) void out1_fwd(const uint16_t* __restrict__ x,
@@ -309,7 +402,12 @@ __global__ __launch_bounds__(256) void out1_fwd(const uint16_t* __restrict__ x,
     const uint16_t* xp =
         x + ((int64_t)(n * H + wrapi(h + t / 3 - 1, H)) * W +
              wrapi(w + t % 3 - 1, W)) * C;
-    for (int c = 0; c < C; ++c) v = fmaf(bf2f(xp[c]), wt[t * C + c], v);
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      float xv[8];
+      load8(xp + c0, xv);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v = fmaf(xv[i], wt[t * C + c0 + i], v);
+    }
   }
   y[p] = v;
 }
@@ -327,34 +425,69 @@ __global__ __launch_bounds__(256) void out1_bwd_data(
   for (int t = 0; t < 9; ++t)
     d[t] = dy[(int64_t)(n * H + wrapi(h - (t / 3 - 1), H)) * W +
               wrapi(w - (t % 3 - 1), W)];
-  for (int c = 0; c < C; ++c) {
-    float v = 0.f;
+  for (int c0 = 0; c0 < C; c0 += 8) {
+    float v[8];
 #pragma unroll
-    for (int t = 0; t < 9; ++t) v = fmaf(d[t], wt[t * C + c], v);
-    dx[p * C + c] = f2bf(v);
+    for (int i = 0; i < 8; ++i) {
+      float a = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) a = fmaf(d[t], wt[t * C + c0 + i], a);
+      v[i] = a;
+    }
+    store8(dx + p * C + c0, v);
   }
 }
 
-// threads = 9*C (tap, c)
-__global__ void out1_wgrad(const uint16_t* __restrict__ x,
-                           const float* __restrict__ dy, int N, int H, int W,
-                           int C, float* __restrict__ dw, int pix_per_block) {
-  const int tc = threadIdx.x;
-  if (tc >= 9 * C) return;
-  const int t = tc / C, c = tc - t * C;
-  const int dr = t / 3 - 1, ds = t % 3 - 1;
+// dweight[t][c] += sum_p x[shift_t(p)][c] * dy[p]: per 256-pixel chunk the
+// 9 shifted rows of x are read by thread (t, c) pairs from global memory
+// (L1/L2 hits) and dy from LDS.
+__global__ __launch_bounds__(256) void out1_wgrad(const uint16_t* __restrict__ x,
+                                                  const float* __restrict__ dy,
+                                                  int N, int H, int W, int C,
+                                                  float* __restrict__ dw,
+                                                  int chunks_per_block) {
+  __shared__ float sd[256];
+  __shared__ int soff[9][256];
   const int64_t np = (int64_t)N * H * W;
-  const int64_t a = (int64_t)blockIdx.x * pix_per_block;
-  const int64_t b = min(np, a + pix_per_block);
   const int HW = H * W;
-  float acc = 0.f;
-  for (int64_t p = a; p < b; ++p) {
-    const int n = p / HW, r = p - (int64_t)n * HW, h = r / W, w = r - h * W;
-    const float xv = bf2f(
-        x[((int64_t)(n * H + wrapi(h + dr, H)) * W + wrapi(w + ds, W)) * C + c]);
-    acc = fmaf(xv, dy[p], acc);
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int c = 0; c < chunks_per_block; ++c) {
+    const int64_t base = ((int64_t)blockIdx.x * chunks_per_block + c) * 256;
+    if (base >= np) break;
+    __syncthreads();
+    const int64_t p = base + threadIdx.x;
+    if (p < np) {
+      const int n = p / HW, r = p - (int64_t)n * HW, h = r / W, w = r - h * W;
+      sd[threadIdx.x] = dy[p];
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        soff[t][threadIdx.x] =
+            (int)((int64_t)(n * H + wrapi(h + t / 3 - 1, H)) * W +
+                  wrapi(w + t % 3 - 1, W) - base);
+    } else {
+      sd[threadIdx.x] = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) soff[t][threadIdx.x] = 0;
+    }
+    __syncthreads();
+    const int64_t lim = np - base;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int tc = threadIdx.x + 256 * q;
+      if (tc < 9 * C) {
+        const int t = tc / C, ch = tc - t * C;
+        float a = acc[q];
+        for (int i = 0; i < 256 && i < lim; ++i)
+          a = fmaf(bf2f(x[(base + soff[t][i]) * C + ch]), sd[i], a);
+        acc[q] = a;
+      }
+    }
   }
-  atomicAdd(&dw[tc], acc);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int tc = threadIdx.x + 256 * q;
+    if (tc < 9 * C) atomicAdd(&dw[tc], acc[q]);
+  }
 }
 
 // ------------------------------------------------------------ elementwise
@@ -507,9 +640,9 @@ int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
                        const float* weight, const float* bias, int cout,
                        int act, void* y, void* pre, void* stream) {
   const int64_t np = (int64_t)n * h * w;
-  if (!x || !weight || !y || np < 1 || cout < 1 || cout > 64 ||
+  if (!x || !weight || !y || np < 1 || cout < 8 || cout > 64 || cout % 8 ||
       (act != 0 && act != 1)) {
-    set_error("sps_nn_conv3x3_in1: invalid arguments");
+    set_error("sps_nn_conv3x3_in1: cout a multiple of 8 in 8..64, act 0/1");
     return SPS_E_INVALID;
   }
   in1_fwd<<<(unsigned)((np + 255) / 256), 256, 0, (hipStream_t)stream>>>(
@@ -520,13 +653,13 @@ int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
 int sps_nn_conv3x3_in1_wgrad(const float* x, const void* dpre, int n, int h,
                              int w, int cout, float* dweight, void* stream) {
   const int64_t np = (int64_t)n * h * w;
-  if (!x || !dpre || !dweight || np < 1 || cout < 1 || cout > 64) {
-    set_error("sps_nn_conv3x3_in1_wgrad: invalid arguments");
+  if (!x || !dpre || !dweight || np < 1 || cout < 8 || cout > 64 || cout % 8) {
+    set_error("sps_nn_conv3x3_in1_wgrad: cout a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
-  const int per = 1024;
-  const int threads = ((9 * cout + 63) / 64) * 64;
-  in1_wgrad<<<(unsigned)((np + per - 1) / per), threads, 0,
+  const int per = 16;  // 256-pixel chunks per block
+  const int64_t chunks = (np + 255) / 256;
+  in1_wgrad<<<(unsigned)((chunks + per - 1) / per), 256, 0,
               (hipStream_t)stream>>>(x, (const uint16_t*)dpre, n, h, w, cout,
                                      dweight, per);
   return launch_ok("sps_nn_conv3x3_in1_wgrad");
@@ -536,8 +669,8 @@ int sps_nn_conv3x3_out1(const void* x, int n, int h, int w, int cin,
                         const float* weight, const float* bias, float* y,
                         void* stream) {
   const int64_t np = (int64_t)n * h * w;
-  if (!x || !weight || !y || np < 1 || cin < 1 || cin > 64) {
-    set_error("sps_nn_conv3x3_out1: invalid arguments");
+  if (!x || !weight || !y || np < 1 || cin < 8 || cin > 64 || cin % 8) {
+    set_error("sps_nn_conv3x3_out1: cin a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
   out1_fwd<<<(unsigned)((np + 255) / 256), 256, 0, (hipStream_t)stream>>>(
@@ -549,8 +682,8 @@ int sps_nn_conv3x3_out1_bwd(const void* x, const float* dy, int n, int h,
                             int w, int cin, const float* weight, void* dx,
                             float* dweight, void* stream) {
   const int64_t np = (int64_t)n * h * w;
-  if (!x || !dy || !weight || np < 1 || cin < 1 || cin > 64) {
-    set_error("sps_nn_conv3x3_out1_bwd: invalid arguments");
+  if (!x || !dy || !weight || np < 1 || cin < 8 || cin > 64 || cin % 8) {
+    set_error("sps_nn_conv3x3_out1_bwd: cin a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
   hipStream_t s = (hipStream_t)stream;
@@ -560,9 +693,9 @@ int sps_nn_conv3x3_out1_bwd(const void* x, const float* dy, int n, int h,
     SPS_TRY(launch_ok("sps_nn_conv3x3_out1_bwd (data)"));
   }
   if (dweight) {
-    const int per = 1024;
-    const int threads = ((9 * cin + 63) / 64) * 64;
-    out1_wgrad<<<(unsigned)((np + per - 1) / per), threads, 0, s>>>(
+    const int per = 16;
+    const int64_t chunks = (np + 255) / 256;
+    out1_wgrad<<<(unsigned)((chunks + per - 1) / per), 256, 0, s>>>(
         (const uint16_t*)x, dy, n, h, w, cin, dweight, per);
     SPS_TRY(launch_ok("sps_nn_conv3x3_out1_bwd (weight)"));
   }

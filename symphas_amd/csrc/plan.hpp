@@ -1,6 +1,8 @@
 // plan.hpp - the single-field plan behind sps_fd_plan*.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "fused.hpp"
 
@@ -42,9 +44,10 @@ struct sps_fd_plan {
   sps::RefParams<double> rp64;
   sps::FusedCoef<float> fc32;
   sps::FusedCoef<double> fc64;
-  int K = 1;    // fused 2D: steps per launch
+  int K = 1;    // fused: steps per launch
   int TY = 32;  // fused 2D: rows per chunk; fused 3D: y rows per wave
   int TZ = 64;  // fused 3D: planes per chunk
+  int VEC = 4;  // fused 2D: values per lane (16 or 8 bytes)
   sps::GraphSlot graphs[2];  // indexed by the buffer parity at capture
 
   int launch_one(int k);
@@ -52,14 +55,22 @@ struct sps_fd_plan {
   int step(int nsteps);
   void drop_graphs();
 
-  // Launch shape before any tuning: the best of a sweep of steps per launch
-  // {1..4} x rows per chunk {16..256} on Model B 4096^2 (FP32: 3 x 32,
-  // FP64: 4 x 64) and planes per chunk {8..128} on 512^3.
+  // Launch shape before any tuning.  2D: the best of a sweep of steps per
+  // launch {1..4} x rows per chunk {16..256} on Model B 4096^2 (FP32: 3 x 32,
+  // FP64: 4 x 64).  3D: two steps per launch (FP32: 8 rows per wave, two
+  // waves per SIMD; FP64: 4 rows, one wave per SIMD) with the z-chunk height
+  // that splits the grid into as few rounds of resident waves as possible
+  // (on Model B 512^3 FP32 that is 171 planes, 3 chunks, 1920 waves).
   void choose_default_shape() {
     if (d.dim == 3) {
-      K = 1;
-      TY = 8;
-      TZ = 64;
+      K = 2;
+      TY = d.dtype == SPS_F32 ? 8 : 4;
+      const int slots = d.dtype == SPS_F32 ? 2048 : 1024;  // resident waves
+      const int halo = 2 * K;  // lanes per side at radius two
+      const int64_t per_chunk = (int64_t)((d.n[0] + 63 - 2 * halo) / (64 - 2 * halo)) *
+                                ((d.n[1] + TY - 1) / TY);
+      const int64_t m = std::max<int64_t>(1, slots / per_chunk);
+      TZ = (int)std::min<int64_t>(1024, std::max<int64_t>(16, (d.n[2] + m - 1) / m));
     } else if (d.dtype == SPS_F32) {
       K = 3;
       TY = 32;
@@ -67,6 +78,8 @@ struct sps_fd_plan {
       K = 4;
       TY = 64;
     }
+    VEC = 16 / (int)esize;
+    if (d.dim == 2 && d.n[0] % VEC != 0) VEC = 8 / (int)esize;
   }
 
   int allocate() {

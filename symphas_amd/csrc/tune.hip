@@ -1,4 +1,5 @@
 // tune.hip - launch-shape autotuner of the fused single-field step.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -6,6 +7,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "fused.hpp"
 #include "plan.hpp"
 
 using namespace sps;
@@ -39,7 +41,7 @@ std::string plan_key(const sps_fd_plan* p) {
   return buf;
 }
 
-bool cache_lookup(const std::string& key, int* k, int* r, int* z) {
+bool cache_lookup(const std::string& key, int* k, int* r, int* z, int* v) {
   const char* path = std::getenv("SPS_FD_TUNE_CACHE");
   if (!path || !*path) return false;
   FILE* f = std::fopen(path, "r");
@@ -50,11 +52,12 @@ bool cache_lookup(const std::string& key, int* k, int* r, int* z) {
     char* bar = std::strstr(line, " | ");
     if (!bar) continue;
     if (std::string(line, bar - line) != key) continue;
-    int a, b, c;
-    if (std::sscanf(bar + 3, "%d %d %d", &a, &b, &c) == 3) {
+    int a, b, c, e;
+    if (std::sscanf(bar + 3, "%d %d %d %d", &a, &b, &c, &e) == 4) {
       *k = a;
       *r = b;
       *z = c;
+      *v = e;
       found = true;  // the last entry for a key wins
     }
   }
@@ -62,19 +65,21 @@ bool cache_lookup(const std::string& key, int* k, int* r, int* z) {
   return found;
 }
 
-void cache_store(const std::string& key, int k, int r, int z, double us) {
+void cache_store(const std::string& key, int k, int r, int z, int v,
+                 double us) {
   const char* path = std::getenv("SPS_FD_TUNE_CACHE");
   if (!path || !*path) return;
   FILE* f = std::fopen(path, "a");
   if (!f) return;
-  std::fprintf(f, "%s | %d %d %d | %.3f us/step\n", key.c_str(), k, r, z, us);
+  std::fprintf(f, "%s | %d %d %d %d | %.3f us/step\n", key.c_str(), k, r, z, v,
+               us);
   std::fclose(f);
 }
 
 }  // namespace
 
 extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
-                               int* out_rows, int* out_planes) {
+                               int* out_rows, int* out_planes, int* out_vec) {
   if (!p) {
     set_error("plan is NULL");
     return SPS_E_INVALID;
@@ -83,6 +88,7 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
     if (out_k) *out_k = p->steps_per_launch();
     if (out_rows) *out_rows = p->TY;
     if (out_planes) *out_planes = p->d.dim == 3 ? p->TZ : 0;
+    if (out_vec) *out_vec = p->d.dim == 2 ? p->VEC : 1;
     return SPS_OK;
   };
   if (p->path != SPS_PATH_FUSED) return report();
@@ -91,33 +97,52 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
     return SPS_E_STATE;
   }
   const std::string key = plan_key(p);
-  int ck, cr, cz;
-  if (use_cache && cache_lookup(key, &ck, &cr, &cz)) {
+  int ck, cr, cz, cv;
+  if (use_cache && cache_lookup(key, &ck, &cr, &cz, &cv)) {
     p->K = ck;
     p->TY = cr;
     if (p->d.dim == 3) p->TZ = cz;
+    if (p->d.dim == 2) p->VEC = cv;
     p->drop_graphs();
     return report();
   }
   struct Cand {
-    int k, rows, planes;
+    int k, rows, planes, vec;
   };
   std::vector<Cand> cands;
   if (p->d.dim == 2) {
-    for (int k = 1; k <= 4; ++k)
-      for (int r : {8, 16, 32, 64}) cands.push_back({k, r, 0});
+    // wide lanes (16 bytes) with short chunks, narrow lanes (8 bytes, twice
+    // the strips) with tall ones
+    const int wide = 16 / (int)p->esize, narrow = 8 / (int)p->esize;
+    for (int k = 1; k <= 4; ++k) {
+      if (p->d.n[0] % wide == 0)
+        for (int r : {8, 16, 32, 64}) cands.push_back({k, r, 0, wide});
+      for (int r : {16, 32, 64, 128}) cands.push_back({k, r, 0, narrow});
+    }
   } else {
-    for (int r : {8, 16})
-      for (int z : {8, 16, 32, 64}) cands.push_back({1, r, z});
-    cands.push_back({1, 4, 64});
+    // chunk heights: short chunks, and Z split evenly into m chunks, which
+    // puts the whole grid in one or a few rounds of resident waves
+    std::vector<int> planes = {8, 16, 32};
+    const int Z = p->d.n[2];
+    for (int m = 1; m <= 8; ++m) {
+      const int z = (Z + m - 1) / m;
+      if (z > 32 && z <= 1024 &&
+          std::find(planes.begin(), planes.end(), z) == planes.end())
+        planes.push_back(z);
+    }
+    for (int k : {1, 2})
+      for (int r : {4, 8, 16})
+        for (int z : planes)
+          if ((k == 1 && r > 4) || (k == 2 && sps::fused3d_shape_ok((int)p->esize, k, r)))
+            cands.push_back({k, r, z, 1});
   }
   const size_t bytes = p->nsites * p->esize;
   void* saved = nullptr;
   SPS_HIP_TRY(hipMalloc(&saved, bytes));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SPS_OK;
-  const int oldK = p->K, oldTY = p->TY, oldTZ = p->TZ;
-  Cand best{oldK, oldTY, oldTZ};
+  const int oldK = p->K, oldTY = p->TY, oldTZ = p->TZ, oldVEC = p->VEC;
+  Cand best{oldK, oldTY, oldTZ, oldVEC};
   double best_us = 1e300;
   do {
     if (hipMemcpyAsync(saved, p->buf[p->cur], bytes, hipMemcpyDeviceToDevice,
@@ -131,6 +156,7 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
       p->K = c.k;
       p->TY = c.rows;
       if (p->d.dim == 3) p->TZ = c.planes;
+      if (p->d.dim == 2) p->VEC = c.vec;
       const int spl = p->steps_per_launch();
       // about 12 steps (a multiple of every K) per trial, twice
       const int launches = p->d.dim == 3 ? 4 : 12 / spl;
@@ -172,13 +198,16 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
     p->K = oldK;
     p->TY = oldTY;
     p->TZ = oldTZ;
+    p->VEC = oldVEC;
     p->drop_graphs();
     return rc;
   }
   p->K = best.k;
   p->TY = best.rows;
   if (p->d.dim == 3) p->TZ = best.planes;
+  if (p->d.dim == 2) p->VEC = best.vec;
   p->drop_graphs();
-  cache_store(key, p->K, p->TY, p->d.dim == 3 ? p->TZ : 0, best_us);
+  cache_store(key, p->K, p->TY, p->d.dim == 3 ? p->TZ : 0,
+              p->d.dim == 2 ? p->VEC : 1, best_us);
   return report();
 }

@@ -114,15 +114,19 @@ class FDPlan:
         abi.check(self.lib.sps_fd_query(self._h, byref(path), byref(spl)))
         return path.value, spl.value
 
-    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0):
-        abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk))
+    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0,
+             values_per_lane: int = 0):
+        abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk,
+                                       values_per_lane))
 
     def autotune(self, use_cache: bool = True):
         """Time candidate launch shapes on the device and keep the fastest;
-        returns (steps_per_launch, rows_per_tile, planes_per_chunk)."""
-        k, r, z = c_int(), c_int(), c_int()
-        abi.check(self.lib.sps_fd_autotune(self._h, int(use_cache), byref(k), byref(r), byref(z)))
-        return k.value, r.value, z.value
+        returns (steps_per_launch, rows_per_tile, planes_per_chunk,
+        values_per_lane)."""
+        k, r, z, v = c_int(), c_int(), c_int(), c_int()
+        abi.check(self.lib.sps_fd_autotune(self._h, int(use_cache), byref(k), byref(r), byref(z),
+                                           byref(v)))
+        return k.value, r.value, z.value, v.value
 
 
 def device_count() -> int:

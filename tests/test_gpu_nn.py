@@ -85,9 +85,10 @@ def test_conv_backward_weight(gpu_lib, N, H, W, C, K):
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max() * np.sqrt(N * H * W)
 
 
-def test_thin_layers(gpu_lib):
-    print("[case] thin layers", flush=True)
-    N, H, W, K = 2, 16, 16, 32
+@pytest.mark.parametrize("N,H,W,K", [(2, 16, 16, 32), (3, 12, 20, 64), (1, 9, 7, 8)])
+def test_thin_layers(gpu_lib, N, H, W, K):
+    # 720 and 63 pixels leave a partial 256-pixel chunk in the weight gradients
+    print(f"[case] thin layers {N}x{H}x{W} K={K}", flush=True)
     lib = nn._lib()
     x = rnd((N, H, W), 12).astype(np.float32)
     w = rnd((9, K), 13, 0.3).astype(np.float32)

@@ -34,10 +34,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def gpu_run(desc, psi, nsteps, K=0, TY=0, chunks=None, TZ=0):
+def gpu_run(desc, psi, nsteps, K=0, TY=0, chunks=None, TZ=0, VEC=0):
     with FDPlan(desc) as p:
-        if K or TY or TZ:
-            p.tune(K, TY, TZ)
+        if K or TY or TZ or VEC:
+            p.tune(K, TY, TZ, VEC)
         p.upload(psi.astype(p.dtype))
         if chunks:
             for c in chunks:
@@ -221,21 +221,61 @@ def test_temporal_blocking_is_bitwise(gpu_lib, dtype, model):
     psi = field(desc, seed=11)
     say("temporal blocking", model, dtype)
     base = gpu_run(desc, psi, 24, K=1, TY=32)
-    for K in (1, 2, 3, 4):
-        for TY in (4, 12, 64, 200, 512):
-            got = gpu_run(desc, psi, 24, K=K, TY=TY)
-            assert np.array_equal(got, base), (K, TY)
+    wide = 4 if dtype == F32 else 2
+    for vec in (wide, wide // 2):
+        for K in (1, 2, 3, 4):
+            for TY in (4, 12, 64, 200, 512):
+                got = gpu_run(desc, psi, 24, K=K, TY=TY, VEC=vec)
+                assert np.array_equal(got, base), (vec, K, TY)
 
 
-def test_3d_chunking_is_bitwise(gpu_lib):
-    desc = models.model_b((70, 33, 21), h=1.0, dt=0.005, stencil=Stencil3d2h(7, 10, 25),
-                          dtype=F32, path=FUSED)
+def test_narrow_lanes_take_widths_of_two(gpu_lib, orc):
+    """x extent 130 (not a multiple of 4): the fused FP32 path runs with two
+    values per lane; every steps-per-launch gives the same bits and the result
+    follows the FP64 oracle."""
+    desc = models.model_b((130, 48), h=1.0, dt=0.01, dtype=F32, path=FUSED)
+    psi = field(desc, seed=24)
+    say("narrow lanes, X=130")
+    base = gpu_run(desc, psi, 12, K=1, TY=16)
+    for K in (2, 3, 4):
+        assert np.array_equal(gpu_run(desc, psi, 12, K=K, TY=32), base), K
+    with FDPlan(desc) as p:
+        with pytest.raises(abi.SpsError):
+            p.tune(0, 0, 0, 4)
+    d64 = models.model_b((130, 48), h=1.0, dt=0.01, dtype=F64, path=REF)
+    want = oracle_run(orc, d64, psi.astype(np.float32).astype(np.float64), 12)
+    assert rel_l2(base.astype(np.float64), want) <= 1e-6
+
+
+@pytest.mark.parametrize("dtype", [F32, F64])
+@pytest.mark.parametrize("model", ["mb", "ac"])
+def test_3d_chunking_and_fused_steps_are_bitwise(gpu_lib, dtype, model):
+    """Every (steps per launch, rows, planes per chunk) shape gives the bits of
+    one step per launch; 11 steps leave a remainder launch for K = 2."""
+    st = Stencil3d2h(7, 10, 25)
+    if model == "mb":
+        desc = models.model_b((70, 33, 21), h=1.0, dt=0.005, stencil=st, dtype=dtype, path=FUSED)
+    else:
+        desc = models.allen_cahn((70, 33, 21), h=1.0, dt=0.01, stencil=st, dtype=dtype, path=FUSED)
     psi = field(desc, seed=12)
-    say("3D z-chunking and kernel variants")
-    base = gpu_run(desc, psi, 10, TY=8, TZ=32)
-    for rows in (4, 8, 16):
+    say("3D z-chunking, rows and fused steps", model, dtype)
+    base = gpu_run(desc, psi, 11, K=1, TY=8, TZ=32)
+    shapes = [(1, 4), (1, 8), (1, 16), (2, 4)] + ([(2, 8)] if dtype == F32 else [])
+    for K, rows in shapes:
         for TZ in (1, 4, 8, 20, 64):
-            assert np.array_equal(gpu_run(desc, psi, 10, TY=rows, TZ=TZ), base), (rows, TZ)
+            got = gpu_run(desc, psi, 11, K=K, TY=rows, TZ=TZ)
+            assert np.array_equal(got, base), (K, rows, TZ)
+
+
+def test_3d_fused_steps_match_oracle(gpu_lib, orc):
+    """Two steps per launch against the FP64 oracle (FP64 fused <= 1e-12)."""
+    desc = models.model_b((40, 36, 20), h=1.0, dt=0.005, stencil=Stencil3d2h(7, 10, 25),
+                          dtype=F64, path=FUSED)
+    psi = field(desc, seed=23)
+    say("3D K=2 vs oracle")
+    got = gpu_run(desc, psi, 8, K=2, TY=4, TZ=8)
+    want = oracle_run(orc, desc, psi, 8)
+    assert rel_l2(got, want) <= 1e-12
 
 
 def test_graph_replay_matches_eager(gpu_lib):
@@ -263,12 +303,16 @@ def test_small_and_ragged_2d(gpu_lib, orc, shape):
     psi = field(desc, seed=14)
     say("ragged 2D", shape)
     assert np.array_equal(gpu_run(desc, psi, 10), oracle_run(orc, desc, psi, 10))
+    # AUTO fuses when the x extent is a multiple of the narrowest lane width
+    # (FP64: one value per lane, so always; FP32: two values per lane)
     desc = models.model_b(shape, h=1.0, dt=0.01, dtype=F64, path=AUTO)
     with FDPlan(desc) as p:
-        path = p.query()[0]
-    assert path == (FUSED if shape[0] % 2 == 0 else REF)
+        assert p.query()[0] == FUSED
     got = gpu_run(desc, psi, 10)
     assert rel_l2(got, oracle_run(orc, desc, psi, 10)) <= 1e-12
+    d32 = models.model_b(shape, h=1.0, dt=0.01, dtype=F32, path=AUTO)
+    with FDPlan(d32) as p:
+        assert p.query()[0] == (FUSED if shape[0] % 2 == 0 else REF)
 
 
 @pytest.mark.parametrize("shape", [(3, 3, 3), (5, 4, 3), (65, 3, 17)])
@@ -287,7 +331,7 @@ def test_small_and_ragged_3d(gpu_lib, orc, shape):
 
 
 def test_fused_unsupported_width_raises(gpu_lib):
-    desc = models.model_b((30, 32), dtype=F32, path=FUSED)
+    desc = models.model_b((31, 32), dtype=F32, path=FUSED)  # odd: no lane width fits
     with pytest.raises(abi.SpsError) as ei:
         FDPlan(desc)
     assert ei.value.status == abi.E_UNSUPPORTED
@@ -354,12 +398,24 @@ def test_state_and_argument_errors(gpu_lib):
         with pytest.raises(abi.SpsError) as ei:
             p.tune(0, 0, 8)
         assert ei.value.status == abi.E_INVALID
+        for vec in (1, 3, 8):
+            with pytest.raises(abi.SpsError) as ei:
+                p.tune(0, 0, 0, vec)
+            assert ei.value.status == abi.E_INVALID, vec
         p.upload(np.zeros((32, 32), np.float32))
         lib = abi.load()
         out = np.empty((32, 32), np.float32)
         import ctypes
         assert lib.sps_fd_download(p._h, out.ctypes.data_as(ctypes.c_void_p), 7) == abi.E_INVALID
         assert lib.sps_fd_apply(p._h, abi.OP_ID, 0, 0, out.ctypes.data_as(ctypes.c_void_p), 0) == abi.E_INVALID
+    d3 = models.model_b((16, 16, 16), dtype=F64, stencil=Stencil3d2h(7, 10, 25), path=FUSED)
+    with FDPlan(d3) as p:
+        for K, rows, planes, vec in ((2, 16, 0, 0), (2, 8, 0, 0), (3, 4, 0, 0), (1, 12, 0, 0),
+                                     (1, 8, 2000, 0), (1, 8, 0, 2)):
+            with pytest.raises(abi.SpsError) as ei:
+                p.tune(K, rows, planes, vec)
+            assert ei.value.status == abi.E_INVALID, (K, rows, planes, vec)
+        p.tune(2, 4, 8)
 
 
 # --------------------------------------------------------------- full size
@@ -388,7 +444,8 @@ def test_autotune_keeps_field_and_caches(gpu_lib, tmp_path, monkeypatch):
         assert np.array_equal(p.download(), psi)
         p.step(24)
         assert np.array_equal(p.download(), base)
-    assert choice[0] in (1, 2, 3, 4) and choice[1] in (8, 16, 32, 64)
+    assert choice[0] in (1, 2, 3, 4) and choice[1] in (8, 16, 32, 64, 128)
+    assert choice[2] == 0 and choice[3] in (4, 2)
     text = cache.read_text()
     assert "dim=2 n=512x384x1 dtype=f32" in text
     with FDPlan(desc) as p:
@@ -404,8 +461,9 @@ def test_autotune_3d(gpu_lib):
     base = gpu_run(desc, psi, 6, TY=8, TZ=32)
     with FDPlan(desc) as p:
         p.upload(psi)
-        k, rows, planes = p.autotune(use_cache=False)
-        assert k == 1 and rows in (4, 8, 16) and planes in (8, 16, 32, 64)
+        k, rows, planes, vec = p.autotune(use_cache=False)
+        assert (k, rows) in ((1, 4), (1, 8), (1, 16), (2, 4), (2, 8))
+        assert planes in (8, 16, 32, 40) and vec == 1
         assert np.array_equal(p.download(), psi)
         p.step(6)
         assert np.array_equal(p.download(), base)
