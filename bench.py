@@ -44,6 +44,8 @@ def parse():
                     help="skip the other configurations (3D, batched, surrogate)")
     ap.add_argument("--surrogate-iters", type=int, default=300)
     ap.add_argument("--values-per-lane", type=int, default=0)
+    ap.add_argument("--extra-only", choices=("cfg1", "cfg3", "cfg4", "cfg5"),
+                    help="run only this one of the other configurations and print its result")
     ap.add_argument("--shape-sweep", action="store_true",
                     help="only run every launch shape the 2D autotuner tries (steps per "
                          "launch 1..4 x [4 values per lane x rows 8..64, 2 values per lane "
@@ -306,8 +308,32 @@ def cpu_baseline(args):
     return out
 
 
+def extra_configs(torch, stream, args):
+    return (("cfg1_allen_cahn_64_fp64", lambda: extra_cfg1(torch, stream)),
+            ("cfg3_model_b_512cubed_fp32", lambda: extra_cfg3(torch, stream)),
+            ("cfg4_batched_256", lambda: extra_cfg4(torch, stream)),
+            ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters)))
+
+
+def extra_only(args):
+    """Run one of the other configurations alone (for profiling) and print its
+    result as one JSON line."""
+    import torch
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    for name, fn in extra_configs(torch, stream, args):
+        if name.startswith(args.extra_only + "_"):
+            with torch.cuda.stream(stream):
+                print(json.dumps({name: fn()}), flush=True)
+            return
+    raise SystemExit(f"unknown configuration {args.extra_only}")
+
+
 def main():
     args = parse()
+    if args.extra_only:
+        extra_only(args)
+        return
     if args.copy_probe:
         copy_probe()
         return
@@ -406,10 +432,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        for name, fn in (("cfg1_allen_cahn_64_fp64", lambda: extra_cfg1(torch, stream)),
-                         ("cfg3_model_b_512cubed_fp32", lambda: extra_cfg3(torch, stream)),
-                         ("cfg4_batched_256", lambda: extra_cfg4(torch, stream)),
-                         ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters))):
+        for name, fn in extra_configs(torch, stream, args):
             t0 = time.perf_counter()
             try:
                 with torch.cuda.stream(stream):

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SPS_FD_ABI_VERSION 2
+#define SPS_FD_ABI_VERSION 3
 
 /* Ghost width of the SymPhas padded layout (datatypes/inc/gridinfo.h:46). */
 #define SPS_BOUNDARY_DEPTH 3
@@ -46,6 +46,8 @@ extern "C" {
  * copies of psi in the reference (lib/inc/spslibmath.h:255-269). */
 #define SPS_MAX_DEGREE 6
 #define SPS_MAX_TERMS 8
+/* Provisional variables per problem (PROVISIONAL_DEF, sol/inc/modelmacros.h). */
+#define SPS_MAX_PROV 2
 
 typedef enum {
   SPS_OK = 0,
@@ -78,11 +80,13 @@ typedef enum {
  *  REFERENCE: one site per thread, every term evaluated in descriptor order
  *             with the reference's stencil groupings and no fused
  *             multiply-add; the FP64 form reproduces the CPU SolverFT bits.
- *  FUSED:     LDS/register-tiled kernel on the canonical form
- *             dpsi/dt = A(psi) + lap(B(psi) + lap(C(psi))), valid when the
+ *             Provisional variables are evaluated by a pass of their own.
+ *  FUSED:     register-streaming kernels on the canonical form
+ *             dpsi/dt = A(psi) + lap(B(psi)) + kc*bilap(psi), valid when the
  *             bilaplacian stencil equals laplacian∘laplacian (2D <5,*,13>,
- *             3D <7,*,25>) and no gradlaplacian term is present; several
- *             steps may be fused per launch.
+ *             3D <7,*,25>), no gradlaplacian term is present and every
+ *             provisional variable substitutes into that form; several steps
+ *             may be fused per launch.
  *  AUTO:      FUSED when the descriptor allows it, else REFERENCE. */
 typedef enum {
   SPS_PATH_AUTO = 0,
@@ -90,19 +94,21 @@ typedef enum {
   SPS_PATH_FUSED = 2
 } sps_path;
 
-/* One additive term of the EVOLUTION right-hand side:
- *     coeff * D[ P(psi) ],   P(psi) = sum_k poly[k] * psi^k.
+/* One additive term of the EVOLUTION right-hand side (or of a provisional
+ * variable's definition):
+ *     coeff * D[ P(u) ],   P(u) = sum_k poly[k] * u^k,
+ * where the operand u is the field psi or a provisional variable var(k).
  * This is the canonical form SolverFT::form_expr_one produces
  * (solverft.h:112-119): apply_operators distributes derivatives over sums
  * (sym/inc/expressionapply.h:1476-1488) and a derivative of a non-trivial
  * operand is evaluated through a temporary grid
- * (sym/inc/expressionderivatives.h:1545-1593).  When P is exactly psi the
- * stencil reads the field directly (expressionderivatives.h:1658-1682). */
+ * (sym/inc/expressionderivatives.h:1545-1593).  When P is exactly u the
+ * stencil reads the grid directly (expressionderivatives.h:1658-1682). */
 typedef struct {
   int32_t op;        /* sps_op */
   int32_t axis;      /* sps_axis, GRADLAP only: get_stride<axis> convention */
   int32_t component; /* GRADLAP with the 2D 6-point stencil only: 0 or 1 */
-  int32_t reserved;
+  int32_t operand;   /* 0: the field psi; k in 1..nprov: var(k) */
   double coeff;
   double poly[SPS_MAX_DEGREE + 1];
 } sps_fd_term;
@@ -118,6 +124,18 @@ typedef struct {
   double h;          /* grid spacing, the same on every axis */
   double dt;
   sps_fd_term terms[SPS_MAX_TERMS];
+  /* Provisional variables (PROVISIONAL_DEF; MODELBB at
+   * examples/models/modeldefinitions.h:150-155 and tests/testdefs.h:29-37).
+   * var(k) = sum of prov[k-1][0 .. prov_nterms[k-1]), whose operands are psi
+   * or var(j), j < k.  As in the reference, each variable is evaluated over
+   * the interior into a grid of its own after the field's ghosts are
+   * refreshed, and that grid's ghosts are refreshed periodically before the
+   * right-hand side reads it (sol/inc/modelspecialized.h:430-434,
+   * sol/inc/provisionalsystem.h:72-99, solverft.h:163-183). */
+  int32_t nprov;                      /* 0 .. SPS_MAX_PROV */
+  int32_t prov_nterms[SPS_MAX_PROV];  /* 1 .. SPS_MAX_TERMS each */
+  int32_t prov_reserved;
+  sps_fd_term prov[SPS_MAX_PROV][SPS_MAX_TERMS];
 } sps_fd_desc;
 
 typedef struct sps_fd_plan sps_fd_plan;

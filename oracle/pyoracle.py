@@ -46,6 +46,7 @@ def load():
         ("orc_stencil", c_double, [P, c_int, c_int, c_int, P, c_int64]),
         ("orc_apply", None, [P, c_int, c_int, c_int, P, P, c_int]),
         ("orc_equation", c_int, [P, P, P, c_int]),
+        ("orc_provisional", c_int, [P, P, c_int, P, c_int]),
         ("orc_euler", None, [P, P, P, c_int]),
         ("orc_run", c_int, [P, P, c_int, c_int]),
         ("orc_mass", c_double, [P, P]),
@@ -108,6 +109,16 @@ def equation(desc, padded, nthreads=1):
     rc = load().orc_equation(_d(desc), _p(padded), _p(out), nthreads)
     if rc:
         raise RuntimeError(f"orc_equation failed ({rc})")
+    return out
+
+
+def provisional(desc, padded, k, nthreads=1):
+    """var(k) over the padded layout, ghosts refreshed (padded psi must have
+    fresh ghosts)."""
+    out = np.zeros_like(padded)
+    rc = load().orc_provisional(_d(desc), _p(padded), int(k), _p(out), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_provisional failed ({rc})")
     return out
 
 

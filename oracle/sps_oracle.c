@@ -48,14 +48,24 @@ int orc_check(const orc_desc* d) {
   if (!(d->h > 0)) return -3;
   if (!valid_points(d)) return -4;
   if (d->nterms < 0 || d->nterms > ORC_MAX_TERMS) return -5;
-  for (int t = 0; t < d->nterms; ++t) {
-    const orc_term* tm = &d->terms[t];
-    if (tm->op < ORC_OP_ID || tm->op > ORC_OP_GRADLAP) return -6;
-    if (tm->op == ORC_OP_GRADLAP) {
-      if (tm->axis < 0 || tm->axis >= d->dim) return -7;
-      if (tm->component < 0 ||
-          tm->component >= ((d->dim == 2 && d->gradlap_points == 6) ? 2 : 1))
-        return -8;
+  if (d->nprov < 0 || d->nprov > ORC_MAX_PROV) return -9;
+  /* group 0: the right-hand side, operands psi or any var; group k: var(k),
+   * operands psi or var(j), j < k */
+  for (int g = 0; g <= d->nprov; ++g) {
+    const orc_term* terms = g == 0 ? d->terms : d->prov[g - 1];
+    const int nt = g == 0 ? d->nterms : d->prov_nterms[g - 1];
+    if (g > 0 && (nt < 1 || nt > ORC_MAX_TERMS)) return -10;
+    for (int t = 0; t < nt; ++t) {
+      const orc_term* tm = &terms[t];
+      if (tm->op < ORC_OP_ID || tm->op > ORC_OP_GRADLAP) return -6;
+      if (tm->operand < 0 || tm->operand > (g == 0 ? d->nprov : g - 1))
+        return -11;
+      if (tm->op == ORC_OP_GRADLAP) {
+        if (tm->axis < 0 || tm->axis >= d->dim) return -7;
+        if (tm->component < 0 ||
+            tm->component >= ((d->dim == 2 && d->gradlap_points == 6) ? 2 : 1))
+          return -8;
+      }
     }
   }
   return 0;
@@ -579,16 +589,17 @@ void orc_apply(const orc_desc* d, int op, int axis, int component,
   FOR_INTERIOR(d, idx, out[idx] = orc_stencil(d, op, axis, component, g, idx));
 }
 
-int orc_equation(const orc_desc* d, const double* psi, double* dframe,
-                 int nthreads) {
-  if (orc_check(d) != 0) return -1;
+/* sum of terms[0..nterms) over the interior into out; fields[0] = psi,
+ * fields[k] = var(k).  A derivative of a non-trivial operand goes through a
+ * temporary grid filled over the whole padded array
+ * (expressionderivatives.h:1583-1588). */
+static int eval_terms(const orc_desc* d, const orc_term* terms, int nterms,
+                      const double* const* fields, double* out, int nthreads) {
   const size_t len = orc_padded_len(d);
   int nt_ = nthreads_or_default(nthreads);
-  /* temporaries for derivatives of non-trivial operands, each filled over
-   * the whole padded array (expressionderivatives.h:1583-1588) */
   double* tmp[ORC_MAX_TERMS] = {0};
-  for (int t = 0; t < d->nterms; ++t) {
-    const orc_term* tm = &d->terms[t];
+  for (int t = 0; t < nterms; ++t) {
+    const orc_term* tm = &terms[t];
     if (tm->op == ORC_OP_ID || poly_is_identity(tm->poly)) continue;
     tmp[t] = (double*)malloc(len * sizeof(double));
     if (!tmp[t]) {
@@ -597,27 +608,80 @@ int orc_equation(const orc_desc* d, const double* psi, double* dframe,
     }
     double* tt = tmp[t];
     const double* pc = tm->poly;
+    const double* src = fields[tm->operand];
 #pragma omp parallel for schedule(static) num_threads(nt_)
-    for (int64_t i = 0; i < (int64_t)len; ++i) tt[i] = poly_eval(pc, psi[i]);
+    for (int64_t i = 0; i < (int64_t)len; ++i) tt[i] = poly_eval(pc, src[i]);
   }
   FOR_INTERIOR(d, idx, {
     double acc = 0.0;
-    for (int t = 0; t < d->nterms; ++t) {
-      const orc_term* tm = &d->terms[t];
+    for (int t = 0; t < nterms; ++t) {
+      const orc_term* tm = &terms[t];
+      const double* u = fields[tm->operand];
       double val;
       if (tm->op == ORC_OP_ID) {
-        val = tm->coeff * poly_eval(tm->poly, psi[idx]);
+        val = tm->coeff * poly_eval(tm->poly, u[idx]);
       } else {
-        const double* src = tmp[t] ? tmp[t] : psi;
+        const double* src = tmp[t] ? tmp[t] : u;
         val = tm->coeff *
               orc_stencil(d, tm->op, tm->axis, tm->component, src, idx);
       }
       acc = t == 0 ? val : acc + val;
     }
-    dframe[idx] = acc;
+    out[idx] = acc;
   });
-  for (int t = 0; t < d->nterms; ++t) free(tmp[t]);
+  for (int t = 0; t < nterms; ++t) free(tmp[t]);
   return 0;
+}
+
+/* var(1..upto) into vars[0..upto), each refreshed after its evaluation */
+static int eval_provisionals(const orc_desc* d, const double* psi, int upto,
+                             double** vars, int nthreads) {
+  const double* fields[ORC_MAX_PROV + 1] = {psi, 0, 0};
+  for (int k = 1; k <= upto; ++k) {
+    int rc = eval_terms(d, d->prov[k - 1], d->prov_nterms[k - 1], fields,
+                        vars[k - 1], nthreads);
+    if (rc) return rc;
+    orc_refresh(d, vars[k - 1]);
+    fields[k] = vars[k - 1];
+  }
+  return 0;
+}
+
+int orc_provisional(const orc_desc* d, const double* psi, int k, double* out,
+                    int nthreads) {
+  if (orc_check(d) != 0 || k < 1 || k > d->nprov) return -1;
+  const size_t len = orc_padded_len(d);
+  double* vars[ORC_MAX_PROV] = {0};
+  for (int j = 0; j < k - 1; ++j) {
+    vars[j] = (double*)calloc(len, sizeof(double));
+    if (!vars[j]) {
+      for (int u = 0; u < j; ++u) free(vars[u]);
+      return -2;
+    }
+  }
+  vars[k - 1] = out;
+  int rc = eval_provisionals(d, psi, k, vars, nthreads);
+  for (int j = 0; j < k - 1; ++j) free(vars[j]);
+  return rc;
+}
+
+int orc_equation(const orc_desc* d, const double* psi, double* dframe,
+                 int nthreads) {
+  if (orc_check(d) != 0) return -1;
+  const size_t len = orc_padded_len(d);
+  double* vars[ORC_MAX_PROV] = {0};
+  const double* fields[ORC_MAX_PROV + 1] = {psi, 0, 0};
+  int rc = 0;
+  for (int j = 0; j < d->nprov && rc == 0; ++j) {
+    vars[j] = (double*)calloc(len, sizeof(double));
+    if (!vars[j]) rc = -2;
+  }
+  if (rc == 0 && d->nprov > 0)
+    rc = eval_provisionals(d, psi, d->nprov, vars, nthreads);
+  for (int j = 0; j < d->nprov; ++j) fields[j + 1] = vars[j];
+  if (rc == 0) rc = eval_terms(d, d->terms, d->nterms, fields, dframe, nthreads);
+  for (int j = 0; j < d->nprov; ++j) free(vars[j]);
+  return rc;
 }
 
 void orc_euler(const orc_desc* d, double* psi, const double* dframe,

@@ -17,13 +17,23 @@
  *                                  datatypes/inc/dataiterator.h:2443-2450
  *   - forward Euler, interior only examples/solvers/solverft.h:46-50
  *   - sequencing                   modules-models.h:436-492, modules-io.h:450-452
+ *   - provisional variables: each evaluated over the interior into its own
+ *     grid after the field's refresh, then that grid's ghosts refreshed
+ *                                  sol/inc/modelspecialized.h:430-434,
+ *                                  sol/inc/provisionalsystem.h:72-99,
+ *                                  examples/solvers/solverft.h:163-183
  *
  * Parity pins (tests/test_oracle.py): every stencil is checked against the
  * moment conditions of a consistent 2nd-order stencil, the Fourier symbols of
  * lap5/lap9/lap7 and bilap = lap∘lap for <13>/<25>, an independent numpy
  * np.roll implementation on tiny grids, the exact single-mode recursion of
  * dpsi/dt = lap psi, constant-field recursions, mass conservation, and the
- * 2D ghost-strip copies against modular indexing.  The gradlaplacian Z-axis
+ * 2D ghost-strip copies against modular indexing.  Provisional variables are
+ * pinned by MBB (var = P(psi), dpsi = -bilap psi + lap var) matching the
+ * same right-hand side written without a provisional (STATE_SPEED,
+ * tests/testdefs.h:23-26) bit for bit, by BB (var = lap psi + P(psi),
+ * dpsi = -lap var) against the exact single-mode recursion with symbol
+ * -(lambda^2 + a*lambda), and by an np.roll restatement.  The gradlaplacian Z-axis
  * sign and the lap19 "vz + vz" term are reproduced and pinned by special
  * cases, and so is the 2D 21-point bilaplacian's 1/12 factor (stencilh2.h:
  * 143-157), which makes it return a quarter of the bilaplacian.  Parity is
@@ -46,11 +56,13 @@ extern "C" {
 #define ORC_DEPTH 3
 #define ORC_MAX_DEGREE 6
 #define ORC_MAX_TERMS 8
+#define ORC_MAX_PROV 2
 
 enum { ORC_OP_ID = 0, ORC_OP_LAP = 1, ORC_OP_BILAP = 2, ORC_OP_GRADLAP = 3 };
 
+/* operand: 0 = the field psi, k >= 1 = provisional variable var(k) */
 typedef struct {
-  int32_t op, axis, component, reserved;
+  int32_t op, axis, component, operand;
   double coeff;
   double poly[ORC_MAX_DEGREE + 1];
 } orc_term;
@@ -64,6 +76,10 @@ typedef struct {
   int32_t nterms, path;
   double h, dt;
   orc_term terms[ORC_MAX_TERMS];
+  int32_t nprov;
+  int32_t prov_nterms[ORC_MAX_PROV];
+  int32_t prov_reserved;
+  orc_term prov[ORC_MAX_PROV][ORC_MAX_TERMS];
 } orc_desc;
 
 size_t orc_desc_size(void);
@@ -91,7 +107,14 @@ double orc_stencil(const orc_desc* d, int op, int axis, int component,
 void orc_apply(const orc_desc* d, int op, int axis, int component,
                const double* g, double* out, int nthreads);
 
-/* RHS of the descriptor's terms over the interior into dframe (padded). */
+/* Provisional variable var(k) (1-based) over the interior into out (padded),
+ * its ghosts then refreshed periodically: the order of Model::update
+ * (sol/inc/modelspecialized.h:430-434).  psi must have fresh ghosts.
+ * Variables below k are evaluated first, as the reference does in turn. */
+int orc_provisional(const orc_desc* d, const double* psi, int k, double* out,
+                    int nthreads);
+/* RHS of the descriptor's terms over the interior into dframe (padded);
+ * provisional variables, if any, are evaluated first. */
 int orc_equation(const orc_desc* d, const double* psi, double* dframe,
                  int nthreads);
 /* psi = psi + dt*dframe over the interior. */

@@ -12,6 +12,7 @@ from pathlib import Path
 
 MAX_DEGREE = 6
 MAX_TERMS = 8
+MAX_PROV = 2
 BOUNDARY_DEPTH = 3
 
 OK, E_INVALID, E_UNSUPPORTED, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -26,8 +27,9 @@ STATUS_NAMES = {OK: "SPS_OK", E_INVALID: "SPS_E_INVALID", E_UNSUPPORTED: "SPS_E_
 
 
 class Term(ctypes.Structure):
+    # operand 0 is the field psi, k >= 1 the provisional variable var(k)
     _fields_ = [("op", c_int32), ("axis", c_int32), ("component", c_int32),
-                ("reserved", c_int32), ("coeff", c_double),
+                ("operand", c_int32), ("coeff", c_double),
                 ("poly", c_double * (MAX_DEGREE + 1))]
 
 
@@ -35,7 +37,9 @@ class Desc(ctypes.Structure):
     _fields_ = [("dim", c_int32), ("dtype", c_int32), ("n", c_int32 * 3),
                 ("lap_points", c_int32), ("gradlap_points", c_int32),
                 ("bilap_points", c_int32), ("nterms", c_int32), ("path", c_int32),
-                ("h", c_double), ("dt", c_double), ("terms", Term * MAX_TERMS)]
+                ("h", c_double), ("dt", c_double), ("terms", Term * MAX_TERMS),
+                ("nprov", c_int32), ("prov_nterms", c_int32 * MAX_PROV),
+                ("prov_reserved", c_int32), ("prov", (Term * MAX_TERMS) * MAX_PROV)]
 
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libsps_fd.so"

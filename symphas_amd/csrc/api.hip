@@ -75,33 +75,54 @@ int validate_desc(const sps_fd_desc* d) {
     set_error("nterms %d not in [0, %d]", d->nterms, SPS_MAX_TERMS);
     return SPS_E_INVALID;
   }
-  for (int t = 0; t < d->nterms; ++t) {
-    const sps_fd_term& tm = d->terms[t];
-    if (tm.op < SPS_OP_ID || tm.op > SPS_OP_GRADLAP) {
-      set_error("term %d: unknown op %d", t, tm.op);
+  if (d->nprov < 0 || d->nprov > SPS_MAX_PROV) {
+    set_error("nprov %d not in [0, %d]", d->nprov, SPS_MAX_PROV);
+    return SPS_E_INVALID;
+  }
+  // group 0: the right-hand side (operands psi or any var); group g >= 1:
+  // the definition of var(g) (operands psi or var(j), j < g)
+  for (int g = 0; g <= d->nprov; ++g) {
+    const sps_fd_term* terms = g == 0 ? d->terms : d->prov[g - 1];
+    const int nt = g == 0 ? d->nterms : d->prov_nterms[g - 1];
+    const char* what = g == 0 ? "term" : "provisional term";
+    if (g > 0 && (nt < 1 || nt > SPS_MAX_TERMS)) {
+      set_error("var(%d): %d terms, not in [1, %d]", g, nt, SPS_MAX_TERMS);
       return SPS_E_INVALID;
     }
-    if (tm.op == SPS_OP_GRADLAP) {
-      if (tm.axis < 0 || tm.axis >= d->dim) {
-        set_error("term %d: gradlaplacian axis %d", t, tm.axis);
+    for (int t = 0; t < nt; ++t) {
+      const sps_fd_term& tm = terms[t];
+      if (tm.op < SPS_OP_ID || tm.op > SPS_OP_GRADLAP) {
+        set_error("%s %d: unknown op %d", what, t, tm.op);
         return SPS_E_INVALID;
       }
-      const int ncomp = (d->dim == 2 && d->gradlap_points == 6) ? 2 : 1;
-      if (tm.component < 0 || tm.component >= ncomp) {
-        set_error("term %d: component %d (stencil has %d)", t, tm.component,
-                  ncomp);
+      const int max_operand = g == 0 ? d->nprov : g - 1;
+      if (tm.operand < 0 || tm.operand > max_operand) {
+        set_error("%s %d: operand %d not in [0, %d]", what, t, tm.operand,
+                  max_operand);
         return SPS_E_INVALID;
       }
+      if (tm.op == SPS_OP_GRADLAP) {
+        if (tm.axis < 0 || tm.axis >= d->dim) {
+          set_error("%s %d: gradlaplacian axis %d", what, t, tm.axis);
+          return SPS_E_INVALID;
+        }
+        const int ncomp = (d->dim == 2 && d->gradlap_points == 6) ? 2 : 1;
+        if (tm.component < 0 || tm.component >= ncomp) {
+          set_error("%s %d: component %d (stencil has %d)", what, t,
+                    tm.component, ncomp);
+          return SPS_E_INVALID;
+        }
+      }
+      if (!std::isfinite(tm.coeff)) {
+        set_error("%s %d: coefficient not finite", what, t);
+        return SPS_E_INVALID;
+      }
+      for (int k = 0; k <= SPS_MAX_DEGREE; ++k)
+        if (!std::isfinite(tm.poly[k])) {
+          set_error("%s %d: polynomial coefficient %d not finite", what, t, k);
+          return SPS_E_INVALID;
+        }
     }
-    if (!std::isfinite(tm.coeff)) {
-      set_error("term %d: coefficient not finite", t);
-      return SPS_E_INVALID;
-    }
-    for (int k = 0; k <= SPS_MAX_DEGREE; ++k)
-      if (!std::isfinite(tm.poly[k])) {
-        set_error("term %d: polynomial coefficient %d not finite", t, k);
-        return SPS_E_INVALID;
-      }
   }
   if (d->path < SPS_PATH_AUTO || d->path > SPS_PATH_FUSED) {
     set_error("path %d unknown", d->path);
@@ -111,13 +132,65 @@ int validate_desc(const sps_fd_desc* d) {
 }
 
 // Lower the descriptor's terms into dpsi/dt = A + lap(B) + kc*bilap(psi).
+// Provisional variables are substituted first.  A term c*D[var(k)] (D the
+// identity or the laplacian, operand exactly var(k)) expands into c times
+// D applied to each term of var(k)'s definition, whose operands must be psi:
+//   D = id:  v*E[P(psi)]  ->  c*v*E[P(psi)]
+//   D = lap: v*P(psi)     ->  c*v*lap[P(psi)]
+//            v*lap[a*psi] ->  c*v*a*bilap[psi]   (lap5∘lap5 = bilap13 and
+//                             lap7∘lap7 = bilap25, so the fused kernels'
+//                             two 5/7-point passes are the reference's
+//                             evaluation of lap(var) with var = lap psi + ...)
+// Anything else keeps the reference-order path.
 Lowered lower(const sps_fd_desc& d) {
   Lowered lw;
   const int lapL = d.dim == 2 ? 5 : 7;
   const int bilB = d.dim == 2 ? 13 : 25;
   bool any_lap = false, any_bilap = false;
+  std::vector<sps_fd_term> flat;
   for (int t = 0; t < d.nterms; ++t) {
     const sps_fd_term& tm = d.terms[t];
+    if (tm.operand == 0) {
+      flat.push_back(tm);
+      continue;
+    }
+    bool linear = tm.poly[1] == 1.0;
+    for (int k = 0; k <= kMaxDeg; ++k)
+      if (k != 1 && tm.poly[k] != 0.0) linear = false;
+    if (!linear || (tm.op != SPS_OP_ID && tm.op != SPS_OP_LAP)) {
+      lw.why_not = "a provisional variable enters other than as c*var or "
+                   "c*lap(var)";
+      return lw;
+    }
+    const int k = tm.operand;
+    for (int j = 0; j < d.prov_nterms[k - 1]; ++j) {
+      sps_fd_term v = d.prov[k - 1][j];
+      if (v.operand != 0) {
+        lw.why_not = "a provisional variable defined through another";
+        return lw;
+      }
+      v.coeff *= tm.coeff;
+      if (tm.op == SPS_OP_LAP) {
+        if (v.op == SPS_OP_ID) {
+          v.op = SPS_OP_LAP;
+        } else if (v.op == SPS_OP_LAP) {
+          for (int q = 2; q <= kMaxDeg; ++q)
+            if (v.poly[q] != 0.0) {
+              lw.why_not = "laplacian of a laplacian of a non-linear operand";
+              return lw;
+            }
+          any_lap = true;  // needs the 5/7-point laplacian, not bilap_points
+          lw.kc += v.coeff * v.poly[1];
+          continue;
+        } else {
+          lw.why_not = "derivative of order above four";
+          return lw;
+        }
+      }
+      flat.push_back(v);
+    }
+  }
+  for (const sps_fd_term& tm : flat) {
     switch (tm.op) {
       case SPS_OP_ID:
         for (int k = 0; k <= kMaxDeg; ++k) lw.a[k] += tm.coeff * tm.poly[k];
@@ -173,14 +246,17 @@ void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c) {
   c.radius = (lw.kc == 0.0 && degb <= 1) ? 1 : 2;
 }
 
+// Reference-order parameters for one term list: group 0 is the right-hand
+// side, group k >= 1 the definition of var(k).
 template <typename T>
-void make_ref_params(const sps_fd_desc& d, RefParams<T>& p) {
+void make_ref_params(const sps_fd_desc& d, RefParams<T>& p, int group = 0) {
   std::memset(&p, 0, sizeof(p));
   for (int a = 0; a < 3; ++a) p.n[a] = a < d.dim ? d.n[a] : 1;
   p.L = d.lap_points;
   p.G = d.gradlap_points;
   p.B = d.bilap_points;
-  p.nterms = d.nterms;
+  const sps_fd_term* terms = group == 0 ? d.terms : d.prov[group - 1];
+  p.nterms = group == 0 ? d.nterms : d.prov_nterms[group - 1];
   // divh = 1/h; divh2..4 as StencilBase{2,3}d2h builds them
   // (sol/inc/stencilincludes.h:100-104,123-127)
   const double divh = 1.0 / d.h;
@@ -191,12 +267,13 @@ void make_ref_params(const sps_fd_desc& d, RefParams<T>& p) {
   p.divh3 = T(divh3);
   p.divh4 = T(divh4);
   p.dt = T(d.dt);
-  for (int t = 0; t < d.nterms; ++t) {
-    const sps_fd_term& tm = d.terms[t];
+  for (int t = 0; t < p.nterms; ++t) {
+    const sps_fd_term& tm = terms[t];
     DevTerm<T>& dt = p.terms[t];
     dt.op = tm.op;
     dt.axis = tm.axis;
     dt.component = tm.component;
+    dt.operand = tm.operand;
     bool ident = tm.poly[1] == 1.0;
     for (int k = 0; k <= kMaxDeg; ++k) {
       dt.poly[k] = T(tm.poly[k]);
@@ -271,11 +348,27 @@ void padded_to_interior(const sps_fd_desc& d, const char* src, char* dst,
 
 }  // namespace sps
 
+int sps_fd_plan::launch_provisionals() {
+  const void* in = buf[cur];
+  for (int k = 0; k < d.nprov; ++k) {
+    int rc = d.dtype == SPS_F32
+                 ? launch_ref<float>((const float*)in, (float*)vars[k],
+                                     prov32[k], d.dim, stream)
+                 : launch_ref<double>((const double*)in, (double*)vars[k],
+                                      prov64[k], d.dim, stream);
+    if (rc != SPS_OK) return rc;
+  }
+  return SPS_OK;
+}
+
 int sps_fd_plan::launch_one(int k) {
   const void* in = buf[cur];
   void* out = buf[1 - cur];
   int rc;
   if (path == SPS_PATH_REFERENCE) {
+    // provisional variables first, from the current field (Model::update,
+    // sol/inc/modelspecialized.h:430-434), then the step reading them
+    if ((rc = launch_provisionals()) != SPS_OK) return rc;
     rc = d.dtype == SPS_F32
              ? launch_ref<float>((const float*)in, (float*)out, rp32, d.dim,
                                  stream)
@@ -404,6 +497,12 @@ int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
                 : SPS_PATH_FUSED;
   make_ref_params(*desc, p->rp32);
   make_ref_params(*desc, p->rp64);
+  for (int k = 0; k < desc->nprov; ++k) {
+    make_ref_params(*desc, p->prov32[k], k + 1);
+    make_ref_params(*desc, p->prov64[k], k + 1);
+    p->prov32[k].mode = 1;  // out = sum of the terms
+    p->prov64[k].mode = 1;
+  }
   make_fused_coef(*desc, p->low, p->fc32);
   make_fused_coef(*desc, p->low, p->fc64);
   p->nsites = sites_of(*desc);
@@ -427,6 +526,8 @@ int sps_fd_destroy(sps_fd_plan* p) {
   for (void* b : p->buf)
     if (b) hipFree(b);
   if (p->work) hipFree(p->work);
+  for (void* v : p->vars)
+    if (v) hipFree(v);
   if (p->partials) hipFree(p->partials);
   if (p->sums) hipFree(p->sums);
   if (p->own_stream) hipStreamDestroy(p->own_stream);
@@ -506,6 +607,7 @@ int sps_fd_step(sps_fd_plan* p, int nsteps) {
 static int run_ref_mode(sps_fd_plan* p, int mode, int op, int axis,
                         int component) {
   if (!p->work) SPS_HIP_TRY(hipMalloc(&p->work, p->nsites * p->esize));
+  if (mode == 1) SPS_TRY(p->launch_provisionals());
   if (p->d.dtype == SPS_F32) {
     RefParams<float> rp = p->rp32;
     rp.mode = mode;

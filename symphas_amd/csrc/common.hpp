@@ -36,7 +36,8 @@ constexpr int kMaxTerms = SPS_MAX_TERMS;
 // One term in device-friendly form (the reference-order kernels read these).
 template <typename T>
 struct DevTerm {
-  int op, axis, component, identity;  // identity: operand is exactly psi
+  int op, axis, component, identity;  // identity: P is exactly the operand
+  int operand;                        // 0: the kernel's input, k: var(k)
   T coeff;
   T poly[kMaxDeg + 1];
 };
@@ -50,6 +51,7 @@ struct RefParams {
   int mode;  // 0: out = in + dt*rhs; 1: out = rhs; 2: out = D[in] (apply)
   int apply_op, apply_axis, apply_component;
   T divh2, divh3, divh4, dt;
+  const T* vars[SPS_MAX_PROV + 1];  // vars[k]: grid of var(k) (k >= 1)
   DevTerm<T> terms[kMaxTerms];
 };
 

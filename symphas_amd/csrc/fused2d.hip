@@ -17,8 +17,10 @@ __global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
   constexpr int HL = (K * R + VEC - 1) / VEC;
   constexpr int W = (64 - 2 * HL) * VEC;
   const int lane = threadIdx.x & 63;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int item = b * 4 + (threadIdx.x >> 6);
+  // the work item is wave-uniform: keep it (and the chunk bounds and row
+  // loop derived from it) in scalar registers
+  const int item = __builtin_amdgcn_readfirstlane(
+      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
   if (item >= nitems) return;
   const int strip = item % nstrips;
   const int chunk = item / nstrips;

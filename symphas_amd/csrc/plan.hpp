@@ -40,8 +40,11 @@ struct sps_fd_plan {
   bool has_data = false;
   int path = SPS_PATH_REFERENCE;
   sps::Lowered low;
-  sps::RefParams<float> rp32;
+  sps::RefParams<float> rp32;   // right-hand side
   sps::RefParams<double> rp64;
+  void* vars[SPS_MAX_PROV] = {nullptr, nullptr};  // provisional grids
+  sps::RefParams<float> prov32[SPS_MAX_PROV];     // their definitions
+  sps::RefParams<double> prov64[SPS_MAX_PROV];
   sps::FusedCoef<float> fc32;
   sps::FusedCoef<double> fc64;
   int K = 1;    // fused: steps per launch
@@ -51,6 +54,8 @@ struct sps_fd_plan {
   sps::GraphSlot graphs[2];  // indexed by the buffer parity at capture
 
   int launch_one(int k);
+  // reference-order passes that fill the provisional grids from buf[cur]
+  int launch_provisionals();
   int steps_per_launch() const;
   int step(int nsteps);
   void drop_graphs();
@@ -96,6 +101,24 @@ struct sps_fd_plan {
     }
     SPS_HIP_TRY(hipMalloc(&partials, 2 * sps::kReduceBlocks * sizeof(double)));
     SPS_HIP_TRY(hipMalloc(&sums, 2 * sizeof(double)));
+    for (int k = 0; k < d.nprov; ++k) {
+      hipError_t e = hipMalloc(&vars[k], nsites * esize);
+      if (e != hipSuccess) {
+        vars[k] = nullptr;
+        sps::set_error("hipMalloc of a provisional grid failed: %s",
+                       hipGetErrorString(e));
+        return SPS_E_NOMEM;
+      }
+    }
+    // every term list reads var(k) from vars[k - 1]
+    for (int k = 1; k <= d.nprov; ++k) {
+      rp32.vars[k] = (const float*)vars[k - 1];
+      rp64.vars[k] = (const double*)vars[k - 1];
+      for (int j = 0; j < d.nprov; ++j) {
+        prov32[j].vars[k] = (const float*)vars[k - 1];
+        prov64[j].vars[k] = (const double*)vars[k - 1];
+      }
+    }
     return SPS_OK;
   }
 };

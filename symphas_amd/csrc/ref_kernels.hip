@@ -169,14 +169,19 @@ __global__ __launch_bounds__(256) void ref_kernel(const T* __restrict__ in,
     T acc = T(0);
     for (int t = 0; t < p.nterms; ++t) {
       const DevTerm<T>& tm = p.terms[t];
+      // the operand's grid: the field, or a provisional variable's grid,
+      // which the reference refreshes periodically like the field
+      // (sol/inc/modelspecialized.h:430-434); same offsets either way
+      Window<T, DIM> wo = w;
+      if (tm.operand > 0) wo.f = p.vars[tm.operand];
       T val;
       if (tm.op == SPS_OP_ID) {
-        val = tm.coeff * poly_eval(tm.poly, in[idx]);
+        val = tm.coeff * poly_eval(tm.poly, wo.f[idx]);
       } else if (tm.identity) {
-        val = tm.coeff * derivative<T, DIM>(p, w, IdentityOp<T>{}, tm.op,
+        val = tm.coeff * derivative<T, DIM>(p, wo, IdentityOp<T>{}, tm.op,
                                             tm.axis, tm.component);
       } else {
-        val = tm.coeff * derivative<T, DIM>(p, w, PolyOp<T>{tm.poly}, tm.op,
+        val = tm.coeff * derivative<T, DIM>(p, wo, PolyOp<T>{tm.poly}, tm.op,
                                             tm.axis, tm.component);
       }
       acc = t == 0 ? val : acc + val;

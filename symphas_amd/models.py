@@ -43,9 +43,10 @@ class Stencil3d2h:
 class TermSpec:
     op: int
     coeff: float
-    poly: Sequence[float]  # poly[k] multiplies psi^k
+    poly: Sequence[float]  # poly[k] multiplies u^k
     axis: int = 0
     component: int = 0
+    operand: int = 0       # u: 0 the field psi, k >= 1 provisional var(k)
 
 
 PSI = (0.0, 1.0)
@@ -54,12 +55,16 @@ PSI3 = (0.0, 0.0, 0.0, 1.0)
 
 def make_desc(shape: Sequence[int], stencil, h: float, dt: float,
               terms: Sequence[TermSpec], dtype: int = abi.F64,
-              path: int = abi.PATH_AUTO) -> abi.Desc:
-    """shape is the interior extent (nx, ny[, nz]), x fastest."""
+              path: int = abi.PATH_AUTO,
+              provisionals: Sequence[Sequence[TermSpec]] = ()) -> abi.Desc:
+    """shape is the interior extent (nx, ny[, nz]), x fastest.  provisionals[k-1]
+    lists the terms defining var(k) (PROVISIONAL_DEF)."""
     if len(shape) != stencil.dim:
         raise ValueError(f"shape {tuple(shape)} does not match a {stencil.dim}D stencil")
     if len(terms) > abi.MAX_TERMS:
         raise ValueError(f"at most {abi.MAX_TERMS} terms")
+    if len(provisionals) > abi.MAX_PROV:
+        raise ValueError(f"at most {abi.MAX_PROV} provisional variables")
     d = abi.Desc()
     d.dim = stencil.dim
     d.dtype = dtype
@@ -70,14 +75,24 @@ def make_desc(shape: Sequence[int], stencil, h: float, dt: float,
     d.path = path
     d.h = float(h)
     d.dt = float(dt)
-    for i, t in enumerate(terms):
+
+    def fill(dst, t):
         if len(t.poly) > abi.MAX_DEGREE + 1:
             raise ValueError("operand degree above 6")
-        tt = d.terms[i]
-        tt.op, tt.axis, tt.component = t.op, t.axis, t.component
-        tt.coeff = float(t.coeff)
+        dst.op, dst.axis, dst.component, dst.operand = t.op, t.axis, t.component, t.operand
+        dst.coeff = float(t.coeff)
         for k, c in enumerate(t.poly):
-            tt.poly[k] = float(c)
+            dst.poly[k] = float(c)
+
+    for i, t in enumerate(terms):
+        fill(d.terms[i], t)
+    d.nprov = len(provisionals)
+    for k, defn in enumerate(provisionals):
+        if not 1 <= len(defn) <= abi.MAX_TERMS:
+            raise ValueError(f"var({k + 1}) needs 1..{abi.MAX_TERMS} terms")
+        d.prov_nterms[k] = len(defn)
+        for i, t in enumerate(defn):
+            fill(d.prov[k][i], t)
     return d
 
 
@@ -119,6 +134,54 @@ def model_b(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
             path=abi.PATH_AUTO, mobility=1.0, kappa=1.0, c1=1.0, c2=1.0) -> abi.Desc:
     return make_desc(shape, stencil, h, dt, model_b_terms(c1, c2, mobility, kappa),
                      dtype, path)
+
+
+# ------------------------------------------------- provisional variables
+
+def mbb_terms(c1: float = 1.0, c2: float = 1.0):
+    """MBB (examples/models/modeldefinitions.h:150-155):
+        var(1) <= -(c(1) - 4*c(2)*psi*psi)*psi
+        dpsi = -bilap(psi) + lap(var(1))
+    Returns (evolution terms, [definition of var(1)])."""
+    var1 = [TermSpec(abi.OP_ID, 1.0, (0.0, -c1, 0.0, 4.0 * c2))]
+    evo = [TermSpec(abi.OP_BILAP, -1.0, PSI), TermSpec(abi.OP_LAP, 1.0, PSI, operand=1)]
+    return evo, [var1]
+
+
+def state_speed_terms(a: float = 1.0, b: float = 1.0):
+    """STATE_SPEED (tests/testdefs.h:23-26), MBB's right-hand side written
+    without a provisional variable:
+        dpsi = -bilap(psi) + lap(-(A - 4*B*psi*psi)*psi)"""
+    return [TermSpec(abi.OP_BILAP, -1.0, PSI),
+            TermSpec(abi.OP_LAP, 1.0, (0.0, -a, 0.0, 4.0 * b))]
+
+
+def bb_terms(a: float = 1.0, b: float = 1.0):
+    """BB (tests/testdefs.h:29-37), Model B with the laplacian moved into the
+    provisional variable so that no bilaplacian stencil is used:
+        var(1) = lap(psi) + (A - 4*B*psi*psi)*psi
+        dpsi = -lap(var(1))
+    Returns (evolution terms, [definition of var(1)])."""
+    var1 = [TermSpec(abi.OP_LAP, 1.0, PSI), TermSpec(abi.OP_ID, 1.0, (0.0, a, 0.0, -4.0 * b))]
+    evo = [TermSpec(abi.OP_LAP, -1.0, PSI, operand=1)]
+    return evo, [var1]
+
+
+def model_mbb(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+              path=abi.PATH_AUTO, c1=1.0, c2=1.0) -> abi.Desc:
+    evo, prov = mbb_terms(c1, c2)
+    return make_desc(shape, stencil, h, dt, evo, dtype, path, prov)
+
+
+def model_bb(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+             path=abi.PATH_AUTO, a=1.0, b=1.0) -> abi.Desc:
+    evo, prov = bb_terms(a, b)
+    return make_desc(shape, stencil, h, dt, evo, dtype, path, prov)
+
+
+def state_speed(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+                path=abi.PATH_AUTO, a=1.0, b=1.0) -> abi.Desc:
+    return make_desc(shape, stencil, h, dt, state_speed_terms(a, b), dtype, path)
 
 
 def batched_instances(count: int = 256, seed: int = 7,

@@ -29,7 +29,7 @@ def test_descriptor_layout_matches_library_and_oracle(orc):
     lib = abi.load()
     assert lib.sps_fd_desc_size() == ctypes.sizeof(abi.Desc)
     assert orc.load().orc_desc_size() == ctypes.sizeof(abi.Desc)
-    assert lib.sps_fd_abi_version() == 2
+    assert lib.sps_fd_abi_version() == 3
 
 
 def _create(desc):

@@ -405,3 +405,90 @@ def test_generator_known_answer(orc):
     b = orc.fill_uniform(1000, 123, -0.1, 0.1)
     assert np.array_equal(a, b)
     assert a.min() >= -0.1 and a.max() < 0.1
+
+
+# ----------------------------------------------------------- provisional variables
+@pytest.mark.parametrize("stencil,n", [(Stencil2d2h(5, 6, 13), (18, 12)),
+                                       (Stencil2d2h(9, 6, 17), (11, 9)),
+                                       (Stencil3d2h(7, 10, 25), (7, 6, 9))])
+def test_mbb_provisional_equals_state_speed_bitwise(orc, stencil, n):
+    """MBB evaluates var(1) = -(c1 - 4 c2 psi^2) psi into its own grid and
+    refreshes its ghosts; STATE_SPEED differentiates the same polynomial through
+    a temporary grid over the padded array.  The values the laplacian reads
+    are the same numbers, so every step agrees bit for bit."""
+    from symphas_amd import inputs
+    psi = inputs.uniform(tuple(reversed(n)), seed=31, lo=-0.7, hi=0.7)
+    prov = models.model_mbb(n, h=0.8, dt=0.002, stencil=stencil, c1=1.3, c2=0.6)
+    flat = models.state_speed(n, h=0.8, dt=0.002, stencil=stencil, a=1.3, b=0.6)
+    assert np.array_equal(orc.run(prov, psi, 7), orc.run(flat, psi, 7))
+
+
+def test_provisional_grid_has_periodic_ghosts(orc):
+    from symphas_amd import inputs
+    n = (9, 7)
+    psi = inputs.uniform((7, 9), seed=32, lo=-0.9, hi=0.9)
+    desc = models.model_bb(n, h=0.5, a=0.8, b=0.3)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    var = orc.provisional(desc, g, 1)
+    inner = orc.unpad(desc, var)
+    want = _lap5_roll(psi, 0.5) + (0.8 - 4 * 0.3 * psi * psi) * psi
+    np.testing.assert_allclose(inner, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    np.testing.assert_array_equal(var, np.pad(inner, 3, mode="wrap"))
+
+
+@pytest.mark.parametrize("L,h", [(5, 1.0), (9, 0.5)])
+def test_bb_single_mode_recursion(orc, L, h):
+    """BB with B = 0 is linear: var = lap psi + a psi, dpsi = -lap var, so a
+    plane wave with laplacian symbol lam follows (1 - dt (lam^2 + a lam))^n."""
+    nx, ny, dt, a = 16, 12, 0.002, 0.7
+    f, kx, ky = _plane_wave_2d(nx, ny, 2, 1, h)
+    if L == 5:
+        lam = (2 * math.cos(kx) + 2 * math.cos(ky) - 4) / h ** 2
+    else:
+        lam = (4 * math.cos(kx) * math.cos(ky) + 8 * (math.cos(kx) + math.cos(ky)) - 20) / (6 * h * h)
+    desc = models.model_bb((nx, ny), h=h, dt=dt, stencil=Stencil2d2h(L, 6, 13), a=a, b=0.0)
+    out = orc.unpad(desc, orc.run(desc, f, 40))
+    np.testing.assert_allclose(out, (1 - dt * (lam * lam + a * lam)) ** 40 * f, atol=1e-11)
+
+
+def test_bb_matches_numpy_and_model_b(orc):
+    """BB against an np.roll restatement of its two passes, and against Model B
+    written with the 13-point bilaplacian (= lap5 of lap5 exactly, so the two
+    differ by rounding only)."""
+    from symphas_amd import inputs
+    n, h, dt = (14, 10), 0.9, 0.004
+    psi = inputs.uniform((10, 14), seed=33, lo=-0.8, hi=0.8)
+    desc = models.model_bb(n, h=h, dt=dt, a=1.0, b=0.25)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    rhs = orc.unpad(desc, orc.equation(desc, g))
+    var = _lap5_roll(psi, h) + (1.0 - psi * psi) * psi
+    want = -_lap5_roll(var, h)
+    np.testing.assert_allclose(rhs, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    mb = models.model_b(n, h=h, dt=dt)
+    a = orc.unpad(desc, orc.run(desc, psi, 20))
+    b = orc.unpad(mb, orc.run(mb, psi, 20))
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+
+
+def test_bb_3d_matches_numpy(orc):
+    from symphas_amd import inputs
+    n, h = (6, 7, 5), 1.0
+    psi = inputs.uniform((5, 7, 6), seed=34, lo=-0.8, hi=0.8)
+    desc = models.model_bb(n, h=h, stencil=Stencil3d2h(7, 10, 25), a=0.9, b=0.2)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    rhs = orc.unpad(desc, orc.equation(desc, g))
+    want = -_lap7_roll(_lap7_roll(psi, h) + (0.9 - 0.8 * psi * psi) * psi, h)
+    np.testing.assert_allclose(rhs, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_check_rejects_bad_provisionals(orc):
+    d = models.model_bb((8, 8))
+    assert orc.check(d) == 0
+    d.terms[0].operand = 2  # var(2) is not defined
+    assert orc.check(d) != 0
+    d = models.model_bb((8, 8))
+    d.prov[0][0].operand = 1  # var(1) may not read itself
+    assert orc.check(d) != 0
+    d = models.model_bb((8, 8))
+    d.prov_nterms[0] = 0
+    assert orc.check(d) != 0
