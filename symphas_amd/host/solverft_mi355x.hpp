@@ -12,6 +12,10 @@
 //                                               :1263), lowered as apply_operators
 //                                               does (sym/inc/expressionapply.h:
 //                                               1368-1394,1466-1488)
+//   var(k), var(k) <= expr                      provisional variables
+//                                               (PROVISIONAL_DEF, MBB at
+//                                               examples/models/modeldefinitions.h:
+//                                               150-155)
 //   SolverFT_MI355X<stencil_t>                  make_solver / form_expr_one /
 //                                               equation / step
 //   SolverSystemFD_MI355X<D>                    the system: values, dframe, update
@@ -30,6 +34,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -68,17 +73,18 @@ struct Stencil3d2h {
 
 // ---------------------------------------------------------- expressions
 //
-// A small runtime expression tree: the order-parameter field psi, constants,
-// sums, products, powers and derivatives.  lower() brings it to the
-// canonical sum of  coeff * D[ P(psi) ]  terms of sps_fd_term.
+// A small runtime expression tree: the order-parameter field psi, the
+// provisional variables var(k), constants, sums, products, powers and
+// derivatives.  lower() brings it to the canonical sum of
+// coeff * D[ P(u) ] terms of sps_fd_term, u = psi or var(k).
 
 struct Node;
 using NodeP = std::shared_ptr<const Node>;
 
 struct Node {
-  enum Kind { Const, Coeff, Field, Add, Mul, Neg, Deriv } kind;
+  enum Kind { Const, Coeff, Field, Var, Add, Mul, Neg, Deriv } kind;
   double value = 0;  // Const
-  int index = 0;     // Coeff: c(index)
+  int index = 0;     // Coeff: c(index); Var: var(index)
   int order = 0;     // Deriv: 2 lap, 3 gradlap, 4 bilap
   int axis = 0;      // Deriv order 3
   NodeP a, b;
@@ -104,6 +110,32 @@ inline Expr op(int index) {
   auto p = std::make_shared<Node>();
   p->kind = Node::Field;
   return {p};
+}
+
+// var(k): the k-th provisional variable (1-based, k <= SPS_MAX_PROV)
+inline Expr var(int index) {
+  if (index < 1 || index > SPS_MAX_PROV)
+    throw solver_error(SPS_E_UNSUPPORTED,
+                       "var(" + std::to_string(index) + "): at most " +
+                           std::to_string(SPS_MAX_PROV) +
+                           " provisional variables");
+  auto p = std::make_shared<Node>();
+  p->kind = Node::Var;
+  p->index = index;
+  return {p};
+}
+
+// The definition of a provisional variable, written var(k) <= expr as in
+// PROVISIONAL_DEF (examples/models/modeldefinitions.h:153-154).
+struct ProvisionalDef {
+  int index;
+  Expr def;
+};
+inline ProvisionalDef operator<=(Expr v, Expr def) {
+  if (v.n->kind != Node::Var)
+    throw solver_error(SPS_E_INVALID,
+                       "the left side of <= must be a provisional variable");
+  return {v.n->index, def};
 }
 
 inline Expr bin(Node::Kind k, Expr x, Expr y) {
@@ -157,7 +189,7 @@ inline Expr c(int index) {
   return {p};
 }
 
-// A polynomial in psi, coefficients of psi^0 .. psi^6
+// A polynomial in one operand, coefficients of u^0 .. u^6
 struct Poly {
   std::array<double, SPS_MAX_DEGREE + 1> k{};
 };
@@ -177,7 +209,7 @@ inline Poly poly_mul(const Poly& a, const Poly& b) {
       if (a.k[i] != 0.0 && b.k[j] != 0.0) {
         if (i + j > SPS_MAX_DEGREE)
           throw solver_error(SPS_E_UNSUPPORTED,
-                             "power of psi above 6 (pow<N> semantics, "
+                             "power above 6 (pow<N> semantics, "
                              "lib/inc/spslibmath.h:255-269)");
         r.k[i + j] += a.k[i] * b.k[j];
       }
@@ -190,20 +222,25 @@ inline bool is_zero(const Poly& p) {
   return true;
 }
 
-inline bool is_const(const Poly& p) {
-  for (int i = 1; i <= SPS_MAX_DEGREE; ++i)
-    if (p.k[i] != 0.0) return false;
-  return true;
-}
-
-// Lowered value of a sub-expression: a polynomial part plus derivative terms.
+// Lowered value of a sub-expression: a constant, a polynomial per operand
+// (0 = psi, k = var(k); no constant coefficient) and derivative terms keyed
+// by (order, axis, operand).
 struct Value {
-  Poly poly;
-  std::map<std::pair<int, int>, Poly> derivs;
+  double c0 = 0;
+  std::map<int, Poly> poly;
+  std::map<std::tuple<int, int, int>, Poly> derivs;
+
+  bool is_const() const {
+    for (auto& kv : poly)
+      if (!is_zero(kv.second)) return false;
+    return derivs.empty();
+  }
 };
 
 inline Value scale(Value v, double s) {
-  for (double& x : v.poly.k) x *= s;
+  v.c0 *= s;
+  for (auto& kv : v.poly)
+    for (double& x : kv.second.k) x *= s;
   for (auto& kv : v.derivs)
     for (double& x : kv.second.k) x *= s;
   return v;
@@ -213,57 +250,76 @@ inline Value lower_node(const NodeP& n, const std::vector<double>& coeffs) {
   Value v;
   switch (n->kind) {
     case Node::Const:
-      v.poly.k[0] = n->value;
+      v.c0 = n->value;
       return v;
     case Node::Coeff:
-      v.poly.k[0] = (n->index >= 1 && (size_t)n->index <= coeffs.size())
-                        ? coeffs[n->index - 1]
-                        : 1.0;
+      v.c0 = (n->index >= 1 && (size_t)n->index <= coeffs.size())
+                 ? coeffs[n->index - 1]
+                 : 1.0;
       return v;
     case Node::Field:
-      v.poly.k[1] = 1.0;
+      v.poly[0].k[1] = 1.0;
+      return v;
+    case Node::Var:
+      v.poly[n->index].k[1] = 1.0;
       return v;
     case Node::Neg:
       return scale(lower_node(n->a, coeffs), -1.0);
     case Node::Add: {
       Value a = lower_node(n->a, coeffs), b = lower_node(n->b, coeffs);
-      a.poly = poly_add(a.poly, b.poly);
+      a.c0 += b.c0;
+      for (auto& kv : b.poly) a.poly[kv.first] = poly_add(a.poly[kv.first], kv.second);
       for (auto& kv : b.derivs)
         a.derivs[kv.first] = poly_add(a.derivs[kv.first], kv.second);
       return a;
     }
     case Node::Mul: {
       Value a = lower_node(n->a, coeffs), b = lower_node(n->b, coeffs);
+      if (b.is_const()) return scale(a, b.c0);
+      if (a.is_const()) return scale(b, a.c0);
       // a derivative may only be multiplied by a constant
-      if (!a.derivs.empty() && !b.derivs.empty())
-        throw solver_error(SPS_E_UNSUPPORTED, "product of two derivatives");
-      if (!b.derivs.empty()) std::swap(a, b);
-      if (!a.derivs.empty()) {
-        if (!is_const(b.poly))
-          throw solver_error(SPS_E_UNSUPPORTED,
-                             "derivative multiplied by a non-constant");
-        Value r = scale(a, b.poly.k[0]);
-        return r;
-      }
-      v.poly = poly_mul(a.poly, b.poly);
+      if (!a.derivs.empty() || !b.derivs.empty())
+        throw solver_error(SPS_E_UNSUPPORTED,
+                           "derivative multiplied by a non-constant");
+      // a product of polynomials in one operand; psi * var(k) has no term form
+      int operand = -1;
+      for (const Value* x : {&a, &b})
+        for (auto& kv : x->poly) {
+          if (is_zero(kv.second)) continue;
+          if (operand >= 0 && operand != kv.first)
+            throw solver_error(SPS_E_UNSUPPORTED,
+                               "product of two different operands");
+          operand = kv.first;
+        }
+      Poly pa = a.poly[operand], pb = b.poly[operand];
+      pa.k[0] = a.c0;
+      pb.k[0] = b.c0;
+      Poly r = poly_mul(pa, pb);
+      v.c0 = r.k[0];
+      r.k[0] = 0.0;
+      v.poly[operand] = r;
       return v;
     }
     case Node::Deriv: {
       Value in = lower_node(n->a, coeffs);
-      // derivatives distribute over sums (expressionapply.h:1476-1488)
-      Poly operand = in.poly;
-      operand.k[0] = 0.0;  // the derivative of a constant is zero
-      if (!is_zero(operand)) v.derivs[{n->order, n->axis}] = operand;
+      // derivatives distribute over sums (expressionapply.h:1476-1488); the
+      // derivative of a constant is zero
+      for (auto& kv : in.poly)
+        if (!is_zero(kv.second))
+          v.derivs[{n->order, n->axis, kv.first}] =
+              poly_add(v.derivs[{n->order, n->axis, kv.first}], kv.second);
       for (auto& kv : in.derivs) {
         // derivative of a derivative adds orders (expressionapply.h:
         // 1368-1394): lap(lap) = bilap; other combinations are not on the
         // Stencil{2,3}d2h path
-        const int order = n->order + kv.first.first;
-        if (!(n->order == 2 && kv.first.first == 2))
+        const int inner = std::get<0>(kv.first);
+        if (!(n->order == 2 && inner == 2))
           throw solver_error(SPS_E_UNSUPPORTED,
-                             "derivative of order " + std::to_string(order) +
+                             "derivative of order " +
+                                 std::to_string(n->order + inner) +
                                  " other than lap(lap)");
-        v.derivs[{4, 0}] = poly_add(v.derivs[{4, 0}], kv.second);
+        const auto key = std::make_tuple(4, 0, std::get<2>(kv.first));
+        v.derivs[key] = poly_add(v.derivs[key], kv.second);
       }
       return v;
     }
@@ -273,13 +329,14 @@ inline Value lower_node(const NodeP& n, const std::vector<double>& coeffs) {
 
 }  // namespace detail
 
-// Lower an EVOLUTION right-hand side into the descriptor's term list, in the
-// order: bilaplacian, laplacian, gradlaplacian (by axis), pointwise.
+// Lower an EVOLUTION right-hand side (or a provisional definition) into the
+// descriptor's term list, in the order: bilaplacian, laplacian,
+// gradlaplacian (by axis), pointwise; within each, psi before var(1), var(2).
 inline int lower_terms(const Expr& rhs, sps_fd_term* terms, int max_terms,
                        const std::vector<double>& coeffs = {}) {
   detail::Value v = detail::lower_node(rhs.n, coeffs);
   int nt = 0;
-  auto push = [&](int op, int axis, const Poly& p) {
+  auto push = [&](int op, int axis, int operand, const Poly& p) {
     if (detail::is_zero(p)) return;
     if (nt >= max_terms)
       throw solver_error(SPS_E_UNSUPPORTED, "more than 8 terms");
@@ -287,6 +344,7 @@ inline int lower_terms(const Expr& rhs, sps_fd_term* terms, int max_terms,
     std::memset(&t, 0, sizeof(t));
     t.op = op;
     t.axis = axis;
+    t.operand = operand;
     t.coeff = 1.0;
     // single-monomial operands keep their coefficient outside, as
     // OpDerivative's value does (expressionderivatives.h:1545-1593)
@@ -300,25 +358,31 @@ inline int lower_terms(const Expr& rhs, sps_fd_term* terms, int max_terms,
       for (int k = 0; k <= SPS_MAX_DEGREE; ++k) t.poly[k] = p.k[k];
     }
   };
-  for (auto& kv : v.derivs) {
-    if (kv.first.first != 4) continue;
-    push(SPS_OP_BILAP, 0, kv.second);
-  }
-  for (auto& kv : v.derivs) {
-    if (kv.first.first != 2) continue;
+  auto each = [&](int order, auto fn) {
+    for (int operand = 0; operand <= SPS_MAX_PROV; ++operand)
+      for (auto& kv : v.derivs)
+        if (std::get<0>(kv.first) == order && std::get<2>(kv.first) == operand)
+          fn(std::get<1>(kv.first), operand, kv.second);
+  };
+  each(4, [&](int, int operand, const Poly& p) { push(SPS_OP_BILAP, 0, operand, p); });
+  each(2, [&](int, int operand, const Poly& p) {
     // a laplacian of a sum is a sum of laplacians: one term per monomial
     for (int k = 1; k <= SPS_MAX_DEGREE; ++k)
-      if (kv.second.k[k] != 0.0) {
+      if (p.k[k] != 0.0) {
         Poly m;
-        m.k[k] = kv.second.k[k];
-        push(SPS_OP_LAP, 0, m);
+        m.k[k] = p.k[k];
+        push(SPS_OP_LAP, 0, operand, m);
       }
-  }
-  for (auto& kv : v.derivs) {
-    if (kv.first.first != 3) continue;
-    push(SPS_OP_GRADLAP, kv.first.second, kv.second);
-  }
-  push(SPS_OP_ID, 0, v.poly);
+  });
+  each(3, [&](int axis, int operand, const Poly& p) {
+    push(SPS_OP_GRADLAP, axis, operand, p);
+  });
+  // pointwise part: psi's polynomial carries the constant
+  Poly id0 = v.poly.count(0) ? v.poly.at(0) : Poly{};
+  id0.k[0] += v.c0;
+  push(SPS_OP_ID, 0, 0, id0);
+  for (int operand = 1; operand <= SPS_MAX_PROV; ++operand)
+    if (v.poly.count(operand)) push(SPS_OP_ID, 0, operand, v.poly.at(operand));
   return nt;
 }
 
@@ -425,6 +489,7 @@ struct SolverFT_MI355X : stencil_t {
   int dtype = SPS_F64;
   int path = SPS_PATH_AUTO;
   std::vector<double> coeffs;  // c(1), c(2), ... of the model
+  std::vector<ProvisionalDef> provisionals;  // var(1), var(2), ... in order
 
   // SolverFT::make_solver (solverft.h:121-161): dims = count + 2*depth,
   // h from the first system.
@@ -451,6 +516,19 @@ struct SolverFT_MI355X : stencil_t {
     d.h = h;
     d.dt = dt;
     d.nterms = lower_terms(rhs, d.terms, SPS_MAX_TERMS, coeffs);
+    // provisional variables, each lowered like a right-hand side
+    // (PROVISIONAL_DEF; their grids live on the device with the plan)
+    if (provisionals.size() > SPS_MAX_PROV)
+      throw solver_error(SPS_E_UNSUPPORTED, "more than 2 provisional variables");
+    d.nprov = (int)provisionals.size();
+    for (size_t k = 0; k < provisionals.size(); ++k) {
+      if (provisionals[k].index != (int)k + 1)
+        throw solver_error(SPS_E_INVALID,
+                           "provisional definitions must be var(1), var(2), "
+                           "... in order");
+      d.prov_nterms[k] =
+          lower_terms(provisionals[k].def, d.prov[k], SPS_MAX_TERMS, coeffs);
+    }
     if (sys.plan) sps_fd_destroy(sys.plan);
     sys.plan = nullptr;
     check(sps_fd_create(&d, &sys.plan), "sps_fd_create");
@@ -505,7 +583,9 @@ struct SolverFT_MI355X : stencil_t {
 
 // Model<D, Sp> with one field: construction builds the solver and the
 // equation (model.h:100-117, modelspecialized.h:370-375); step/update/
-// equation follow model_iteration (modules-models.h:436-467).
+// equation follow model_iteration (modules-models.h:436-467).  Provisional
+// variables are evaluated before every right-hand side, after the field's
+// refresh, as Specialized::update does (modelspecialized.h:430-434).
 template <size_t D, typename Sp>
 struct Model {
   Sp solver;
@@ -517,11 +597,13 @@ struct Model {
   Expr rhs;
 
   Model(const problem_parameters<D>& p, Expr evolution, int dtype = SPS_F64,
-        int path = SPS_PATH_AUTO, std::vector<double> coeffs = {})
+        int path = SPS_PATH_AUTO, std::vector<double> coeffs = {},
+        std::vector<ProvisionalDef> provisionals = {})
       : solver(Sp::make_solver(p)), system0(p), rhs(evolution) {
     solver.dtype = dtype;
     solver.path = path;
     solver.coeffs = std::move(coeffs);
+    solver.provisionals = std::move(provisionals);
   }
 
   SolverSystemFD_MI355X<D>& system() { return system0; }

@@ -18,7 +18,8 @@ static void print_terms(const char* name, const Expr& e,
   int n = lower_terms(e, t, SPS_MAX_TERMS, coeffs);
   std::printf("%s", name);
   for (int i = 0; i < n; ++i) {
-    std::printf(" | op=%d axis=%d coeff=%.17g poly=", t[i].op, t[i].axis, t[i].coeff);
+    std::printf(" | op=%d axis=%d operand=%d coeff=%.17g poly=", t[i].op, t[i].axis,
+                t[i].operand, t[i].coeff);
     for (int k = 0; k <= SPS_MAX_DEGREE; ++k) std::printf("%s%.17g", k ? "," : "", t[i].poly[k]);
   }
   std::printf("\n");
@@ -84,6 +85,16 @@ int main(int argc, char** argv) {
     print_error("derivative_times_field", [] { auto p = op(1); return lap(p) * p; });
     print_error("power7", [] { auto p = op(1); return power(p, 7); });
     print_error("second_field", [] { return op(2); });
+    // provisional variables: MBB (examples/models/modeldefinitions.h:150-155)
+    // and BB (tests/testdefs.h:29-37) with c(1) = 1.1, c(2) = 0.4
+    const std::vector<double> cf = {1.1, 0.4};
+    print_terms("mbb_evolution", -bilap(psi) + lap(var(1)), cf);
+    print_terms("mbb_var1", -(c(1) - 4.0 * c(2) * psi * psi) * psi, cf);
+    print_terms("bb_evolution", -lap(var(1)), cf);
+    print_terms("bb_var1", lap(psi) + (c(1) - 4.0 * c(2) * psi * psi) * psi, cf);
+    print_terms("mixed_operands", 0.5 * var(2) + psi - 2.0 + lap(var(1) * var(1)));
+    print_error("field_times_var", [] { auto p = op(1); return p * var(1); });
+    print_error("var3", [] { return var(3); });
     return 0;
   }
   if (argc < 4 || std::string(argv[1]) != "run") {
@@ -129,6 +140,27 @@ int main(int argc, char** argv) {
           SPS_PATH_AUTO);
       find_solution(model, 0.005, 40);
       write_interior(model.system(), argv[3]);
+    } else if (which == "mbb" || which == "bb") {
+      // provisional variables, FP64; MBB on the reference-order path (var
+      // grid pass, then the step), BB on whichever path AUTO picks (fused)
+      problem_parameters<2> p;
+      p.dims = {40, 28};
+      p.h = 0.9;
+      p.initial = uniform(40 * 28, 4, -0.5, 0.5);
+      const std::vector<double> cf = {1.1, 0.4};
+      if (which == "mbb") {
+        Model<2, SolverFT_MI355X<Stencil2d2h<5, 6, 13>>> model(
+            p, -bilap(psi) + lap(var(1)), SPS_F64, SPS_PATH_REFERENCE, cf,
+            {var(1) <= -(c(1) - 4.0 * c(2) * psi * psi) * psi});
+        find_solution(model, 0.002, 30);
+        write_interior(model.system(), argv[3]);
+      } else {
+        Model<2, SolverFT_MI355X<Stencil2d2h<5, 6, 13>>> model(
+            p, -lap(var(1)), SPS_F64, SPS_PATH_AUTO, cf,
+            {var(1) <= lap(psi) + (c(1) - 4.0 * c(2) * psi * psi) * psi});
+        find_solution(model, 0.002, 30);
+        write_interior(model.system(), argv[3]);
+      }
     } else {
       std::fprintf(stderr, "unknown case %s\n", which.c_str());
       return 2;

@@ -38,14 +38,16 @@ def _parse(text):
         for t in rest:
             kv = dict(x.split("=", 1) for x in t.split())
             terms.append((int(kv["op"]), int(kv["axis"]), float(kv["coeff"]),
-                          [float(v) for v in kv["poly"].split(",")]))
+                          [float(v) for v in kv["poly"].split(",")], int(kv["operand"])))
         out[name] = terms
     return out
 
 
-def _expected(desc):
-    return [(desc.terms[i].op, desc.terms[i].axis, desc.terms[i].coeff,
-             list(desc.terms[i].poly)) for i in range(desc.nterms)]
+def _expected(desc, group=0):
+    terms = desc.terms if group == 0 else desc.prov[group - 1]
+    n = desc.nterms if group == 0 else desc.prov_nterms[group - 1]
+    return [(terms[i].op, terms[i].axis, terms[i].coeff, list(terms[i].poly), terms[i].operand)
+            for i in range(n)]
 
 
 def test_lowering_matches_model_definitions(driver):
@@ -57,11 +59,23 @@ def test_lowering_matches_model_definitions(driver):
     assert got["model_a"] == _expected(models.make_desc((8, 8), models.Stencil2d2h(), 1, 0.01,
                                                         models.model_a_terms()))
     assert got["model_b_coeffs"] == _expected(models.model_b((8, 8), c1=0.7, c2=1.3))
-    assert got["lap_lap"] == [(abi.OP_BILAP, 0, 1.0, [0, 1, 0, 0, 0, 0, 0]),
-                              (abi.OP_LAP, 0, 0.5, [0, 0, 1, 0, 0, 0, 0])]
-    assert got["gradlap_y"] == [(abi.OP_GRADLAP, 1, 2.0, [0, 0, 1, 0, 0, 0, 0])]
-    for name in ("derivative_product", "derivative_times_field", "power7", "second_field"):
+    assert got["lap_lap"] == [(abi.OP_BILAP, 0, 1.0, [0, 1, 0, 0, 0, 0, 0], 0),
+                              (abi.OP_LAP, 0, 0.5, [0, 0, 1, 0, 0, 0, 0], 0)]
+    assert got["gradlap_y"] == [(abi.OP_GRADLAP, 1, 2.0, [0, 0, 1, 0, 0, 0, 0], 0)]
+    for name in ("derivative_product", "derivative_times_field", "power7", "second_field",
+                 "field_times_var", "var3"):
         assert got[name].startswith("error -2"), name
+    # provisional variables (PROVISIONAL_DEF) lower to the models' term lists
+    mbb = models.model_mbb((8, 8), c1=1.1, c2=0.4)
+    bb = models.model_bb((8, 8), a=1.1, b=0.4)
+    assert got["mbb_evolution"] == _expected(mbb)
+    assert got["mbb_var1"] == _expected(mbb, 1)
+    assert got["bb_evolution"] == _expected(bb)
+    assert got["bb_var1"] == _expected(bb, 1)
+    # 0.5 var(2) + psi - 2 + lap(var(1)^2): one term per operand, psi first
+    assert got["mixed_operands"] == [(abi.OP_LAP, 0, 1.0, [0, 0, 1, 0, 0, 0, 0], 1),
+                                     (abi.OP_ID, 0, 1.0, [-2, 1, 0, 0, 0, 0, 0], 0),
+                                     (abi.OP_ID, 0, 1.0, [0, 0.5, 0, 0, 0, 0, 0], 2)]
 
 
 def _run(driver, case, tmp_path):
@@ -98,3 +112,19 @@ def test_adapter_model_b_3d_fp32(gpu_lib, orc, driver, tmp_path):
     psi = inputs.uniform((16, 24, 32), seed=3, lo=-0.05, hi=0.05).astype(np.float32)
     want = orc.unpad(d, orc.run(d, psi.astype(np.float64), 40, 8))
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-5
+
+
+@pytest.mark.gpu
+def test_adapter_mbb_provisional_bitwise(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "mbb", tmp_path).reshape(28, 40)
+    d = models.model_mbb((40, 28), h=0.9, dt=0.002, c1=1.1, c2=0.4)
+    want = orc.unpad(d, orc.run(d, inputs.uniform((28, 40), seed=4, lo=-0.5, hi=0.5), 30, 8))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_adapter_bb_provisional_fused(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "bb", tmp_path).reshape(28, 40)
+    d = models.model_bb((40, 28), h=0.9, dt=0.002, a=1.1, b=0.4)
+    want = orc.unpad(d, orc.run(d, inputs.uniform((28, 40), seed=4, lo=-0.5, hi=0.5), 30, 8))
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-12
