@@ -44,7 +44,7 @@ def parse():
                     help="skip the other configurations (3D, batched, surrogate)")
     ap.add_argument("--surrogate-iters", type=int, default=300)
     ap.add_argument("--values-per-lane", type=int, default=0)
-    ap.add_argument("--extra-only", choices=("cfg1", "cfg3", "cfg4", "cfg5"),
+    ap.add_argument("--extra-only", choices=("cfg1", "cfg3", "cfg4", "cfg5", "checkpoint"),
                     help="run only this one of the other configurations and print its result")
     ap.add_argument("--shape-sweep", action="store_true",
                     help="only run every launch shape the 2D autotuner tries (steps per "
@@ -308,11 +308,56 @@ def cpu_baseline(args):
     return out
 
 
+def extra_checkpoint(torch, stream):
+    """GNU checkpoint file (the reference's save_grid / read_block format) of
+    the 4096^2 FP32 field, written and read on the GPU, against the
+    reference's algorithm on the CPU (one fprintf("%.10E ") per value,
+    restated in the oracle) for the same field."""
+    import numpy as np
+    from symphas_amd import abi, inputs, models
+    from symphas_amd.solver import FDPlan
+    psi = inputs.uniform((N, N), seed=SEED, dtype=np.float32)
+    d32 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
+    with FDPlan(d32) as p, FDPlan(d32) as q:
+        p.upload(psi)
+        p.checkpoint_text(index=1)  # warm
+        t0 = time.perf_counter()
+        text = p.checkpoint_text(index=10000)
+        t_write = time.perf_counter() - t0
+        q.read_checkpoint(text)
+        t0 = time.perf_counter()
+        idx = q.read_checkpoint(text)
+        t_read = time.perf_counter() - t0
+        same = bool(np.array_equal(q.download(), psi)) and idx == 10000
+    out = {"workload": "GNU checkpoint text of Model B 4096^2 FP32 (16.8M values, %.10E)",
+           "bytes": len(text), "write_s": t_write, "read_s": t_read,
+           "write_Mvalues_per_s": N * N / t_write / 1e6,
+           "read_Mvalues_per_s": N * N / t_read / 1e6,
+           "round_trip_exact": same,
+           "note": "write = length pass + format pass + device-to-host copy into pageable memory; "
+                   "read = host-to-device copy + tokenise + parse"}
+    try:
+        from oracle import pyoracle as orc  # CPU baseline: the reference's loop
+        d64 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F64, path=abi.PATH_REFERENCE)
+        g = orc.pad(d64, psi.astype(np.float64))
+        t0 = time.perf_counter()
+        ref = orc.checkpoint_text(d64, g, index=10000)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"write_s": t_cpu, "cores": 1, "kind": "port",
+                               "byte_identical": ref == text,
+                               "sample": "oracle/sps_oracle.c orc_checkpoint_text, the full field"}
+        out["write_speedup_vs_cpu"] = t_cpu / t_write
+    except Exception as exc:  # reported, never hidden
+        out["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
+    return out
+
+
 def extra_configs(torch, stream, args):
     return (("cfg1_allen_cahn_64_fp64", lambda: extra_cfg1(torch, stream)),
             ("cfg3_model_b_512cubed_fp32", lambda: extra_cfg3(torch, stream)),
             ("cfg4_batched_256", lambda: extra_cfg4(torch, stream)),
-            ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters)))
+            ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters)),
+            ("checkpoint_gnu_text", lambda: extra_checkpoint(torch, stream)))
 
 
 def extra_only(args):

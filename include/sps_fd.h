@@ -220,6 +220,30 @@ int sps_fd_device_field(sps_fd_plan* plan, void** dptr);
  * states without a host round trip. */
 int sps_fd_copy_device(sps_fd_plan* plan, void* dev, int direction);
 
+/* ------------------------------------------------------------- checkpoints */
+
+/* The reference's GNU checkpoint file of the current field, formatted on the
+ * device: the header line of print_gp_header (io/src/writegp.cpp:427-465),
+ * "dim n0 n1 [n2] left0 right0 ... index @", with domain bounds
+ * left = origin[a] (0 when origin is NULL) and right = left + (n[a]-1)*h,
+ * then every value as fprintf("%.10E ") writes it, x fastest, with a newline
+ * after each x row (save_grid, io/src/writegp.cpp:780-803; called from
+ * checkpoint_system, modules-io.h:152-230).  The text is byte-identical to the
+ * C library's.  With text NULL only *length is computed; otherwise text must
+ * hold capacity >= *length bytes (a terminating NUL is added when it fits).
+ * FP32 fields are written as the doubles of their values. */
+int sps_fd_checkpoint_text(sps_fd_plan* plan, int index, const double* origin,
+                           char* text, size_t capacity, size_t* length);
+
+/* Read a GNU checkpoint file into the plan's field, as read_header and
+ * read_block (io/src/readgp.cpp) do with fscanf("%lf"): the header's
+ * dimension and counts must match the plan (a subdomain header is
+ * SPS_E_INVALID); values beyond the field's sites are ignored.  *index (if
+ * not NULL) receives the header's solution index.  Values are parsed on the
+ * device and rounded as strtod rounds; FP32 fields round the double. */
+int sps_fd_checkpoint_read(sps_fd_plan* plan, const char* text, size_t length,
+                           int* index);
+
 /* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 

@@ -52,6 +52,8 @@ def load():
         ("orc_mass", c_double, [P, P]),
         ("orc_energy", c_double, [P, P]),
         ("orc_fill_uniform", None, [P, c_size_t, c_uint64, c_double, c_double]),
+        ("orc_checkpoint_text", c_int64, [P, P, c_int, P, P, c_size_t]),
+        ("orc_checkpoint_read", c_int, [P, ctypes.c_char_p, P, POINTER(c_int)]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res
@@ -143,6 +145,28 @@ def fill_uniform(count, seed, lo, hi):
     out = np.empty(count)
     load().orc_fill_uniform(_p(out), count, seed, lo, hi)
     return out
+
+
+def checkpoint_text(desc, padded, index=0, origin=None):
+    """The reference's GNU checkpoint text of the interior (bytes)."""
+    o = None if origin is None else _p(np.asarray(origin, dtype=np.float64))
+    n = load().orc_checkpoint_text(_d(desc), _p(padded), int(index), o, None, 0)
+    if n < 0:
+        raise RuntimeError("orc_checkpoint_text failed")
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    load().orc_checkpoint_text(_d(desc), _p(padded), int(index), o,
+                               ctypes.cast(buf, c_void_p), int(n) + 1)
+    return buf.raw[:int(n)]
+
+
+def checkpoint_read(desc, text):
+    """(index, padded field) read the way the reference reads a checkpoint."""
+    g = np.zeros(padded_shape(desc))
+    idx = c_int(0)
+    rc = load().orc_checkpoint_read(_d(desc), text, _p(g), ctypes.byref(idx))
+    if rc:
+        raise RuntimeError(f"orc_checkpoint_read failed ({rc})")
+    return idx.value, g
 
 
 def check(desc):

@@ -10,6 +10,7 @@
 #include "sps_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -768,3 +769,82 @@ void orc_fill_uniform(double* out, size_t count, uint64_t seed, double lo,
     out[i] = lo + (hi - lo) * u;
   }
 }
+
+/* ------------------------------------------------------------- checkpoint */
+
+int64_t orc_checkpoint_text(const orc_desc* d, const double* psi, int index,
+                            const double* origin, char* out, size_t cap) {
+  if (orc_check(d) != 0) return -1;
+  size_t n = 0;
+  char tmp[128];
+#define EMIT(...)                                                     \
+  do {                                                                \
+    int k_ = snprintf(tmp, sizeof(tmp), __VA_ARGS__);                 \
+    for (int q_ = 0; q_ < k_; ++q_, ++n)                              \
+      if (n < cap) out[n] = tmp[q_];                                  \
+  } while (0)
+  /* print_gp_header: dimension, counts, domain bounds (x, y, z) */
+  EMIT("%d ", d->dim);
+  for (int a = 0; a < d->dim; ++a) EMIT("%d ", d->n[a]);
+  for (int a = 0; a < d->dim; ++a) {
+    const double left = origin ? origin[a] : 0.0;
+    EMIT("%lf %lf ", left, left + (d->n[a] - 1) * d->h);
+  }
+  EMIT("%d", index);
+  EMIT(" %c", '@');
+  EMIT("\n");
+  /* save_grid: k (z), j (y), i (x) */
+  const int64_t X = pdim(d, 0), Y = pdim(d, 1);
+  const int nz = d->dim == 3 ? d->n[2] : 1;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < d->n[1]; ++j) {
+      const int64_t row =
+          X * ((j + ORC_DEPTH) + Y * (d->dim == 3 ? k + ORC_DEPTH : 0));
+      for (int i = 0; i < d->n[0]; ++i) EMIT("%.10E ", psi[row + i + ORC_DEPTH]);
+      EMIT("\n");
+    }
+#undef EMIT
+  if (n < cap) out[n] = 0;
+  return (int64_t)n;
+}
+
+int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
+                        int* index) {
+  if (orc_check(d) != 0) return -1;
+  const char* p = text;
+  int used = 0, dim = 0;
+  if (sscanf(p, "%d%n", &dim, &used) != 1 || dim != d->dim) return -2;
+  p += used;
+  for (int a = 0; a < dim; ++a) {
+    int c = 0;
+    if (sscanf(p, "%d%n", &c, &used) != 1 || c != d->n[a]) return -3;
+    p += used;
+  }
+  for (int a = 0; a < dim; ++a) {
+    double l, r;
+    if (sscanf(p, "%lf %lf%n", &l, &r, &used) != 2) return -4;
+    p += used;
+  }
+  int idx = 0;
+  if (sscanf(p, "%d%n", &idx, &used) != 1) return -5;
+  p += used;
+  char buf[4] = {0};
+  if (sscanf(p, " %3s%n", buf, &used) != 1 || buf[0] != '@') return -6;
+  p += used;
+  const int64_t X = pdim(d, 0), Y = pdim(d, 1);
+  const int nz = d->dim == 3 ? d->n[2] : 1;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < d->n[1]; ++j) {
+      const int64_t row =
+          X * ((j + ORC_DEPTH) + Y * (d->dim == 3 ? k + ORC_DEPTH : 0));
+      for (int i = 0; i < d->n[0]; ++i) {
+        double v;
+        if (sscanf(p, "%lf%n", &v, &used) != 1) return -7;
+        p += used;
+        psi[row + i + ORC_DEPTH] = v;
+      }
+    }
+  if (index) *index = idx;
+  return 0;
+}
+

@@ -129,6 +129,21 @@ double orc_mass(const orc_desc* d, const double* psi);
  * psi must have fresh ghosts. */
 double orc_energy(const orc_desc* d, const double* psi);
 
+/* The reference's GNU checkpoint file of a field (io/src/writegp.cpp:
+ * print_gp_header :427-465, save_grid :780-803): a header line
+ * "dim n0 n1 [n2] left0 right0 ... index @" (counts with %d, domain bounds
+ * with %lf, right = left + (n-1)*h), then every interior value as "%.10E "
+ * in x-fastest order with "\n" after each x row.  Writes at most cap bytes
+ * into out; returns the full length, or -1 on a bad descriptor.  origin may
+ * be NULL (all zeros).  psi is padded. */
+int64_t orc_checkpoint_text(const orc_desc* d, const double* psi, int index,
+                            const double* origin, char* out, size_t cap);
+/* Read such a file as io/src/readgp.cpp does (read_header, read_block with
+ * fscanf("%lf")): checks the header against d, stores the index and the
+ * values into the interior of psi (padded).  Returns 0, or a negative code. */
+int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
+                        int* index);
+
 /* SplitMix64 stream -> 53-bit uniform doubles in [lo, hi). */
 void orc_fill_uniform(double* out, size_t count, uint64_t seed, double lo,
                       double hi);

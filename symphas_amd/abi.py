@@ -83,6 +83,8 @@ def load() -> ctypes.CDLL:
         "sps_fd_copy_device": (c_int, [P, P, c_int]),
         "sps_fd_batch_copy_device": (c_int, [P, c_int, P, c_int]),
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
+        "sps_fd_checkpoint_text": (c_int, [P, c_int, P, P, c_size_t, POINTER(c_size_t)]),
+        "sps_fd_checkpoint_read": (c_int, [P, ctypes.c_char_p, c_size_t, POINTER(c_int)]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
         "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                     POINTER(c_int)]),

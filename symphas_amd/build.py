@@ -28,6 +28,7 @@ SOURCES = {
     "batch.hip": [],
     "nn.hip": [],
     "tune.hip": [],
+    "textio.hip": [],
 }
 
 COMMON = [

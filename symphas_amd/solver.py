@@ -9,7 +9,7 @@ SymPhas padded layout (datatypes/inc/gridinfo.h:46).
 from __future__ import annotations
 
 import ctypes
-from ctypes import byref, c_double, c_int, c_void_p
+from ctypes import byref, c_double, c_int, c_size_t, c_void_p
 
 import numpy as np
 
@@ -113,6 +113,29 @@ class FDPlan:
         path, spl = c_int(), c_int()
         abi.check(self.lib.sps_fd_query(self._h, byref(path), byref(spl)))
         return path.value, spl.value
+
+    def checkpoint_text(self, index: int = 0, origin=None) -> bytes:
+        """The field as the reference's GNU checkpoint file (formatted on the
+        device; byte-identical to fprintf("%.10E "))."""
+        o = None
+        if origin is not None:
+            o_arr = np.ascontiguousarray(origin, dtype=np.float64)
+            o = o_arr.ctypes.data_as(c_void_p)
+        # upper bound: 19 bytes per value ("-d.ddddddddddE+ddd "), one newline
+        # per row, the header line
+        sites = int(np.prod(interior_shape(self.desc)))
+        cap = 19 * sites + sites // int(self.desc.n[0]) + 4096
+        buf = np.empty(cap, dtype=np.uint8)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fd_checkpoint_text(self._h, int(index), o,
+                                                  buf.ctypes.data_as(c_void_p), cap, byref(n)))
+        return buf[:n.value].tobytes()
+
+    def read_checkpoint(self, text: bytes) -> int:
+        """Load a GNU checkpoint file into the field; returns its index."""
+        idx = c_int(0)
+        abi.check(self.lib.sps_fd_checkpoint_read(self._h, text, len(text), byref(idx)))
+        return idx.value
 
     def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0,
              values_per_lane: int = 0):
