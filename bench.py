@@ -44,7 +44,8 @@ def parse():
                     help="skip the other configurations (3D, batched, surrogate)")
     ap.add_argument("--surrogate-iters", type=int, default=300)
     ap.add_argument("--values-per-lane", type=int, default=0)
-    ap.add_argument("--extra-only", choices=("cfg1", "cfg3", "cfg4", "cfg5", "checkpoint"),
+    ap.add_argument("--extra-only", choices=("cfg1", "cfg3", "cfg4", "cfg5", "checkpoint",
+                                             "coupled"),
                     help="run only this one of the other configurations and print its result")
     ap.add_argument("--shape-sweep", action="store_true",
                     help="only run every launch shape the 2D autotuner tries (steps per "
@@ -352,12 +353,51 @@ def extra_checkpoint(torch, stream):
     return out
 
 
+def extra_coupled(torch, stream, steps=1000):
+    """Two coupled fields (Gray-Scott and Model C, 4096^2 FP32): the fused
+    two-field kernel, autotuned; 16 B of algorithmic traffic per site-update
+    (both fields read once and written once)."""
+    import numpy as np
+    from symphas_amd import abi, inputs, models
+    from symphas_amd.solver import MultiPlan
+    out = {}
+    u = np.ones((N, N), np.float32)
+    v = np.zeros((N, N), np.float32)
+    u[N // 2 - 64:N // 2 + 64, N // 2 - 64:N // 2 + 64] = 0.5
+    v[N // 2 - 64:N // 2 + 64, N // 2 - 64:N // 2 + 64] = 0.25
+    psi = inputs.uniform((N, N), seed=SEED, lo=-0.1, hi=0.1, dtype=np.float32)
+    rho = inputs.uniform((N, N), seed=SEED + 1, lo=-0.1, hi=0.1, dtype=np.float32)
+    for name, d, fields in (
+            ("gray_scott_4096_fp32", models.gray_scott((N, N), dtype=abi.F32, path=abi.PATH_FUSED),
+             (u, v)),
+            ("model_c_4096_fp32", models.model_c((N, N), h=1.0, dt=0.01, dtype=abi.F32,
+                                                 path=abi.PATH_FUSED), (psi, rho))):
+        with MultiPlan(d) as p:
+            p.set_stream(stream.cuda_stream)
+            for f, a in enumerate(fields):
+                p.upload(f, a)
+            k, rows = p.autotune()
+            p.step(8)
+            a_, b_ = _events(torch)
+            a_.record(stream); p.step(steps); b_.record(stream); stream.synchronize()
+            ms = a_.elapsed_time(b_)
+            finite = all(bool(np.isfinite(p.download(f)).all()) for f in range(2))
+        g = N * N * steps / (ms / 1e3) / 1e9
+        out[name] = {"steps": steps, "Gsite_updates_per_s": g,
+                     "launch_shape": {"steps_per_launch": k, "rows_per_chunk": rows},
+                     "roofline": {"achieved_GBs": 16 * g, "peak_GBs": HBM_PEAK_GBS,
+                                  "frac": 16 * g / HBM_PEAK_GBS},
+                     "finite": finite}
+    return out
+
+
 def extra_configs(torch, stream, args):
     return (("cfg1_allen_cahn_64_fp64", lambda: extra_cfg1(torch, stream)),
             ("cfg3_model_b_512cubed_fp32", lambda: extra_cfg3(torch, stream)),
             ("cfg4_batched_256", lambda: extra_cfg4(torch, stream)),
             ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters)),
-            ("checkpoint_gnu_text", lambda: extra_checkpoint(torch, stream)))
+            ("checkpoint_gnu_text", lambda: extra_checkpoint(torch, stream)),
+            ("coupled_fields", lambda: extra_coupled(torch, stream)))
 
 
 def extra_only(args):

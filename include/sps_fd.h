@@ -312,6 +312,67 @@ int sps_fd_batch_autotune(sps_fd_batch* batch, int* steps_per_launch,
  * sps_fd_set_stream). */
 int sps_fd_batch_set_stream(sps_fd_batch* batch, void* hip_stream);
 
+/* ------------------------------------------------------------ coupled fields */
+
+/* Models of several coupled fields (MODEL(MC, (SCALARS(2)), ...), GRAYSCOTT,
+ * Turing: examples/models/modeldefinitions.h:63-81,123-137): each field's
+ * EVOLUTION is a sum of terms coeff * D[ prod_f psi_f^pow[f] ], a derivative
+ * of a monomial of the fields, which is what apply_operators leaves of those
+ * right-hand sides (products distributed, a derivative of a non-trivial
+ * operand through a temporary grid, expressionderivatives.h:1583-1588).  Every
+ * field is stepped with the same SolverFT semantics as a single field: all
+ * right-hand sides from the current fields, forward Euler on each, periodic
+ * refresh of each. */
+#define SPS_MAX_FIELDS 4
+#define SPS_MAX_MTERMS 12
+
+typedef struct {
+  int32_t op;        /* sps_op */
+  int32_t axis;      /* GRADLAP only */
+  int32_t component; /* GRADLAP with the 2D 6-point stencil only */
+  int32_t reserved;
+  double coeff;
+  int32_t pow[SPS_MAX_FIELDS]; /* exponent of each field, 0..6, sum <= 6 */
+} sps_fdm_term;
+
+typedef struct {
+  int32_t dim, dtype;
+  int32_t n[3];
+  int32_t lap_points, gradlap_points, bilap_points;
+  int32_t nfields; /* 1 .. SPS_MAX_FIELDS */
+  int32_t path;    /* sps_path */
+  double h, dt;
+  int32_t nterms[SPS_MAX_FIELDS];
+  sps_fdm_term terms[SPS_MAX_FIELDS][SPS_MAX_MTERMS];
+} sps_fdm_desc;
+
+typedef struct sps_fdm_plan sps_fdm_plan;
+
+/* Create / destroy a coupled-field plan.  FUSED (2D, two fields): each
+ * field's right-hand side lowered to A_f(psi) + lap(B_f(psi)) + kc_f*bilap
+ * (psi_f) with A_f, B_f polynomials of degree <= 3 in each field, 5-point
+ * laplacian, 13-point bilaplacian and no gradlaplacian; anything else runs
+ * the reference-order kernel (FP64: the oracle's bits). */
+int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out);
+int sps_fdm_destroy(sps_fdm_plan* plan);
+/* Field f (0-based) in the interior or padded layout, as sps_fd_upload. */
+int sps_fdm_upload(sps_fdm_plan* plan, int field, const void* host,
+                   int layout);
+int sps_fdm_download(sps_fdm_plan* plan, int field, void* host, int layout);
+int sps_fdm_step(sps_fdm_plan* plan, int nsteps);
+int sps_fdm_query(const sps_fdm_plan* plan, int* path, int* steps_per_launch);
+/* Launch shape of the fused coupled kernel (0 keeps the current value):
+ * steps_per_launch 1..4, rows_per_tile 4..512 (rows per streamed chunk). */
+int sps_fdm_tune(sps_fdm_plan* plan, int steps_per_launch, int rows_per_tile);
+/* Time steps_per_launch {1..4} x rows_per_tile {16,32,64} on the device
+ * (fields saved and restored) and keep the fastest (no reference
+ * counterpart, as sps_fd_autotune). */
+int sps_fdm_autotune(sps_fdm_plan* plan, int* steps_per_launch,
+                     int* rows_per_tile);
+int sps_fdm_sync(sps_fdm_plan* plan);
+/* Caller-owned hipStream_t for later work (NULL: the HIP null stream). */
+int sps_fdm_set_stream(sps_fdm_plan* plan, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,10 @@ def load():
         ("orc_energy", c_double, [P, P]),
         ("orc_fill_uniform", None, [P, c_size_t, c_uint64, c_double, c_double]),
         ("orc_checkpoint_text", c_int64, [P, P, c_int, P, P, c_size_t]),
+        ("orcm_desc_size", c_size_t, []),
+        ("orcm_check", c_int, [P]),
+        ("orcm_equation", c_int, [P, P, P, c_int]),
+        ("orcm_run", c_int, [P, P, c_int, c_int]),
         ("orc_checkpoint_read", c_int, [P, ctypes.c_char_p, P, POINTER(c_int)]),
     ]:
         fn = getattr(lib, name)
@@ -167,6 +171,31 @@ def checkpoint_read(desc, text):
     if rc:
         raise RuntimeError(f"orc_checkpoint_read failed ({rc})")
     return idx.value, g
+
+
+def mrun(mdesc, fields, nsteps, nthreads=1):
+    """Coupled fields: `fields` is a sequence of interior arrays; returns the
+    list of interior arrays after nsteps."""
+    g = [pad(mdesc, f) for f in fields]
+    buf = np.ascontiguousarray(np.stack(g))
+    rc = load().orcm_run(_d(mdesc), _p(buf), int(nsteps), nthreads)
+    if rc:
+        raise RuntimeError(f"orcm_run failed ({rc})")
+    return [unpad(mdesc, buf[f]) for f in range(len(fields))]
+
+
+def mequation(mdesc, fields, nthreads=1):
+    """Right-hand sides of all fields (interior arrays), ghosts refreshed first."""
+    g = np.ascontiguousarray(np.stack([refresh(mdesc, pad(mdesc, f)) for f in fields]))
+    out = np.zeros_like(g)
+    rc = load().orcm_equation(_d(mdesc), _p(g), _p(out), nthreads)
+    if rc:
+        raise RuntimeError(f"orcm_equation failed ({rc})")
+    return [unpad(mdesc, out[f]) for f in range(len(fields))]
+
+
+def mcheck(mdesc):
+    return load().orcm_check(_d(mdesc))
 
 
 def check(desc):

@@ -848,3 +848,146 @@ int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
   return 0;
 }
 
+/* ------------------------------------------------------------- coupled fields */
+
+size_t orcm_desc_size(void) { return sizeof(orcm_desc); }
+
+/* the single-field descriptor with the same grid and stencils */
+static orc_desc orcm_geometry(const orcm_desc* d) {
+  orc_desc g;
+  memset(&g, 0, sizeof(g));
+  g.dim = d->dim;
+  g.dtype = d->dtype;
+  for (int a = 0; a < 3; ++a) g.n[a] = d->n[a];
+  g.lap_points = d->lap_points;
+  g.gradlap_points = d->gradlap_points;
+  g.bilap_points = d->bilap_points;
+  g.path = d->path;
+  g.h = d->h;
+  g.dt = d->dt;
+  return g;
+}
+
+int orcm_check(const orcm_desc* d) {
+  if (!d) return -1;
+  orc_desc g = orcm_geometry(d);
+  int rc = orc_check(&g);
+  if (rc) return rc;
+  if (d->nfields < 1 || d->nfields > ORC_MAX_FIELDS) return -12;
+  for (int f = 0; f < d->nfields; ++f) {
+    if (d->nterms[f] < 0 || d->nterms[f] > ORC_MAX_MTERMS) return -13;
+    for (int t = 0; t < d->nterms[f]; ++t) {
+      const orcm_term* tm = &d->terms[f][t];
+      if (tm->op < ORC_OP_ID || tm->op > ORC_OP_GRADLAP) return -6;
+      int deg = 0;
+      for (int g2 = 0; g2 < ORC_MAX_FIELDS; ++g2) {
+        if (tm->pow[g2] < 0 || (g2 >= d->nfields && tm->pow[g2] != 0)) return -14;
+        deg += tm->pow[g2];
+      }
+      if (deg > ORC_MAX_DEGREE) return -14;
+      if (tm->op == ORC_OP_GRADLAP) {
+        if (tm->axis < 0 || tm->axis >= d->dim) return -7;
+        if (tm->component < 0 ||
+            tm->component >= ((d->dim == 2 && d->gradlap_points == 6) ? 2 : 1))
+          return -8;
+      }
+    }
+  }
+  return 0;
+}
+
+/* left fold of the factors psi_0^p0, psi_1^p1, ... in field order; 1 for the
+ * empty product */
+static double monomial(const orcm_term* tm, const double* fields, size_t len,
+                       int nf, int64_t idx) {
+  double acc = 1.0;
+  int started = 0;
+  for (int f = 0; f < nf; ++f)
+    for (int k = 0; k < tm->pow[f]; ++k) {
+      const double x = fields[(size_t)f * len + idx];
+      acc = started ? acc * x : x;
+      started = 1;
+    }
+  return acc;
+}
+
+/* the field read directly when the monomial is exactly one field, else -1 */
+static int monomial_field(const orcm_term* tm, int nf) {
+  int which = -1, deg = 0;
+  for (int f = 0; f < nf; ++f) {
+    deg += tm->pow[f];
+    if (tm->pow[f] == 1) which = f;
+  }
+  return deg == 1 ? which : -1;
+}
+
+int orcm_equation(const orcm_desc* d, const double* fields, double* dframe,
+                  int nthreads) {
+  if (orcm_check(d) != 0) return -1;
+  const orc_desc g = orcm_geometry(d);
+  const orc_desc* gd = &g;
+  const size_t len = orc_padded_len(gd);
+  const int nf = d->nfields;
+  int nt_ = nthreads_or_default(nthreads);
+  for (int f = 0; f < nf; ++f) {
+    double* tmp[ORC_MAX_MTERMS] = {0};
+    for (int t = 0; t < d->nterms[f]; ++t) {
+      const orcm_term* tm = &d->terms[f][t];
+      if (tm->op == ORC_OP_ID || monomial_field(tm, nf) >= 0) continue;
+      tmp[t] = (double*)malloc(len * sizeof(double));
+      if (!tmp[t]) {
+        for (int u = 0; u < t; ++u) free(tmp[u]);
+        return -2;
+      }
+      double* tt = tmp[t];
+#pragma omp parallel for schedule(static) num_threads(nt_)
+      for (int64_t i = 0; i < (int64_t)len; ++i)
+        tt[i] = monomial(tm, fields, len, nf, i);
+    }
+    double* out = dframe + (size_t)f * len;
+    FOR_INTERIOR(gd, idx, {
+      double acc = 0.0;
+      for (int t = 0; t < d->nterms[f]; ++t) {
+        const orcm_term* tm = &d->terms[f][t];
+        double val;
+        if (tm->op == ORC_OP_ID) {
+          val = tm->coeff * monomial(tm, fields, len, nf, idx);
+        } else {
+          const int which = monomial_field(tm, nf);
+          const double* src =
+              tmp[t] ? tmp[t] : fields + (size_t)(which < 0 ? 0 : which) * len;
+          val = tm->coeff *
+                orc_stencil(gd, tm->op, tm->axis, tm->component, src, idx);
+        }
+        acc = t == 0 ? val : acc + val;
+      }
+      out[idx] = acc;
+    });
+    for (int t = 0; t < d->nterms[f]; ++t) free(tmp[t]);
+  }
+  return 0;
+}
+
+int orcm_run(const orcm_desc* d, double* fields, int nsteps, int nthreads) {
+  if (orcm_check(d) != 0) return -1;
+  const orc_desc g = orcm_geometry(d);
+  const size_t len = orc_padded_len(&g);
+  const int nf = d->nfields;
+  double* dframe = (double*)calloc(len * (size_t)nf, sizeof(double));
+  if (!dframe) return -2;
+  for (int f = 0; f < nf; ++f) orc_refresh(&g, fields + (size_t)f * len);
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = orcm_equation(d, fields, dframe, nthreads);
+    if (rc) {
+      free(dframe);
+      return rc;
+    }
+    for (int f = 0; f < nf; ++f) {
+      orc_euler(&g, fields + (size_t)f * len, dframe + (size_t)f * len, nthreads);
+      orc_refresh(&g, fields + (size_t)f * len);
+    }
+  }
+  free(dframe);
+  return 0;
+}
+

@@ -144,6 +144,44 @@ int64_t orc_checkpoint_text(const orc_desc* d, const double* psi, int index,
 int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
                         int* index);
 
+/* ------------------------------------------------------------ coupled fields
+ * Several fields stepped together (Model C, Gray-Scott, Turing:
+ * examples/models/modeldefinitions.h:63-81,123-137).  Each field's
+ * right-hand side is a sum of coeff * D[prod_f psi_f^pow[f]]; a monomial is
+ * the left-fold product of its factors in field order (psi*rho*rho =
+ * (psi*rho)*rho, as the reference evaluates a written product), a derivative
+ * of a non-trivial monomial goes through a temporary grid over the padded
+ * array, and every field gets Euler and the periodic refresh.  Same layout
+ * as the product's sps_fdm_desc. */
+#define ORC_MAX_FIELDS 4
+#define ORC_MAX_MTERMS 12
+
+typedef struct {
+  int32_t op, axis, component, reserved;
+  double coeff;
+  int32_t pow[ORC_MAX_FIELDS];
+} orcm_term;
+
+typedef struct {
+  int32_t dim, dtype;
+  int32_t n[3];
+  int32_t lap_points, gradlap_points, bilap_points;
+  int32_t nfields, path;
+  double h, dt;
+  int32_t nterms[ORC_MAX_FIELDS];
+  orcm_term terms[ORC_MAX_FIELDS][ORC_MAX_MTERMS];
+} orcm_desc;
+
+size_t orcm_desc_size(void);
+int orcm_check(const orcm_desc* d);
+/* fields: nfields padded arrays back to back (field f at f * padded_len);
+ * dframe likewise. */
+int orcm_equation(const orcm_desc* d, const double* fields, double* dframe,
+                  int nthreads);
+/* Refresh, then nsteps x (all right-hand sides, Euler on each field,
+ * refresh of each field). */
+int orcm_run(const orcm_desc* d, double* fields, int nsteps, int nthreads);
+
 /* SplitMix64 stream -> 53-bit uniform doubles in [lo, hi). */
 void orc_fill_uniform(double* out, size_t count, uint64_t seed, double lo,
                       double hi);

@@ -13,6 +13,8 @@ from pathlib import Path
 MAX_DEGREE = 6
 MAX_TERMS = 8
 MAX_PROV = 2
+MAX_FIELDS = 4
+MAX_MTERMS = 12
 BOUNDARY_DEPTH = 3
 
 OK, E_INVALID, E_UNSUPPORTED, E_HIP, E_NOMEM, E_STATE = 0, -1, -2, -3, -4, -5
@@ -40,6 +42,20 @@ class Desc(ctypes.Structure):
                 ("h", c_double), ("dt", c_double), ("terms", Term * MAX_TERMS),
                 ("nprov", c_int32), ("prov_nterms", c_int32 * MAX_PROV),
                 ("prov_reserved", c_int32), ("prov", (Term * MAX_TERMS) * MAX_PROV)]
+
+
+class MTerm(ctypes.Structure):
+    # coeff * D[prod_f psi_f^pow[f]]
+    _fields_ = [("op", c_int32), ("axis", c_int32), ("component", c_int32),
+                ("reserved", c_int32), ("coeff", c_double), ("pow", c_int32 * MAX_FIELDS)]
+
+
+class MDesc(ctypes.Structure):
+    _fields_ = [("dim", c_int32), ("dtype", c_int32), ("n", c_int32 * 3),
+                ("lap_points", c_int32), ("gradlap_points", c_int32),
+                ("bilap_points", c_int32), ("nfields", c_int32), ("path", c_int32),
+                ("h", c_double), ("dt", c_double), ("nterms", c_int32 * MAX_FIELDS),
+                ("terms", (MTerm * MAX_MTERMS) * MAX_FIELDS)]
 
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libsps_fd.so"
@@ -85,6 +101,16 @@ def load() -> ctypes.CDLL:
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
         "sps_fd_checkpoint_text": (c_int, [P, c_int, P, P, c_size_t, POINTER(c_size_t)]),
         "sps_fd_checkpoint_read": (c_int, [P, ctypes.c_char_p, c_size_t, POINTER(c_int)]),
+        "sps_fdm_create": (c_int, [P, POINTER(c_void_p)]),
+        "sps_fdm_destroy": (c_int, [P]),
+        "sps_fdm_upload": (c_int, [P, c_int, P, c_int]),
+        "sps_fdm_download": (c_int, [P, c_int, P, c_int]),
+        "sps_fdm_step": (c_int, [P, c_int]),
+        "sps_fdm_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
+        "sps_fdm_sync": (c_int, [P]),
+        "sps_fdm_tune": (c_int, [P, c_int, c_int]),
+        "sps_fdm_autotune": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
+        "sps_fdm_set_stream": (c_int, [P, P]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
         "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                     POINTER(c_int)]),

@@ -29,6 +29,8 @@ SOURCES = {
     "nn.hip": [],
     "tune.hip": [],
     "textio.hip": [],
+    "multi_ref.hip": ["-ffp-contract=off"],
+    "multi.hip": [],
 }
 
 COMMON = [

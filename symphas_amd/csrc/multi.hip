@@ -1,0 +1,787 @@
+// multi.hip - coupled-field plans (sps_fdm_*): two-field fused streaming
+// kernel, plan state and the C entry points.
+//
+// The fused kernel streams rows like the single-field one (fused_tile.hpp): a
+// wave owns a strip of 64 lanes x VEC columns, halo lanes feed the
+// neighbours, and K time steps are pipelined per launch; here every stage
+// carries both fields, because each field's right-hand side reads both.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+#include "lanes.hpp"
+#include "multi.hpp"
+#include "plan.hpp"
+
+namespace sps {
+namespace {
+
+// sum_i x^i * sum_j c[i][j] y^j, two-level Horner
+template <typename T>
+__device__ __forceinline__ T bipoly(const T (&c)[kMDeg + 1][kMDeg + 1], T x,
+                                    T y) {
+  T r = T(0);
+#pragma unroll
+  for (int i = kMDeg; i >= 0; --i) {
+    T ci = c[i][kMDeg];
+#pragma unroll
+    for (int j = kMDeg - 1; j >= 0; --j) ci = fmad(ci, y, c[i][j]);
+    r = i == kMDeg ? ci : fmad(r, x, ci);
+  }
+  return r;
+}
+
+template <typename T, int VEC, int R>
+struct Stage2 {
+  T P[2][3][VEC];  // input rows of both fields (slots rotate with the phase)
+  T M[2][3][VEC];  // mu rows (radius 2)
+
+  __device__ __forceinline__ void clear() {
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) P[f][s][j] = M[f][s][j] = T(0);
+  }
+
+  template <int C, int U, int D>
+  __device__ __forceinline__ void diff5(const T (&A)[3][VEC],
+                                        T (&d)[VEC]) const {
+    const T left0 = from_prev_lane<false>(A[C][VEC - 1]);
+    const T rightN = from_next_lane<false>(A[C][0]);
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const T l = j == 0 ? left0 : A[C][j - 1];
+      const T r = j == VEC - 1 ? rightN : A[C][j + 1];
+      d[j] = fmad(T(-4), A[C][j], (l + r) + (A[U][j] + A[D][j]));
+    }
+  }
+
+  template <int PH>
+  __device__ __forceinline__ void push(const MFusedCoef<T>& c,
+                                       T (&v)[2][VEC]) {
+    constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) P[f][S2][j] = v[f][j];
+    if constexpr (R == 1) {
+      T d[2][VEC];
+      diff5<S1, S2, S0>(P[0], d[0]);
+      diff5<S1, S2, S0>(P[1], d[1]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const T x = P[0][S1][j], y = P[1][S1][j];
+        v[0][j] = fmad(c.k1[0], d[0][j], x + bipoly(c.adt[0], x, y));
+        v[1][j] = fmad(c.k1[1], d[1][j], y + bipoly(c.adt[1], x, y));
+      }
+    } else {
+      T d[2][VEC];
+      diff5<S1, S2, S0>(P[0], d[0]);
+      diff5<S1, S2, S0>(P[1], d[1]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const T x = P[0][S1][j], y = P[1][S1][j];
+        M[0][S2][j] = fmad(c.kmu[0], d[0][j], bipoly(c.b[0], x, y));
+        M[1][S2][j] = fmad(c.kmu[1], d[1][j], bipoly(c.b[1], x, y));
+      }
+      diff5<S1, S2, S0>(M[0], d[0]);
+      diff5<S1, S2, S0>(M[1], d[1]);
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) {
+        const T x = P[0][S0][j], y = P[1][S0][j];
+        v[0][j] = fmad(c.kd, d[0][j], x + bipoly(c.adt[0], x, y));
+        v[1][j] = fmad(c.kd, d[1][j], y + bipoly(c.adt[1], x, y));
+      }
+    }
+  }
+};
+
+template <typename T, int VEC, int K, int R>
+__global__ __launch_bounds__(256) void fused_multi(
+    const T* __restrict__ in0, const T* __restrict__ in1, T* __restrict__ out0,
+    T* __restrict__ out1, int X, int Y, int TY, int nstrips, int nitems,
+    MFusedCoef<T> c) {
+  constexpr int HL = (K * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  constexpr int LAG = K * R;
+  const int lane = threadIdx.x & 63;
+  const int item = __builtin_amdgcn_readfirstlane(
+      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (item >= nitems) return;
+  const int strip = item % nstrips;
+  const int chunk = item / nstrips;
+  const int cb = strip * W + (lane - HL) * VEC;
+  int col = cb % X;
+  if (col < 0) col += X;
+  const bool store_lane = lane >= HL && lane < 64 - HL && cb < X;
+  const int y0 = chunk * TY;
+  const int ty = min(TY, Y - y0);
+  const int NI = ty + 2 * LAG;
+  int rload = (y0 - LAG) % Y;
+  if (rload < 0) rload += Y;
+
+  Stage2<T, VEC, R> st[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) st[s].clear();
+
+  T bufA[3][2][VEC], bufB[3][2][VEC];
+  auto load3 = [&](T (&buf)[3][2][VEC], int i) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (i + q < NI) {
+        const int64_t o = (int64_t)rload * X + col;
+        load_vec<VEC>(in0 + o, buf[q][0]);
+        load_vec<VEC>(in1 + o, buf[q][1]);
+        rload = rload + 1 == Y ? 0 : rload + 1;
+      }
+    }
+  };
+  auto process3 = [&](T (&buf)[3][2][VEC], int i) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (i + q < NI) {
+        T v[2][VEC];
+#pragma unroll
+        for (int f = 0; f < 2; ++f)
+#pragma unroll
+          for (int j = 0; j < VEC; ++j) v[f][j] = buf[q][f][j];
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+          // stage s has valid input from stream row 2*R*s on (fill rows
+          // before that are skipped; valid outputs never read them)
+          if (s > 0 && i + q < 2 * R * s) continue;
+          if (q == 0) st[s].template push<0>(c, v);
+          if (q == 1) st[s].template push<1>(c, v);
+          if (q == 2) st[s].template push<2>(c, v);
+        }
+        const int orow = i + q - 2 * LAG;
+        if (orow >= 0 && store_lane) {
+          const int64_t o = (int64_t)(y0 + orow) * X + col;
+          store_vec<VEC>(out0 + o, v[0]);
+          store_vec<VEC>(out1 + o, v[1]);
+        }
+      }
+    }
+  };
+  load3(bufA, 0);
+  for (int i = 0; i < NI; i += 6) {
+    load3(bufB, i + 3);
+    process3(bufA, i);
+    load3(bufA, i + 6);
+    process3(bufB, i + 3);
+  }
+}
+
+template <typename T, int K, int R>
+int launch_multi_k(const T* const* in, T* const* out, int X, int Y, int TY,
+                   const MFusedCoef<T>& c, hipStream_t s) {
+  constexpr int VEC = 8 / sizeof(T);
+  constexpr int HL = (K * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  const int nstrips = (X + W - 1) / W;
+  const int nchunks = (Y + TY - 1) / TY;
+  const int nitems = nstrips * nchunks;
+  fused_multi<T, VEC, K, R><<<(nitems + 3) / 4, 256, 0, s>>>(
+      in[0], in[1], out[0], out[1], X, Y, TY, nstrips, nitems, c);
+  return SPS_OK;
+}
+
+}  // namespace
+
+template <typename T>
+int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
+                       int TY, const MFusedCoef<T>& c, hipStream_t s) {
+  if (X % (8 / (int)sizeof(T)) != 0 || TY < 1) {
+    set_error("fused coupled step: X=%d must be a multiple of %d", X,
+              8 / (int)sizeof(T));
+    return SPS_E_UNSUPPORTED;
+  }
+  const bool r1 = c.radius == 1;
+  int rc;
+  switch (K) {
+    case 1: rc = r1 ? launch_multi_k<T, 1, 1>(in, out, X, Y, TY, c, s)
+                    : launch_multi_k<T, 1, 2>(in, out, X, Y, TY, c, s); break;
+    case 2: rc = r1 ? launch_multi_k<T, 2, 1>(in, out, X, Y, TY, c, s)
+                    : launch_multi_k<T, 2, 2>(in, out, X, Y, TY, c, s); break;
+    case 3: rc = r1 ? launch_multi_k<T, 3, 1>(in, out, X, Y, TY, c, s)
+                    : launch_multi_k<T, 3, 2>(in, out, X, Y, TY, c, s); break;
+    case 4: rc = r1 ? launch_multi_k<T, 4, 1>(in, out, X, Y, TY, c, s)
+                    : launch_multi_k<T, 4, 2>(in, out, X, Y, TY, c, s); break;
+    default:
+      set_error("fused coupled step: steps per launch %d not in 1..4", K);
+      return SPS_E_INVALID;
+  }
+  if (rc != SPS_OK) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("fused coupled launch failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+template int launch_fused_multi<float>(const float* const*, float* const*, int,
+                                       int, int, int, const MFusedCoef<float>&,
+                                       hipStream_t);
+template int launch_fused_multi<double>(const double* const*, double* const*,
+                                        int, int, int, int,
+                                        const MFusedCoef<double>&, hipStream_t);
+
+}  // namespace sps
+
+// ------------------------------------------------------------------ plans
+
+struct sps_fdm_plan {
+  sps_fdm_desc d;
+  sps::MLowered low;
+  int path = SPS_PATH_REFERENCE;
+  hipStream_t stream = nullptr, own_stream = nullptr;
+  size_t nsites = 0, esize = 0;
+  void* buf[SPS_MAX_FIELDS][2] = {};
+  int cur = 0;
+  bool has_data = false;
+  int K = 1, TY = 32;
+  sps::MRefParams<float> rp32;
+  sps::MRefParams<double> rp64;
+  sps::MFusedCoef<float> fc32;
+  sps::MFusedCoef<double> fc64;
+
+  int launch_one(int k);
+  int steps_per_launch() const { return path == SPS_PATH_FUSED ? K : 1; }
+};
+
+namespace sps {
+namespace {
+
+int validate_mdesc(const sps_fdm_desc* d) {
+  if (!d) {
+    set_error("descriptor is NULL");
+    return SPS_E_INVALID;
+  }
+  // geometry and stencil checks of the single-field descriptor
+  sps_fd_desc g;
+  std::memset(&g, 0, sizeof(g));
+  g.dim = d->dim;
+  g.dtype = d->dtype;
+  for (int a = 0; a < 3; ++a) g.n[a] = d->n[a];
+  g.lap_points = d->lap_points;
+  g.gradlap_points = d->gradlap_points;
+  g.bilap_points = d->bilap_points;
+  g.path = d->path;
+  g.h = d->h;
+  g.dt = d->dt;
+  SPS_TRY(validate_desc(&g));
+  if (d->nfields < 1 || d->nfields > SPS_MAX_FIELDS) {
+    set_error("nfields %d not in [1, %d]", d->nfields, SPS_MAX_FIELDS);
+    return SPS_E_INVALID;
+  }
+  for (int f = 0; f < d->nfields; ++f) {
+    if (d->nterms[f] < 0 || d->nterms[f] > SPS_MAX_MTERMS) {
+      set_error("field %d: nterms %d not in [0, %d]", f, d->nterms[f],
+                SPS_MAX_MTERMS);
+      return SPS_E_INVALID;
+    }
+    for (int t = 0; t < d->nterms[f]; ++t) {
+      const sps_fdm_term& tm = d->terms[f][t];
+      if (tm.op < SPS_OP_ID || tm.op > SPS_OP_GRADLAP) {
+        set_error("field %d term %d: unknown op %d", f, t, tm.op);
+        return SPS_E_INVALID;
+      }
+      int deg = 0;
+      for (int g2 = 0; g2 < SPS_MAX_FIELDS; ++g2) {
+        if (tm.pow[g2] < 0 || (g2 >= d->nfields && tm.pow[g2] != 0)) {
+          set_error("field %d term %d: exponent of field %d invalid", f, t, g2);
+          return SPS_E_INVALID;
+        }
+        deg += tm.pow[g2];
+      }
+      if (deg > SPS_MAX_DEGREE) {
+        set_error("field %d term %d: degree %d above %d", f, t, deg,
+                  SPS_MAX_DEGREE);
+        return SPS_E_INVALID;
+      }
+      if (tm.op == SPS_OP_GRADLAP) {
+        const int ncomp = (d->dim == 2 && d->gradlap_points == 6) ? 2 : 1;
+        if (tm.axis < 0 || tm.axis >= d->dim || tm.component < 0 ||
+            tm.component >= ncomp) {
+          set_error("field %d term %d: gradlaplacian axis/component", f, t);
+          return SPS_E_INVALID;
+        }
+      }
+      if (!std::isfinite(tm.coeff)) {
+        set_error("field %d term %d: coefficient not finite", f, t);
+        return SPS_E_INVALID;
+      }
+    }
+  }
+  return SPS_OK;
+}
+
+MLowered lower_multi(const sps_fdm_desc& d) {
+  MLowered lw;
+  if (d.dim != 2 || d.nfields != 2) {
+    lw.why_not = "the fused coupled kernel is 2D with two fields";
+    return lw;
+  }
+  bool any_lap = false, any_bilap = false;
+  for (int f = 0; f < 2; ++f)
+    for (int t = 0; t < d.nterms[f]; ++t) {
+      const sps_fdm_term& tm = d.terms[f][t];
+      const int p0 = tm.pow[0], p1 = tm.pow[1];
+      if (p0 > kMDeg || p1 > kMDeg) {
+        lw.why_not = "a field power above 3";
+        return lw;
+      }
+      switch (tm.op) {
+        case SPS_OP_ID:
+          lw.a[f][p0][p1] += tm.coeff;
+          break;
+        case SPS_OP_LAP:
+          any_lap = true;
+          if (p0 + p1 > 0) lw.b[f][p0][p1] += tm.coeff;
+          break;
+        case SPS_OP_BILAP:
+          if (tm.pow[f] != 1 || p0 + p1 != 1) {
+            lw.why_not = "bilaplacian of other than the field itself";
+            return lw;
+          }
+          any_bilap = true;
+          lw.kc[f] += tm.coeff;
+          break;
+        default:
+          lw.why_not = "gradlaplacian terms";
+          return lw;
+      }
+    }
+  if (any_lap && d.lap_points != 5) {
+    lw.why_not = "laplacian stencil is not the 5-point one";
+    return lw;
+  }
+  if (any_bilap && d.bilap_points != 13) {
+    lw.why_not = "bilaplacian stencil is not the 13-point one";
+    return lw;
+  }
+  // radius one when every B_f is b_f * psi_f and there is no bilaplacian
+  bool r1 = lw.kc[0] == 0.0 && lw.kc[1] == 0.0;
+  for (int f = 0; f < 2 && r1; ++f)
+    for (int i = 0; i <= kMDeg; ++i)
+      for (int j = 0; j <= kMDeg; ++j) {
+        const bool own = (f == 0 && i == 1 && j == 0) || (f == 1 && i == 0 && j == 1);
+        if (!own && lw.b[f][i][j] != 0.0) r1 = false;
+      }
+  lw.radius = r1 ? 1 : 2;
+  lw.fusable = true;
+  return lw;
+}
+
+template <typename T>
+void make_mfused_coef(const sps_fdm_desc& d, const MLowered& lw,
+                      MFusedCoef<T>& c) {
+  std::memset(&c, 0, sizeof(c));
+  const double divh = 1.0 / d.h, divh2 = divh * divh;
+  for (int f = 0; f < 2; ++f) {
+    for (int i = 0; i <= kMDeg; ++i)
+      for (int j = 0; j <= kMDeg; ++j) {
+        c.adt[f][i][j] = T(d.dt * lw.a[f][i][j]);
+        c.b[f][i][j] = T(lw.b[f][i][j]);
+      }
+    c.kmu[f] = T(lw.kc[f] * divh2);
+    const double bown = f == 0 ? lw.b[0][1][0] : lw.b[1][0][1];
+    c.k1[f] = T(d.dt * divh2 * bown);
+  }
+  c.kd = T(d.dt * divh2);
+  c.radius = lw.radius;
+}
+
+template <typename T>
+void make_mref_params(const sps_fdm_desc& d, MRefParams<T>& p) {
+  std::memset(&p, 0, sizeof(p));
+  for (int a = 0; a < 3; ++a) p.n[a] = a < d.dim ? d.n[a] : 1;
+  p.L = d.lap_points;
+  p.G = d.gradlap_points;
+  p.B = d.bilap_points;
+  p.nfields = d.nfields;
+  const double divh = 1.0 / d.h, divh2 = divh * divh;
+  p.divh2 = T(divh2);
+  p.divh3 = T(divh * divh2);
+  p.divh4 = T(divh2 * divh2);
+  p.dt = T(d.dt);
+  for (int f = 0; f < d.nfields; ++f) {
+    p.nterms[f] = d.nterms[f];
+    for (int t = 0; t < d.nterms[f]; ++t) {
+      const sps_fdm_term& tm = d.terms[f][t];
+      MDevTerm<T>& dt = p.terms[f][t];
+      dt.op = tm.op;
+      dt.axis = tm.axis;
+      dt.component = tm.component;
+      int deg = 0, which = -1;
+      for (int g2 = 0; g2 < SPS_MAX_FIELDS; ++g2) {
+        dt.pow[g2] = tm.pow[g2];
+        deg += tm.pow[g2];
+        if (tm.pow[g2] == 1) which = g2;
+      }
+      dt.single = deg == 1 ? which : -1;
+      dt.coeff = T(tm.coeff);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace sps
+
+int sps_fdm_plan::launch_one(int k) {
+  using namespace sps;
+  int rc;
+  if (path == SPS_PATH_REFERENCE) {
+    if (d.dtype == SPS_F32) {
+      MRefParams<float> p = rp32;
+      for (int f = 0; f < d.nfields; ++f) {
+        p.in[f] = (const float*)buf[f][cur];
+        p.out[f] = (float*)buf[f][1 - cur];
+      }
+      rc = launch_ref_multi<float>(p, d.dim, stream);
+    } else {
+      MRefParams<double> p = rp64;
+      for (int f = 0; f < d.nfields; ++f) {
+        p.in[f] = (const double*)buf[f][cur];
+        p.out[f] = (double*)buf[f][1 - cur];
+      }
+      rc = launch_ref_multi<double>(p, d.dim, stream);
+    }
+    if (rc == SPS_OK) {
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) {
+        set_error("coupled reference launch failed: %s", hipGetErrorString(e));
+        rc = SPS_E_HIP;
+      }
+    }
+  } else if (d.dtype == SPS_F32) {
+    const float* in[2] = {(const float*)buf[0][cur], (const float*)buf[1][cur]};
+    float* out[2] = {(float*)buf[0][1 - cur], (float*)buf[1][1 - cur]};
+    rc = launch_fused_multi<float>(in, out, d.n[0], d.n[1], k, TY, fc32, stream);
+  } else {
+    const double* in[2] = {(const double*)buf[0][cur],
+                           (const double*)buf[1][cur]};
+    double* out[2] = {(double*)buf[0][1 - cur], (double*)buf[1][1 - cur]};
+    rc = launch_fused_multi<double>(in, out, d.n[0], d.n[1], k, TY, fc64,
+                                    stream);
+  }
+  if (rc == SPS_OK) cur = 1 - cur;
+  return rc;
+}
+
+using namespace sps;
+
+extern "C" {
+
+int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out) {
+  if (!out) {
+    set_error("output pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  *out = nullptr;
+  SPS_TRY(validate_mdesc(desc));
+  if (sps_fd_device_count() < 1) {
+    set_error("no HIP device: this library runs the step on an MI355X only");
+    return SPS_E_HIP;
+  }
+  sps_fdm_plan* p = new (std::nothrow) sps_fdm_plan();
+  if (!p) {
+    set_error("host allocation failed");
+    return SPS_E_NOMEM;
+  }
+  p->d = *desc;
+  p->esize = desc->dtype == SPS_F32 ? 4 : 8;
+  p->nsites = (size_t)desc->n[0] * desc->n[1] * (desc->dim == 3 ? desc->n[2] : 1);
+  p->low = lower_multi(*desc);
+  const int vec = 8 / (int)p->esize;
+  const bool fusable = p->low.fusable && desc->n[0] % vec == 0;
+  if (desc->path == SPS_PATH_FUSED && !fusable) {
+    set_error("fused coupled path unavailable: %s",
+              p->low.fusable ? "x extent not a multiple of the lane width"
+                             : p->low.why_not);
+    delete p;
+    return SPS_E_UNSUPPORTED;
+  }
+  p->path = (desc->path == SPS_PATH_REFERENCE || !fusable) ? SPS_PATH_REFERENCE
+                                                           : SPS_PATH_FUSED;
+  make_mref_params(*desc, p->rp32);
+  make_mref_params(*desc, p->rp64);
+  if (p->path == SPS_PATH_FUSED) {
+    make_mfused_coef(*desc, p->low, p->fc32);
+    make_mfused_coef(*desc, p->low, p->fc64);
+    p->K = p->low.radius == 1 ? 4 : 2;
+    p->TY = 32;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("stream creation failed: %s", hipGetErrorString(e));
+    delete p;
+    return SPS_E_HIP;
+  }
+  p->stream = p->own_stream;
+  for (int f = 0; f < desc->nfields; ++f)
+    for (int i = 0; i < 2; ++i) {
+      e = hipMalloc(&p->buf[f][i], p->nsites * p->esize);
+      if (e != hipSuccess) {
+        p->buf[f][i] = nullptr;
+        set_error("hipMalloc of %zu bytes failed: %s", p->nsites * p->esize,
+                  hipGetErrorString(e));
+        sps_fdm_destroy(p);
+        return SPS_E_NOMEM;
+      }
+    }
+  *out = p;
+  return SPS_OK;
+}
+
+int sps_fdm_destroy(sps_fdm_plan* p) {
+  if (!p) return SPS_OK;
+  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->own_stream && p->own_stream != p->stream)
+    hipStreamSynchronize(p->own_stream);
+  for (auto& fb : p->buf)
+    for (void* b : fb)
+      if (b) hipFree(b);
+  if (p->own_stream) hipStreamDestroy(p->own_stream);
+  delete p;
+  return SPS_OK;
+}
+
+static int check_field(const sps_fdm_plan* p, int field, const void* host,
+                       int layout) {
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (field < 0 || field >= p->d.nfields) {
+    set_error("field %d not in [0, %d)", field, p->d.nfields);
+    return SPS_E_INVALID;
+  }
+  if (layout != SPS_LAYOUT_INTERIOR && layout != SPS_LAYOUT_PADDED) {
+    set_error("unknown layout %d", layout);
+    return SPS_E_INVALID;
+  }
+  return SPS_OK;
+}
+
+// interior <-> padded copies on the host (ghosts written periodic)
+static void padded_to_interior_m(const sps_fdm_desc& d, const char* in,
+                                 char* out, size_t es) {
+  const int D = SPS_BOUNDARY_DEPTH;
+  const int64_t X = d.n[0] + 2 * D, Y = d.n[1] + 2 * D;
+  const int nz = d.dim == 3 ? d.n[2] : 1;
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < d.n[1]; ++y) {
+      const int64_t src = ((int64_t)(d.dim == 3 ? z + D : 0) * Y + (y + D)) * X + D;
+      const int64_t dst = ((int64_t)z * d.n[1] + y) * d.n[0];
+      std::memcpy(out + dst * es, in + src * es, (size_t)d.n[0] * es);
+    }
+}
+
+static void interior_to_padded_m(const sps_fdm_desc& d, const char* in,
+                                 char* out, size_t es) {
+  const int D = SPS_BOUNDARY_DEPTH;
+  const int64_t X = d.n[0] + 2 * D, Y = d.n[1] + 2 * D;
+  const int64_t Z = d.dim == 3 ? d.n[2] + 2 * D : 1;
+  const int64_t nx = d.n[0], ny = d.n[1], nz = d.dim == 3 ? d.n[2] : 1;
+  for (int64_t z = 0; z < Z; ++z)
+    for (int64_t y = 0; y < Y; ++y)
+      for (int64_t x = 0; x < X; ++x) {
+        const int64_t ix = ((x - D) % nx + nx) % nx;
+        const int64_t iy = ((y - D) % ny + ny) % ny;
+        const int64_t iz = d.dim == 3 ? ((z - D) % nz + nz) % nz : 0;
+        std::memcpy(out + ((z * Y + y) * X + x) * es,
+                    in + ((iz * ny + iy) * nx + ix) * es, es);
+      }
+}
+
+int sps_fdm_upload(sps_fdm_plan* p, int field, const void* host, int layout) {
+  SPS_TRY(check_field(p, field, host, layout));
+  const size_t bytes = p->nsites * p->esize;
+  const void* src = host;
+  std::vector<char> tmp;
+  if (layout == SPS_LAYOUT_PADDED) {
+    tmp.resize(bytes);
+    padded_to_interior_m(p->d, (const char*)host, tmp.data(), p->esize);
+    src = tmp.data();
+  }
+  SPS_HIP_TRY(hipMemcpyAsync(p->buf[field][p->cur], src, bytes,
+                             hipMemcpyHostToDevice, p->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  p->has_data = true;
+  return SPS_OK;
+}
+
+int sps_fdm_download(sps_fdm_plan* p, int field, void* host, int layout) {
+  SPS_TRY(check_field(p, field, host, layout));
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  const size_t bytes = p->nsites * p->esize;
+  std::vector<char> tmp;
+  void* dst = host;
+  if (layout == SPS_LAYOUT_PADDED) {
+    tmp.resize(bytes);
+    dst = tmp.data();
+  }
+  SPS_HIP_TRY(hipMemcpyAsync(dst, p->buf[field][p->cur], bytes,
+                             hipMemcpyDeviceToHost, p->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  if (layout == SPS_LAYOUT_PADDED)
+    interior_to_padded_m(p->d, tmp.data(), (char*)host, p->esize);
+  return SPS_OK;
+}
+
+int sps_fdm_step(sps_fdm_plan* p, int nsteps) {
+  if (!p || nsteps < 0) {
+    set_error("plan is NULL or nsteps < 0");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  const int spl = p->steps_per_launch();
+  for (int done = 0; done < nsteps;) {
+    const int k = std::min(spl, nsteps - done);
+    SPS_TRY(p->launch_one(k));
+    done += k;
+  }
+  return SPS_OK;
+}
+
+int sps_fdm_query(const sps_fdm_plan* p, int* path, int* spl) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (path) *path = p->path;
+  if (spl) *spl = p->steps_per_launch();
+  return SPS_OK;
+}
+
+int sps_fdm_sync(sps_fdm_plan* p) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  return SPS_OK;
+}
+
+int sps_fdm_set_stream(sps_fdm_plan* p, void* hip_stream) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  p->stream = (hipStream_t)hip_stream;
+  return SPS_OK;
+}
+
+int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (steps_per_launch < 0 || steps_per_launch > 4 || rows_per_tile < 0 ||
+      rows_per_tile > 512 || (rows_per_tile != 0 && rows_per_tile < 4)) {
+    set_error("tune (coupled): steps_per_launch 0..4, rows_per_tile 0 or "
+              "4..512");
+    return SPS_E_INVALID;
+  }
+  if (steps_per_launch) p->K = steps_per_launch;
+  if (rows_per_tile) p->TY = rows_per_tile;
+  return SPS_OK;
+}
+
+int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  auto report = [&]() {
+    if (out_k) *out_k = p->steps_per_launch();
+    if (out_rows) *out_rows = p->TY;
+    return SPS_OK;
+  };
+  if (p->path != SPS_PATH_FUSED) return report();
+  if (!p->has_data) {
+    set_error("autotune needs uploaded fields");
+    return SPS_E_STATE;
+  }
+  const size_t bytes = p->nsites * p->esize;
+  const int nf = p->d.nfields;
+  std::vector<void*> saved(nf, nullptr);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = SPS_OK;
+  const int oldK = p->K, oldTY = p->TY, cur0 = p->cur;
+  int bestK = oldK, bestTY = oldTY;
+  double best = 1e300;
+  do {
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    for (int f = 0; f < nf && ok; ++f)
+      ok = hipMalloc(&saved[f], bytes) == hipSuccess &&
+           hipMemcpyAsync(saved[f], p->buf[f][p->cur], bytes,
+                          hipMemcpyDeviceToDevice, p->stream) == hipSuccess;
+    if (!ok) {
+      set_error("coupled autotune set-up failed");
+      rc = SPS_E_HIP;
+      break;
+    }
+    for (int k = 1; k <= 4 && rc == SPS_OK; ++k)
+      for (int rows : {16, 32, 64}) {
+        p->K = k;
+        p->TY = rows;
+        double us = 1e300;
+        for (int rep = 0; rep < 3 && rc == SPS_OK; ++rep) {
+          if ((rc = p->launch_one(k)) != SPS_OK) break;  // warm
+          hipEventRecord(e0, p->stream);
+          for (int i = 0; i < 12 / k && rc == SPS_OK; ++i) rc = p->launch_one(k);
+          hipEventRecord(e1, p->stream);
+          if (rc != SPS_OK || hipEventSynchronize(e1) != hipSuccess) {
+            if (rc == SPS_OK) rc = SPS_E_HIP;
+            break;
+          }
+          float ms = 0;
+          hipEventElapsedTime(&ms, e0, e1);
+          if (rep > 0) us = std::min(us, 1e3 * ms / ((12 / k) * k));
+        }
+        if (rc != SPS_OK) break;
+        if (us < best) {
+          best = us;
+          bestK = k;
+          bestTY = rows;
+        }
+      }
+    // restore the caller's fields
+    p->cur = cur0;
+    for (int f = 0; f < nf; ++f)
+      if (saved[f] && hipMemcpyAsync(p->buf[f][p->cur], saved[f], bytes,
+                                     hipMemcpyDeviceToDevice, p->stream) != hipSuccess &&
+          rc == SPS_OK)
+        rc = SPS_E_HIP;
+    if (hipStreamSynchronize(p->stream) != hipSuccess && rc == SPS_OK) rc = SPS_E_HIP;
+  } while (false);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  for (void* v : saved)
+    if (v) hipFree(v);
+  if (rc != SPS_OK) {
+    p->K = oldK;
+    p->TY = oldTY;
+    return rc;
+  }
+  p->K = bestK;
+  p->TY = bestTY;
+  return report();
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// multi.hpp - coupled-field (several order parameters) definitions shared by
+// the reference-order and fused kernels of sps_fdm_*.
+#pragma once
+
+#include "common.hpp"
+
+namespace sps {
+
+// One term coeff * D[prod_f psi_f^pow[f]] in device form.
+template <typename T>
+struct MDevTerm {
+  int op, axis, component;
+  int single;  // the field read directly when the monomial is one field, else -1
+  int pow[SPS_MAX_FIELDS];
+  T coeff;
+};
+
+// Reference-order parameters: all fields' term lists, passed by value.
+template <typename T>
+struct MRefParams {
+  int n[3];
+  int L, G, B;
+  int nfields;
+  int nterms[SPS_MAX_FIELDS];
+  T divh2, divh3, divh4, dt;
+  const T* in[SPS_MAX_FIELDS];
+  T* out[SPS_MAX_FIELDS];
+  MDevTerm<T> terms[SPS_MAX_FIELDS][SPS_MAX_MTERMS];
+};
+
+template <typename T>
+int launch_ref_multi(const MRefParams<T>& p, int dim, hipStream_t stream);
+
+// Fused two-field form, per field f (fields 0 and 1, 2D):
+//   dpsi_f/dt = A_f(psi0, psi1) + lap(B_f(psi0, psi1)) + kc_f * bilap(psi_f)
+// with A_f, B_f dense bivariate polynomials, a[f][i][j] multiplying
+// psi0^i psi1^j (degree <= 3 in each field), evaluated as
+//   radius 2:  mu_f = B_f + kmu_f * D(psi_f),  psi_f' = psi_f + Adt_f + kd * D(mu_f)
+//   radius 1:  psi_f' = psi_f + Adt_f + k1_f * D(psi_f)   (B_f = b_f * psi_f)
+constexpr int kMDeg = 3;
+template <typename T>
+struct MFusedCoef {
+  T adt[2][kMDeg + 1][kMDeg + 1];  // dt * A_f
+  T b[2][kMDeg + 1][kMDeg + 1];
+  T kmu[2], k1[2], kd;
+  int radius;
+};
+
+struct MLowered {
+  bool fusable = false;
+  const char* why_not = "";
+  double a[2][kMDeg + 1][kMDeg + 1] = {};
+  double b[2][kMDeg + 1][kMDeg + 1] = {};
+  double kc[2] = {0, 0};
+  int radius = 2;
+};
+
+template <typename T>
+int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
+                       int TY, const MFusedCoef<T>& c, hipStream_t s);
+
+}  // namespace sps

@@ -184,6 +184,124 @@ def state_speed(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.
     return make_desc(shape, stencil, h, dt, state_speed_terms(a, b), dtype, path)
 
 
+# ------------------------------------------------------- coupled fields
+
+@dataclass
+class MTermSpec:
+    """coeff * D[prod_f psi_f^pow[f]] in the equation of one field."""
+    op: int
+    coeff: float
+    pow: Sequence[int]
+    axis: int = 0
+    component: int = 0
+
+
+def make_mdesc(shape: Sequence[int], stencil, h: float, dt: float,
+               equations: Sequence[Sequence[MTermSpec]], dtype: int = abi.F64,
+               path: int = abi.PATH_AUTO) -> abi.MDesc:
+    """equations[f] lists the terms of d(field f)/dt."""
+    if len(shape) != stencil.dim:
+        raise ValueError(f"shape {tuple(shape)} does not match a {stencil.dim}D stencil")
+    if not 1 <= len(equations) <= abi.MAX_FIELDS:
+        raise ValueError(f"1..{abi.MAX_FIELDS} fields")
+    d = abi.MDesc()
+    d.dim = stencil.dim
+    d.dtype = dtype
+    for a in range(3):
+        d.n[a] = int(shape[a]) if a < len(shape) else 1
+    d.lap_points, d.gradlap_points, d.bilap_points = stencil.L, stencil.G, stencil.B
+    d.nfields = len(equations)
+    d.path = path
+    d.h, d.dt = float(h), float(dt)
+    for f, eq in enumerate(equations):
+        if len(eq) > abi.MAX_MTERMS:
+            raise ValueError(f"at most {abi.MAX_MTERMS} terms per field")
+        d.nterms[f] = len(eq)
+        for i, t in enumerate(eq):
+            tt = d.terms[f][i]
+            tt.op, tt.axis, tt.component, tt.coeff = t.op, t.axis, t.component, float(t.coeff)
+            for g, p in enumerate(t.pow):
+                tt.pow[g] = int(p)
+    return d
+
+
+def model_c_terms(c=(1.0, 1.0, 1.0, 1.0, 1.0)):
+    """MC (examples/models/modeldefinitions.h:74-80), fields (psi, rho):
+        dpsi = -bilap(psi) - lap((c1 - c2*psi*psi)*psi + c5*rho*rho)
+        drho = lap(rho) + (c3 - c4*rho*rho)*rho + 2*c5*psi*rho"""
+    c1, c2, c3, c4, c5 = c
+    psi = [
+        MTermSpec(abi.OP_BILAP, -1.0, (1, 0)),
+        MTermSpec(abi.OP_LAP, -c1, (1, 0)),
+        MTermSpec(abi.OP_LAP, c2, (3, 0)),
+        MTermSpec(abi.OP_LAP, -c5, (0, 2)),
+    ]
+    rho = [
+        MTermSpec(abi.OP_LAP, 1.0, (0, 1)),
+        MTermSpec(abi.OP_ID, c3, (0, 1)),
+        MTermSpec(abi.OP_ID, -c4, (0, 3)),
+        MTermSpec(abi.OP_ID, 2.0 * c5, (1, 1)),
+    ]
+    return [psi, rho]
+
+
+def gray_scott_terms(c=(0.16, 0.08, 0.035, 0.065)):
+    """GRAYSCOTT (examples/models/modeldefinitions.h:123-128), fields (psi, rho),
+    prr = psi*rho*rho:
+        dpsi = c1*lap(psi) - prr + c3*(1 - psi)
+        drho = c2*lap(rho) + prr - (c3 + c4)*rho"""
+    c1, c2, c3, c4 = c
+    psi = [
+        MTermSpec(abi.OP_LAP, c1, (1, 0)),
+        MTermSpec(abi.OP_ID, -1.0, (1, 2)),
+        MTermSpec(abi.OP_ID, c3, (0, 0)),
+        MTermSpec(abi.OP_ID, -c3, (1, 0)),
+    ]
+    rho = [
+        MTermSpec(abi.OP_LAP, c2, (0, 1)),
+        MTermSpec(abi.OP_ID, 1.0, (1, 2)),
+        MTermSpec(abi.OP_ID, -(c3 + c4), (0, 1)),
+    ]
+    return [psi, rho]
+
+
+def turing_terms(c=(1.0, 1.0, 1.0, 1.0, 1.0, 1.0)):
+    """Turing (examples/models/modeldefinitions.h:130-136), fields (psi, rho):
+        dpsi = c1*lap(psi) + c2*(psi + c3*rho - psi*rho*rho - c4*psi*rho)
+        drho = lap(rho) + c2*(c5*rho + c6*psi + psi*rho*rho + c4*psi*rho)"""
+    c1, c2, c3, c4, c5, c6 = c
+    psi = [
+        MTermSpec(abi.OP_LAP, c1, (1, 0)),
+        MTermSpec(abi.OP_ID, c2, (1, 0)),
+        MTermSpec(abi.OP_ID, c2 * c3, (0, 1)),
+        MTermSpec(abi.OP_ID, -c2, (1, 2)),
+        MTermSpec(abi.OP_ID, -c2 * c4, (1, 1)),
+    ]
+    rho = [
+        MTermSpec(abi.OP_LAP, 1.0, (0, 1)),
+        MTermSpec(abi.OP_ID, c2 * c5, (0, 1)),
+        MTermSpec(abi.OP_ID, c2 * c6, (1, 0)),
+        MTermSpec(abi.OP_ID, c2, (1, 2)),
+        MTermSpec(abi.OP_ID, c2 * c4, (1, 1)),
+    ]
+    return [psi, rho]
+
+
+def model_c(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+            path=abi.PATH_AUTO, c=(1.0, 1.0, 1.0, 1.0, 1.0)) -> abi.MDesc:
+    return make_mdesc(shape, stencil, h, dt, model_c_terms(c), dtype, path)
+
+
+def gray_scott(shape, h=1.0, dt=1.0, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+               path=abi.PATH_AUTO, c=(0.16, 0.08, 0.035, 0.065)) -> abi.MDesc:
+    return make_mdesc(shape, stencil, h, dt, gray_scott_terms(c), dtype, path)
+
+
+def turing(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+           path=abi.PATH_AUTO, c=(1.0, 1.0, 1.0, 1.0, 1.0, 1.0)) -> abi.MDesc:
+    return make_mdesc(shape, stencil, h, dt, turing_terms(c), dtype, path)
+
+
 def batched_instances(count: int = 256, seed: int = 7,
                       sides=(128, 192, 256, 384, 512), h: float = 1.0,
                       dt: float = 0.01, coeff_range=(0.5, 2.0), dtype=abi.F32,

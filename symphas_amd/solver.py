@@ -229,3 +229,70 @@ class FDBatch:
 
     def copy_from_device(self, index: int, dev_ptr: int):
         abi.check(self.lib.sps_fd_batch_copy_device(self._h, index, c_void_p(dev_ptr), 1))
+
+
+class MultiPlan:
+    """A plan of coupled fields (sps_fdm_*): field f is uploaded and
+    downloaded by index; step() advances all of them."""
+
+    def __init__(self, desc: abi.MDesc):
+        self.lib = abi.load()
+        self.desc = desc
+        h = c_void_p()
+        abi.check(self.lib.sps_fdm_create(byref(desc), byref(h)))
+        self._h = h
+        self.dtype = np.float32 if desc.dtype == abi.F32 else np.float64
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.sps_fdm_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, layout):
+        return interior_shape(self.desc) if layout == abi.LAYOUT_INTERIOR else padded_shape(self.desc)
+
+    def upload(self, field: int, values: np.ndarray, layout: int = abi.LAYOUT_INTERIOR):
+        a = np.ascontiguousarray(values, dtype=self.dtype)
+        if a.shape != self._shape(layout):
+            raise ValueError(f"field shape {a.shape}, expected {self._shape(layout)}")
+        abi.check(self.lib.sps_fdm_upload(self._h, int(field), a.ctypes.data_as(c_void_p), layout))
+
+    def download(self, field: int, layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        out = np.empty(self._shape(layout), dtype=self.dtype)
+        abi.check(self.lib.sps_fdm_download(self._h, int(field), out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def step(self, nsteps: int = 1):
+        abi.check(self.lib.sps_fdm_step(self._h, int(nsteps)))
+
+    def sync(self):
+        abi.check(self.lib.sps_fdm_sync(self._h))
+
+    def query(self):
+        path, spl = c_int(), c_int()
+        abi.check(self.lib.sps_fdm_query(self._h, byref(path), byref(spl)))
+        return path.value, spl.value
+
+    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0):
+        abi.check(self.lib.sps_fdm_tune(self._h, steps_per_launch, rows_per_tile))
+
+    def autotune(self):
+        """Time launch shapes on the device (fields kept); returns (K, rows)."""
+        k, r = c_int(), c_int()
+        abi.check(self.lib.sps_fdm_autotune(self._h, byref(k), byref(r)))
+        return k.value, r.value
+
+    def set_stream(self, stream_handle: int):
+        abi.check(self.lib.sps_fdm_set_stream(self._h, c_void_p(stream_handle)))

@@ -1,0 +1,159 @@
+"""Coupled-field models (Model C, Gray-Scott, Turing) on the GPU against the
+oracle: the reference-order path bit for bit in FP64, the fused two-field
+path within 1e-12 (FP64) and bitwise invariant across launch shapes."""
+import numpy as np
+import pytest
+
+from symphas_amd import abi, inputs, models
+from symphas_amd.models import Stencil2d2h, Stencil3d2h
+from symphas_amd.solver import MultiPlan
+
+pytestmark = pytest.mark.gpu
+
+F32, F64 = abi.F32, abi.F64
+REF, FUSED, AUTO = abi.PATH_REFERENCE, abi.PATH_FUSED, abi.PATH_AUTO
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def fields_for(d, seed, lo=-0.5, hi=0.5, offset=(0.0, 0.0)):
+    shape = tuple(int(d.n[a]) for a in reversed(range(d.dim)))
+    return [inputs.uniform(shape, seed=seed + f, lo=lo, hi=hi) + offset[f]
+            for f in range(d.nfields)]
+
+
+def gpu_run(d, fields, nsteps, K=0, TY=0):
+    with MultiPlan(d) as p:
+        if K or TY:
+            p.tune(K, TY)
+        for f, a in enumerate(fields):
+            p.upload(f, a.astype(p.dtype))
+        p.step(nsteps)
+        return [p.download(f) for f in range(d.nfields)]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+CASES = [
+    ("model_c", Stencil2d2h(5, 6, 13), (40, 28), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
+    ("model_c", Stencil2d2h(9, 6, 17), (21, 19), 0.002, (1.0, 1.0, 0.5, 0.9, 0.3)),
+    ("model_c", Stencil3d2h(7, 10, 25), (14, 11, 9), 0.002, (1.0, 1.0, 0.5, 0.9, 0.3)),
+    ("gray_scott", Stencil2d2h(5, 6, 13), (48, 36), 0.5, (0.16, 0.08, 0.035, 0.065)),
+    ("turing", Stencil2d2h(5, 6, 13), (32, 30), 0.01, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+    ("turing", Stencil3d2h(15, 10, 41), (9, 8, 7), 0.005, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+]
+
+
+def build(name, stencil, n, dt, c, dtype, path):
+    return getattr(models, name)(n, h=1.0, dt=dt, stencil=stencil, dtype=dtype, path=path, c=c)
+
+
+@pytest.mark.parametrize("name,stencil,n,dt,c", CASES)
+def test_reference_path_bitwise(gpu_lib, orc, name, stencil, n, dt, c):
+    say("coupled reference", name, stencil, n)
+    d = build(name, stencil, n, dt, c, F64, REF)
+    fields = fields_for(d, 71)
+    got = gpu_run(d, fields, 15)
+    want = orc.mrun(d, fields, 15, 8)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+@pytest.mark.parametrize("name,stencil,n,dt,c", [c for c in CASES if c[1].dim == 2 and c[1].L == 5])
+def test_fused_path(gpu_lib, orc, name, stencil, n, dt, c):
+    say("coupled fused", name, n)
+    d64 = build(name, stencil, n, dt, c, F64, AUTO)
+    with MultiPlan(d64) as p:
+        assert p.query()[0] == FUSED
+    fields = fields_for(d64, 72)
+    want = orc.mrun(d64, fields, 40, 8)
+    for g, w in zip(gpu_run(d64, fields, 40), want):
+        assert rel_l2(g, w) <= 1e-12
+    d32 = build(name, stencil, n, dt, c, F32, FUSED)
+    f32 = [a.astype(np.float32) for a in fields]
+    base = gpu_run(d32, f32, 24, K=1, TY=8)
+    for K in (2, 3, 4):
+        for TY in (4, 16, 64):
+            got = gpu_run(d32, f32, 24, K=K, TY=TY)
+            for g, b in zip(got, base):
+                assert np.array_equal(g, b), (K, TY)
+    want32 = orc.mrun(d64, [a.astype(np.float64) for a in f32], 24, 8)
+    for g, w in zip(base, want32):
+        assert rel_l2(g.astype(np.float64), w) <= 2e-6
+
+
+def test_gray_scott_pattern_full_size(gpu_lib, orc):
+    """2048^2 Gray-Scott seeded with a square of the second species: the
+    fused FP32 run stays within FP32 rounding of the FP64 reference path."""
+    n = (2048, 2048)
+    say("gray-scott 2048^2")
+    u = np.ones((2048, 2048))
+    v = np.zeros((2048, 2048))
+    u[960:1088, 960:1088] = 0.5
+    v[960:1088, 960:1088] = 0.25
+    d32 = models.gray_scott(n, dtype=F32, path=FUSED)
+    d64 = models.gray_scott(n, dtype=F64, path=REF)
+    got = gpu_run(d32, [u, v], 200)
+    want = gpu_run(d64, [u, v], 200)
+    for g, w in zip(got, want):
+        assert rel_l2(g.astype(np.float64), w) <= 1e-5
+
+
+def test_paths_and_layouts(gpu_lib):
+    d = models.model_c((12, 10), stencil=Stencil2d2h(5, 6, 13), dtype=F64, path=AUTO)
+    with MultiPlan(d) as p:
+        assert p.query()[0] == FUSED
+        a = inputs.uniform((10, 12), seed=73)
+        p.upload(1, np.pad(a, 3, mode="wrap"), abi.LAYOUT_PADDED)
+        p.upload(0, a)
+        assert np.array_equal(p.download(1), a)
+        assert np.array_equal(p.download(0, abi.LAYOUT_PADDED), np.pad(a, 3, mode="wrap"))
+    for d in (models.model_c((12, 10, 6), stencil=Stencil3d2h(7, 10, 25), path=AUTO),
+              models.turing((12, 10), stencil=Stencil2d2h(9, 6, 13), path=AUTO)):
+        with MultiPlan(d) as p:
+            assert p.query()[0] == REF
+    g = models.make_mdesc((12, 10), Stencil2d2h(5, 6, 13), 1.0, 0.01,
+                          [[models.MTermSpec(abi.OP_GRADLAP, 1.0, (1, 0), axis=0)],
+                           [models.MTermSpec(abi.OP_LAP, 1.0, (0, 1))]], F64, AUTO)
+    with MultiPlan(g) as p:
+        assert p.query()[0] == REF
+
+
+def test_invalid_descriptors(gpu_lib):
+    d = models.gray_scott((16, 16))
+    d.terms[0][1].pow[2] = 1
+    with pytest.raises(abi.SpsError) as ei:
+        MultiPlan(d)
+    assert ei.value.status == abi.E_INVALID
+    d = models.gray_scott((16, 16))
+    d.nfields = 0
+    with pytest.raises(abi.SpsError):
+        MultiPlan(d)
+    d = models.model_c((16, 16, 8), stencil=Stencil3d2h(7, 10, 25), path=FUSED)
+    with pytest.raises(abi.SpsError) as ei:
+        MultiPlan(d)
+    assert ei.value.status == abi.E_UNSUPPORTED
+
+
+def test_autotune_keeps_fields(gpu_lib):
+    """The coupled autotuner times launch shapes on copies of the fields: the
+    state after autotune + steps equals the state after the same steps alone."""
+    d = models.gray_scott((256, 192), dtype=F32, path=FUSED)
+    u = np.ones((192, 256), np.float32)
+    v = np.zeros((192, 256), np.float32)
+    u[80:112, 100:140] = 0.5
+    v[80:112, 100:140] = 0.25
+    say("coupled autotune 256x192")
+    want = gpu_run(d, [u, v], 12, K=1, TY=8)
+    with MultiPlan(d) as p:
+        p.upload(0, u)
+        p.upload(1, v)
+        k, rows = p.autotune()
+        assert 1 <= k <= 4 and rows in (16, 32, 64)
+        p.step(12)
+        for f in range(2):
+            assert np.array_equal(p.download(f), want[f])

@@ -492,3 +492,94 @@ def test_check_rejects_bad_provisionals(orc):
     d = models.model_bb((8, 8))
     d.prov_nterms[0] = 0
     assert orc.check(d) != 0
+
+
+# ----------------------------------------------------------- coupled fields
+MT = models.MTermSpec
+
+
+def test_model_c_without_rho_is_model_b_bitwise(orc):
+    from symphas_amd import inputs
+    n = (20, 14)
+    psi = inputs.uniform((14, 20), seed=61, lo=-0.6, hi=0.6)
+    mc = models.model_c(n, h=0.9, dt=0.003, c=(1.2, 0.7, 0.4, 0.9, 0.3))
+    mb = models.model_b(n, h=0.9, dt=0.003, c1=1.2, c2=0.7)
+    got = orc.mrun(mc, [psi, np.zeros_like(psi)], 9)
+    want = orc.unpad(mb, orc.run(mb, psi, 9))
+    assert np.array_equal(got[0], want)
+    assert np.array_equal(got[1], np.zeros_like(psi))
+
+
+def test_coupled_linear_mode_recursion(orc):
+    """dpsi = a lap psi + b rho, drho = c psi + e lap rho: a plane wave with
+    laplacian symbol lam follows (I + dt [[a lam, b], [c, e lam]])^n."""
+    nx, ny, h, dt = 16, 12, 0.5, 0.004
+    f, kx, ky = _plane_wave_2d(nx, ny, 2, 1, h)
+    lam = (2 * math.cos(kx) + 2 * math.cos(ky) - 4) / h ** 2
+    a, b, c, e = 0.8, -0.3, 0.5, 1.4
+    eqs = [[MT(LAP, a, (1, 0)), MT(abi.OP_ID, b, (0, 1))],
+           [MT(abi.OP_ID, c, (1, 0)), MT(LAP, e, (0, 1))]]
+    d = models.make_mdesc((nx, ny), Stencil2d2h(5, 6, 13), h, dt, eqs)
+    got = orc.mrun(d, [f, 0.5 * f], 30)
+    A = np.linalg.matrix_power(np.eye(2) + dt * np.array([[a * lam, b], [c, e * lam]]), 30)
+    want = A @ np.array([1.0, 0.5])
+    np.testing.assert_allclose(got[0], want[0] * f, atol=1e-12)
+    np.testing.assert_allclose(got[1], want[1] * f, atol=1e-12)
+
+
+def _rhs_roll(name, psi, rho, h, c, lapf):
+    if name == "model_c":
+        c1, c2, c3, c4, c5 = c
+        return (-lapf(lapf(psi, h), h) - lapf((c1 - c2 * psi * psi) * psi + c5 * rho * rho, h),
+                lapf(rho, h) + (c3 - c4 * rho * rho) * rho + 2 * c5 * psi * rho)
+    if name == "gray_scott":
+        c1, c2, c3, c4 = c
+        prr = psi * rho * rho
+        return (c1 * lapf(psi, h) - prr + c3 * (1 - psi),
+                c2 * lapf(rho, h) + prr - (c3 + c4) * rho)
+    c1, c2, c3, c4, c5, c6 = c
+    return (c1 * lapf(psi, h) + c2 * (psi + c3 * rho - psi * rho * rho - c4 * psi * rho),
+            lapf(rho, h) + c2 * (c5 * rho + c6 * psi + psi * rho * rho + c4 * psi * rho))
+
+
+@pytest.mark.parametrize("name,c", [("model_c", (1.1, 0.8, 0.6, 1.3, 0.4)),
+                                    ("gray_scott", (0.16, 0.08, 0.035, 0.065)),
+                                    ("turing", (1.0, 0.7, 1.2, 0.5, -0.4, 0.9))])
+def test_coupled_rhs_matches_numpy(orc, name, c):
+    from symphas_amd import inputs
+    n, h = (18, 13), 0.8
+    psi = inputs.uniform((13, 18), seed=62, lo=-0.9, hi=0.9)
+    rho = inputs.uniform((13, 18), seed=63, lo=-0.9, hi=0.9)
+    d = getattr(models, name)(n, h=h, dt=0.01, c=c)
+    got = orc.mequation(d, [psi, rho])
+    want = _rhs_roll(name, psi, rho, h, c, _lap5_roll)
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(g, w, rtol=0, atol=1e-12 * np.abs(w).max())
+
+
+def test_model_c_3d_rhs_matches_numpy(orc):
+    from symphas_amd import inputs
+    n, h, c = (6, 7, 5), 1.0, (1.0, 1.0, 0.5, 0.8, 0.3)
+    psi = inputs.uniform((5, 7, 6), seed=64, lo=-0.8, hi=0.8)
+    rho = inputs.uniform((5, 7, 6), seed=65, lo=-0.8, hi=0.8)
+    d = models.model_c(n, h=h, stencil=Stencil3d2h(7, 10, 25), c=c)
+    got = orc.mequation(d, [psi, rho])
+    want = _rhs_roll("model_c", psi, rho, h, c, _lap7_roll)
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(g, w, rtol=0, atol=1e-12 * np.abs(w).max())
+
+
+def test_gray_scott_homogeneous_state_is_fixed(orc):
+    d = models.gray_scott((8, 8))
+    got = orc.mrun(d, [np.ones((8, 8)), np.zeros((8, 8))], 25)
+    assert np.array_equal(got[0], np.ones((8, 8))) and np.array_equal(got[1], np.zeros((8, 8)))
+
+
+def test_mcheck_rejects_bad_descriptors(orc):
+    d = models.gray_scott((8, 8))
+    assert orc.mcheck(d) == 0
+    d.terms[0][1].pow[2] = 1  # a third field the model does not have
+    assert orc.mcheck(d) != 0
+    d = models.gray_scott((8, 8))
+    d.nfields = 5
+    assert orc.mcheck(d) != 0
