@@ -376,7 +376,7 @@ def extra_coupled(torch, stream, steps=1000):
             p.set_stream(stream.cuda_stream)
             for f, a in enumerate(fields):
                 p.upload(f, a)
-            k, rows = p.autotune()
+            k, rows, vec = p.autotune()
             p.step(8)
             a_, b_ = _events(torch)
             a_.record(stream); p.step(steps); b_.record(stream); stream.synchronize()
@@ -384,7 +384,8 @@ def extra_coupled(torch, stream, steps=1000):
             finite = all(bool(np.isfinite(p.download(f)).all()) for f in range(2))
         g = N * N * steps / (ms / 1e3) / 1e9
         out[name] = {"steps": steps, "Gsite_updates_per_s": g,
-                     "launch_shape": {"steps_per_launch": k, "rows_per_chunk": rows},
+                     "launch_shape": {"steps_per_launch": k, "rows_per_chunk": rows,
+                                      "values_per_lane": vec},
                      "roofline": {"achieved_GBs": 16 * g, "peak_GBs": HBM_PEAK_GBS,
                                   "frac": 16 * g / HBM_PEAK_GBS},
                      "finite": finite}

@@ -362,13 +362,16 @@ int sps_fdm_download(sps_fdm_plan* plan, int field, void* host, int layout);
 int sps_fdm_step(sps_fdm_plan* plan, int nsteps);
 int sps_fdm_query(const sps_fdm_plan* plan, int* path, int* steps_per_launch);
 /* Launch shape of the fused coupled kernel (0 keeps the current value):
- * steps_per_launch 1..4, rows_per_tile 4..512 (rows per streamed chunk). */
-int sps_fdm_tune(sps_fdm_plan* plan, int steps_per_launch, int rows_per_tile);
-/* Time steps_per_launch {1..4} x rows_per_tile {16,32,64} on the device
- * (fields saved and restored) and keep the fastest (no reference
- * counterpart, as sps_fd_autotune). */
+ * steps_per_launch 1..4, rows_per_tile 4..512 (rows per streamed chunk),
+ * values_per_lane 16/size or 8/size (16- or 8-byte lanes; must divide the x
+ * extent). */
+int sps_fdm_tune(sps_fdm_plan* plan, int steps_per_launch, int rows_per_tile,
+                 int values_per_lane);
+/* Time steps_per_launch {1..4} x rows_per_tile {16,32,64} x both lane widths
+ * on the device (fields saved and restored) and keep the fastest (no
+ * reference counterpart, as sps_fd_autotune). */
 int sps_fdm_autotune(sps_fdm_plan* plan, int* steps_per_launch,
-                     int* rows_per_tile);
+                     int* rows_per_tile, int* values_per_lane);
 int sps_fdm_sync(sps_fdm_plan* plan);
 /* Caller-owned hipStream_t for later work (NULL: the HIP null stream). */
 int sps_fdm_set_stream(sps_fdm_plan* plan, void* hip_stream);

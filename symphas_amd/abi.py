@@ -108,8 +108,8 @@ def load() -> ctypes.CDLL:
         "sps_fdm_step": (c_int, [P, c_int]),
         "sps_fdm_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
         "sps_fdm_sync": (c_int, [P]),
-        "sps_fdm_tune": (c_int, [P, c_int, c_int]),
-        "sps_fdm_autotune": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
+        "sps_fdm_tune": (c_int, [P, c_int, c_int, c_int]),
+        "sps_fdm_autotune": (c_int, [P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "sps_fdm_set_stream": (c_int, [P, P]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
         "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),

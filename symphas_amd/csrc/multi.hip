@@ -34,6 +34,37 @@ __device__ __forceinline__ T bipoly(const T (&c)[kMDeg + 1][kMDeg + 1], T x,
   return r;
 }
 
+// Total degree <= 3: sum_i x^i * sum_{j <= 3-i} c[i][j] y^j.
+template <typename T>
+__device__ __forceinline__ T tripoly(const T (&c)[kMDeg + 1][kMDeg + 1], T x,
+                                     T y) {
+  const T r2 = fmad(c[2][1], y, c[2][0]);
+  const T r1 = fmad(fmad(c[1][2], y, c[1][1]), y, c[1][0]);
+  const T r0 = fmad(fmad(fmad(c[0][3], y, c[0][2]), y, c[0][1]), y, c[0][0]);
+  return fmad(fmad(fmad(c[3][0], x, r2), x, r1), x, r0);
+}
+
+// The polynomial of class cls at every (x[j], y[j]), into p[j].
+template <typename T, int VEC>
+__device__ __forceinline__ void eval_poly(int cls,
+                                          const T (&c)[kMDeg + 1][kMDeg + 1],
+                                          const T* x, const T* y, T* p) {
+  if (cls == kPolyZero) {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) p[j] = T(0);
+  } else if (cls == kPolyLin) {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j)
+      p[j] = fmad(c[1][0], x[j], fmad(c[0][1], y[j], c[0][0]));
+  } else if (cls == kPolyTri) {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) p[j] = tripoly(c, x[j], y[j]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) p[j] = bipoly(c, x[j], y[j]);
+  }
+}
+
 template <typename T, int VEC, int R>
 struct Stage2 {
   T P[2][3][VEC];  // input rows of both fields (slots rotate with the phase)
@@ -69,33 +100,32 @@ struct Stage2 {
     for (int f = 0; f < 2; ++f)
 #pragma unroll
       for (int j = 0; j < VEC; ++j) P[f][S2][j] = v[f][j];
+    T d[2][VEC], a[2][VEC];
+    diff5<S1, S2, S0>(P[0], d[0]);
+    diff5<S1, S2, S0>(P[1], d[1]);
     if constexpr (R == 1) {
-      T d[2][VEC];
-      diff5<S1, S2, S0>(P[0], d[0]);
-      diff5<S1, S2, S0>(P[1], d[1]);
+      eval_poly<T, VEC>(c.adt_class[0], c.adt[0], P[0][S1], P[1][S1], a[0]);
+      eval_poly<T, VEC>(c.adt_class[1], c.adt[1], P[0][S1], P[1][S1], a[1]);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        const T x = P[0][S1][j], y = P[1][S1][j];
-        v[0][j] = fmad(c.k1[0], d[0][j], x + bipoly(c.adt[0], x, y));
-        v[1][j] = fmad(c.k1[1], d[1][j], y + bipoly(c.adt[1], x, y));
+        v[0][j] = fmad(c.k1[0], d[0][j], P[0][S1][j] + a[0][j]);
+        v[1][j] = fmad(c.k1[1], d[1][j], P[1][S1][j] + a[1][j]);
       }
     } else {
-      T d[2][VEC];
-      diff5<S1, S2, S0>(P[0], d[0]);
-      diff5<S1, S2, S0>(P[1], d[1]);
+      eval_poly<T, VEC>(c.b_class[0], c.b[0], P[0][S1], P[1][S1], a[0]);
+      eval_poly<T, VEC>(c.b_class[1], c.b[1], P[0][S1], P[1][S1], a[1]);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j) {
-        const T x = P[0][S1][j], y = P[1][S1][j];
-        M[0][S2][j] = fmad(c.kmu[0], d[0][j], bipoly(c.b[0], x, y));
-        M[1][S2][j] = fmad(c.kmu[1], d[1][j], bipoly(c.b[1], x, y));
-      }
+      for (int f = 0; f < 2; ++f)
+#pragma unroll
+        for (int j = 0; j < VEC; ++j) M[f][S2][j] = fmad(c.kmu[f], d[f][j], a[f][j]);
       diff5<S1, S2, S0>(M[0], d[0]);
       diff5<S1, S2, S0>(M[1], d[1]);
+      eval_poly<T, VEC>(c.adt_class[0], c.adt[0], P[0][S0], P[1][S0], a[0]);
+      eval_poly<T, VEC>(c.adt_class[1], c.adt[1], P[0][S0], P[1][S0], a[1]);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        const T x = P[0][S0][j], y = P[1][S0][j];
-        v[0][j] = fmad(c.kd, d[0][j], x + bipoly(c.adt[0], x, y));
-        v[1][j] = fmad(c.kd, d[1][j], y + bipoly(c.adt[1], x, y));
+        v[0][j] = fmad(c.kd, d[0][j], P[0][S0][j] + a[0][j]);
+        v[1][j] = fmad(c.kd, d[1][j], P[1][S0][j] + a[1][j]);
       }
     }
   }
@@ -177,10 +207,9 @@ __global__ __launch_bounds__(256) void fused_multi(
   }
 }
 
-template <typename T, int K, int R>
-int launch_multi_k(const T* const* in, T* const* out, int X, int Y, int TY,
+template <typename T, int VEC, int K, int R>
+int launch_multi_v(const T* const* in, T* const* out, int X, int Y, int TY,
                    const MFusedCoef<T>& c, hipStream_t s) {
-  constexpr int VEC = 8 / sizeof(T);
   constexpr int HL = (K * R + VEC - 1) / VEC;
   constexpr int W = (64 - 2 * HL) * VEC;
   const int nstrips = (X + W - 1) / W;
@@ -191,27 +220,37 @@ int launch_multi_k(const T* const* in, T* const* out, int X, int Y, int TY,
   return SPS_OK;
 }
 
+// 16-byte lanes (VEC = 16 / sizeof(T)) or 8-byte lanes (VEC = 8 / sizeof(T))
+template <typename T, int K, int R>
+int launch_multi_k(const T* const* in, T* const* out, int X, int Y, int TY,
+                   int vec, const MFusedCoef<T>& c, hipStream_t s) {
+  if (vec == 16 / (int)sizeof(T))
+    return launch_multi_v<T, 16 / sizeof(T), K, R>(in, out, X, Y, TY, c, s);
+  return launch_multi_v<T, 8 / sizeof(T), K, R>(in, out, X, Y, TY, c, s);
+}
+
 }  // namespace
 
 template <typename T>
 int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
-                       int TY, const MFusedCoef<T>& c, hipStream_t s) {
-  if (X % (8 / (int)sizeof(T)) != 0 || TY < 1) {
-    set_error("fused coupled step: X=%d must be a multiple of %d", X,
-              8 / (int)sizeof(T));
+                       int TY, int vec, const MFusedCoef<T>& c, hipStream_t s) {
+  if ((vec != 8 / (int)sizeof(T) && vec != 16 / (int)sizeof(T)) ||
+      X % vec != 0 || TY < 1) {
+    set_error("fused coupled step: X=%d must be a multiple of %d values per "
+              "lane (8 or 16 bytes)", X, vec);
     return SPS_E_UNSUPPORTED;
   }
   const bool r1 = c.radius == 1;
   int rc;
   switch (K) {
-    case 1: rc = r1 ? launch_multi_k<T, 1, 1>(in, out, X, Y, TY, c, s)
-                    : launch_multi_k<T, 1, 2>(in, out, X, Y, TY, c, s); break;
-    case 2: rc = r1 ? launch_multi_k<T, 2, 1>(in, out, X, Y, TY, c, s)
-                    : launch_multi_k<T, 2, 2>(in, out, X, Y, TY, c, s); break;
-    case 3: rc = r1 ? launch_multi_k<T, 3, 1>(in, out, X, Y, TY, c, s)
-                    : launch_multi_k<T, 3, 2>(in, out, X, Y, TY, c, s); break;
-    case 4: rc = r1 ? launch_multi_k<T, 4, 1>(in, out, X, Y, TY, c, s)
-                    : launch_multi_k<T, 4, 2>(in, out, X, Y, TY, c, s); break;
+    case 1: rc = r1 ? launch_multi_k<T, 1, 1>(in, out, X, Y, TY, vec, c, s)
+                    : launch_multi_k<T, 1, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 2: rc = r1 ? launch_multi_k<T, 2, 1>(in, out, X, Y, TY, vec, c, s)
+                    : launch_multi_k<T, 2, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 3: rc = r1 ? launch_multi_k<T, 3, 1>(in, out, X, Y, TY, vec, c, s)
+                    : launch_multi_k<T, 3, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 4: rc = r1 ? launch_multi_k<T, 4, 1>(in, out, X, Y, TY, vec, c, s)
+                    : launch_multi_k<T, 4, 2>(in, out, X, Y, TY, vec, c, s); break;
     default:
       set_error("fused coupled step: steps per launch %d not in 1..4", K);
       return SPS_E_INVALID;
@@ -226,10 +265,10 @@ int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
 }
 
 template int launch_fused_multi<float>(const float* const*, float* const*, int,
-                                       int, int, int, const MFusedCoef<float>&,
+                                       int, int, int, int, const MFusedCoef<float>&,
                                        hipStream_t);
 template int launch_fused_multi<double>(const double* const*, double* const*,
-                                        int, int, int, int,
+                                        int, int, int, int, int,
                                         const MFusedCoef<double>&, hipStream_t);
 
 }  // namespace sps
@@ -245,7 +284,7 @@ struct sps_fdm_plan {
   void* buf[SPS_MAX_FIELDS][2] = {};
   int cur = 0;
   bool has_data = false;
-  int K = 1, TY = 32;
+  int K = 1, TY = 32, VEC = 1;
   sps::MRefParams<float> rp32;
   sps::MRefParams<double> rp64;
   sps::MFusedCoef<float> fc32;
@@ -379,6 +418,18 @@ MLowered lower_multi(const sps_fdm_desc& d) {
   return lw;
 }
 
+// The smallest class whose evaluation covers every nonzero coefficient.
+int poly_class(const double (&c)[kMDeg + 1][kMDeg + 1]) {
+  int maxdeg = -1;
+  for (int i = 0; i <= kMDeg; ++i)
+    for (int j = 0; j <= kMDeg; ++j)
+      if (c[i][j] != 0.0) maxdeg = std::max(maxdeg, i + j);
+  if (maxdeg < 0) return kPolyZero;
+  if (maxdeg <= 1) return kPolyLin;
+  if (maxdeg <= 3) return kPolyTri;
+  return kPolyDense;
+}
+
 template <typename T>
 void make_mfused_coef(const sps_fdm_desc& d, const MLowered& lw,
                       MFusedCoef<T>& c) {
@@ -396,6 +447,10 @@ void make_mfused_coef(const sps_fdm_desc& d, const MLowered& lw,
   }
   c.kd = T(d.dt * divh2);
   c.radius = lw.radius;
+  for (int f = 0; f < 2; ++f) {
+    c.adt_class[f] = poly_class(lw.a[f]);
+    c.b_class[f] = poly_class(lw.b[f]);
+  }
 }
 
 template <typename T>
@@ -463,12 +518,13 @@ int sps_fdm_plan::launch_one(int k) {
   } else if (d.dtype == SPS_F32) {
     const float* in[2] = {(const float*)buf[0][cur], (const float*)buf[1][cur]};
     float* out[2] = {(float*)buf[0][1 - cur], (float*)buf[1][1 - cur]};
-    rc = launch_fused_multi<float>(in, out, d.n[0], d.n[1], k, TY, fc32, stream);
+    rc = launch_fused_multi<float>(in, out, d.n[0], d.n[1], k, TY, VEC, fc32,
+                                   stream);
   } else {
     const double* in[2] = {(const double*)buf[0][cur],
                            (const double*)buf[1][cur]};
     double* out[2] = {(double*)buf[0][1 - cur], (double*)buf[1][1 - cur]};
-    rc = launch_fused_multi<double>(in, out, d.n[0], d.n[1], k, TY, fc64,
+    rc = launch_fused_multi<double>(in, out, d.n[0], d.n[1], k, TY, VEC, fc64,
                                     stream);
   }
   if (rc == SPS_OK) cur = 1 - cur;
@@ -517,6 +573,9 @@ int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out) {
     make_mfused_coef(*desc, p->low, p->fc64);
     p->K = p->low.radius == 1 ? 4 : 2;
     p->TY = 32;
+    // 8-byte lanes: the 16-byte variant holds twice the rows in registers
+    // and measured slower at every launch shape on 4096^2 FP32
+    p->VEC = 8 / (int)p->esize;
   }
   hipError_t e = hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -686,7 +745,8 @@ int sps_fdm_set_stream(sps_fdm_plan* p, void* hip_stream) {
   return SPS_OK;
 }
 
-int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile) {
+int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile,
+                 int values_per_lane) {
   if (!p) {
     set_error("plan is NULL");
     return SPS_E_INVALID;
@@ -697,12 +757,23 @@ int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile) {
               "4..512");
     return SPS_E_INVALID;
   }
+  if (values_per_lane != 0) {
+    const int narrow = 8 / (int)p->esize, wide = 16 / (int)p->esize;
+    if ((values_per_lane != narrow && values_per_lane != wide) ||
+        p->d.n[0] % values_per_lane != 0) {
+      set_error("tune (coupled): values_per_lane must be %d or %d and divide "
+                "the x extent %d", narrow, wide, p->d.n[0]);
+      return SPS_E_INVALID;
+    }
+  }
   if (steps_per_launch) p->K = steps_per_launch;
   if (rows_per_tile) p->TY = rows_per_tile;
+  if (values_per_lane) p->VEC = values_per_lane;
   return SPS_OK;
 }
 
-int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
+int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
+                     int* out_vec) {
   if (!p) {
     set_error("plan is NULL");
     return SPS_E_INVALID;
@@ -710,6 +781,7 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
   auto report = [&]() {
     if (out_k) *out_k = p->steps_per_launch();
     if (out_rows) *out_rows = p->TY;
+    if (out_vec) *out_vec = p->VEC;
     return SPS_OK;
   };
   if (p->path != SPS_PATH_FUSED) return report();
@@ -722,8 +794,11 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
   std::vector<void*> saved(nf, nullptr);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = SPS_OK;
-  const int oldK = p->K, oldTY = p->TY, cur0 = p->cur;
-  int bestK = oldK, bestTY = oldTY;
+  const int oldK = p->K, oldTY = p->TY, oldVEC = p->VEC, cur0 = p->cur;
+  int bestK = oldK, bestTY = oldTY, bestVEC = oldVEC;
+  std::vector<int> vecs;
+  for (int v : {16 / (int)p->esize, 8 / (int)p->esize})
+    if (p->d.n[0] % v == 0) vecs.push_back(v);
   double best = 1e300;
   do {
     bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
@@ -736,10 +811,12 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
       rc = SPS_E_HIP;
       break;
     }
+    for (int vec : vecs)
     for (int k = 1; k <= 4 && rc == SPS_OK; ++k)
       for (int rows : {16, 32, 64}) {
         p->K = k;
         p->TY = rows;
+        p->VEC = vec;
         double us = 1e300;
         for (int rep = 0; rep < 3 && rc == SPS_OK; ++rep) {
           if ((rc = p->launch_one(k)) != SPS_OK) break;  // warm
@@ -759,6 +836,7 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
           best = us;
           bestK = k;
           bestTY = rows;
+          bestVEC = vec;
         }
       }
     // restore the caller's fields
@@ -777,10 +855,12 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows) {
   if (rc != SPS_OK) {
     p->K = oldK;
     p->TY = oldTY;
+    p->VEC = oldVEC;
     return rc;
   }
   p->K = bestK;
   p->TY = bestTY;
+  p->VEC = bestVEC;
   return report();
 }
 

@@ -37,13 +37,22 @@ int launch_ref_multi(const MRefParams<T>& p, int dim, hipStream_t stream);
 // psi0^i psi1^j (degree <= 3 in each field), evaluated as
 //   radius 2:  mu_f = B_f + kmu_f * D(psi_f),  psi_f' = psi_f + Adt_f + kd * D(mu_f)
 //   radius 1:  psi_f' = psi_f + Adt_f + k1_f * D(psi_f)   (B_f = b_f * psi_f)
+// Each of the four polynomials carries a shape class chosen on the host from
+// its nonzero coefficients, so the kernel evaluates only what is there:
+//   kPolyZero  identically 0                                   (0 FMAs)
+//   kPolyLin   c00 + c10 x + c01 y                             (2 FMAs)
+//   kPolyTri   total degree <= 3, triangular two-level Horner  (9 FMAs)
+//   kPolyDense every c[i][j], i, j <= 3                        (15 FMAs)
+// The class is a kernel argument, so the branch on it is uniform.
 constexpr int kMDeg = 3;
+enum { kPolyZero = 0, kPolyLin = 1, kPolyTri = 2, kPolyDense = 3 };
 template <typename T>
 struct MFusedCoef {
   T adt[2][kMDeg + 1][kMDeg + 1];  // dt * A_f
   T b[2][kMDeg + 1][kMDeg + 1];
   T kmu[2], k1[2], kd;
   int radius;
+  int adt_class[2], b_class[2];
 };
 
 struct MLowered {
@@ -57,6 +66,6 @@ struct MLowered {
 
 template <typename T>
 int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
-                       int TY, const MFusedCoef<T>& c, hipStream_t s);
+                       int TY, int vec, const MFusedCoef<T>& c, hipStream_t s);
 
 }  // namespace sps

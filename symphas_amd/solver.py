@@ -285,14 +285,17 @@ class MultiPlan:
         abi.check(self.lib.sps_fdm_query(self._h, byref(path), byref(spl)))
         return path.value, spl.value
 
-    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0):
-        abi.check(self.lib.sps_fdm_tune(self._h, steps_per_launch, rows_per_tile))
+    def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0,
+             values_per_lane: int = 0):
+        abi.check(self.lib.sps_fdm_tune(self._h, steps_per_launch, rows_per_tile,
+                                        values_per_lane))
 
     def autotune(self):
-        """Time launch shapes on the device (fields kept); returns (K, rows)."""
-        k, r = c_int(), c_int()
-        abi.check(self.lib.sps_fdm_autotune(self._h, byref(k), byref(r)))
-        return k.value, r.value
+        """Time launch shapes on the device (fields kept); returns
+        (steps_per_launch, rows_per_tile, values_per_lane)."""
+        k, r, v = c_int(), c_int(), c_int()
+        abi.check(self.lib.sps_fdm_autotune(self._h, byref(k), byref(r), byref(v)))
+        return k.value, r.value, v.value
 
     def set_stream(self, stream_handle: int):
         abi.check(self.lib.sps_fdm_set_stream(self._h, c_void_p(stream_handle)))

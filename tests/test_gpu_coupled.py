@@ -24,10 +24,10 @@ def fields_for(d, seed, lo=-0.5, hi=0.5, offset=(0.0, 0.0)):
             for f in range(d.nfields)]
 
 
-def gpu_run(d, fields, nsteps, K=0, TY=0):
+def gpu_run(d, fields, nsteps, K=0, TY=0, vec=0):
     with MultiPlan(d) as p:
-        if K or TY:
-            p.tune(K, TY)
+        if K or TY or vec:
+            p.tune(K, TY, vec)
         for f, a in enumerate(fields):
             p.upload(f, a.astype(p.dtype))
         p.step(nsteps)
@@ -75,12 +75,13 @@ def test_fused_path(gpu_lib, orc, name, stencil, n, dt, c):
         assert rel_l2(g, w) <= 1e-12
     d32 = build(name, stencil, n, dt, c, F32, FUSED)
     f32 = [a.astype(np.float32) for a in fields]
-    base = gpu_run(d32, f32, 24, K=1, TY=8)
-    for K in (2, 3, 4):
-        for TY in (4, 16, 64):
-            got = gpu_run(d32, f32, 24, K=K, TY=TY)
-            for g, b in zip(got, base):
-                assert np.array_equal(g, b), (K, TY)
+    base = gpu_run(d32, f32, 24, K=1, TY=8, vec=2)
+    for vec in (2, 4):
+        for K in (2, 3, 4):
+            for TY in (4, 16, 64):
+                got = gpu_run(d32, f32, 24, K=K, TY=TY, vec=vec)
+                for g, b in zip(got, base):
+                    assert np.array_equal(g, b), (K, TY, vec)
     want32 = orc.mrun(d64, [a.astype(np.float64) for a in f32], 24, 8)
     for g, w in zip(base, want32):
         assert rel_l2(g.astype(np.float64), w) <= 2e-6
@@ -152,8 +153,8 @@ def test_autotune_keeps_fields(gpu_lib):
     with MultiPlan(d) as p:
         p.upload(0, u)
         p.upload(1, v)
-        k, rows = p.autotune()
-        assert 1 <= k <= 4 and rows in (16, 32, 64)
+        k, rows, vec = p.autotune()
+        assert 1 <= k <= 4 and rows in (16, 32, 64) and vec in (2, 4)
         p.step(12)
         for f in range(2):
             assert np.array_equal(p.download(f), want[f])
