@@ -350,6 +350,51 @@ def extra_checkpoint(torch, stream):
         out["write_speedup_vs_cpu"] = t_cpu / t_write
     except Exception as exc:  # reported, never hidden
         out["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
+    out["plot_and_xdr"] = extra_plot_files(psi, d32)
+    return out
+
+
+def extra_plot_files(psi, d32):
+    """The reference's other data files of the same field, written on the GPU:
+    GNU matrix and CSV plot text ("% 10lE"), XDR (big-endian doubles; also
+    read back), with the oracle's byte-at-a-time restatements timed beside
+    them."""
+    import numpy as np
+    from symphas_amd import abi, models
+    from symphas_amd.solver import FDPlan
+    out = {}
+    with FDPlan(d32) as p:
+        p.upload(psi)
+        for name, fn in (("gnu_plot", lambda: p.plot_text(abi.PLOT_GNU)),
+                         ("csv", lambda: p.plot_text(abi.PLOT_CSV)),
+                         ("xdr", lambda: p.checkpoint_xdr(index=10000))):
+            fn()  # warm
+            t0 = time.perf_counter()
+            data = fn()
+            t = time.perf_counter() - t0
+            out[name] = {"bytes": len(data), "write_s": t,
+                         "write_Mvalues_per_s": N * N / t / 1e6}
+            if name == "xdr":
+                p.read_checkpoint_xdr(data)
+                t0 = time.perf_counter()
+                p.read_checkpoint_xdr(data)
+                out[name]["read_s"] = time.perf_counter() - t0
+                out[name]["round_trip_exact"] = bool(np.array_equal(p.download(), psi))
+            try:
+                from oracle import pyoracle as orc
+                d64 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F64,
+                                     path=abi.PATH_REFERENCE)
+                g = orc.pad(d64, psi.astype(np.float64))
+                t0 = time.perf_counter()
+                ref = (orc.xdr_write(d64, g, 10000) if name == "xdr" else
+                       orc.plot_text(d64, g, abi.PLOT_GNU if name == "gnu_plot" else abi.PLOT_CSV))
+                t_cpu = time.perf_counter() - t0
+                out[name]["cpu_baseline"] = {"write_s": t_cpu, "cores": 1, "kind": "port",
+                                             "byte_identical": ref == data}
+                out[name]["write_speedup_vs_cpu"] = t_cpu / t
+            except Exception as exc:  # reported, never hidden
+                out[name]["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
+            del data
     return out
 
 

@@ -244,6 +244,33 @@ int sps_fd_checkpoint_text(sps_fd_plan* plan, int index, const double* origin,
 int sps_fd_checkpoint_read(sps_fd_plan* plan, const char* text, size_t length,
                            int* index);
 
+/* The reference's plot data file of the current field, formatted on the
+ * device: SPS_PLOT_GNU is the GNU matrix of gp::save_grid_plotting
+ * (io/src/writegp.cpp:470-507: per z slice a line of "% 13d " index and
+ * "% 13.2f " x positions, then rows of "% 13.2f " y position and "% 10lE "
+ * values, slices separated by a blank line, an empty last line);
+ * SPS_PLOT_CSV is csv::save_grid_plotting (io/src/writecsv.cpp: "% 10lE"
+ * values joined by ',' per x row).  Positions are origin + i*h (origin NULL:
+ * zeros).  Byte-identical to the C library's output.  text NULL: only the
+ * length. */
+#define SPS_PLOT_GNU 1
+#define SPS_PLOT_CSV 2
+int sps_fd_plot_text(sps_fd_plan* plan, int format, const double* origin,
+                     char* text, size_t capacity, size_t* length);
+
+/* The reference's XDR file of the current field (io/src/writexdr.cpp
+ * print_xdr_header :45-87 + save_xdr :92-98, the XDR writer's checkpoint and
+ * data file): big-endian int dim, counts, domain and interval bounds, the
+ * index, then every value as a big-endian IEEE double in x-fastest order,
+ * converted on the device.  data NULL: only the length. */
+int sps_fd_checkpoint_xdr(sps_fd_plan* plan, int index, const double* origin,
+                          void* data, size_t capacity, size_t* length);
+/* Read such a file into the plan's field (io/src/readxdr.cpp read_header,
+ * read_block); dimension and counts must match the plan.  FP32 plans round
+ * each double to nearest. */
+int sps_fd_checkpoint_read_xdr(sps_fd_plan* plan, const void* data,
+                               size_t length, int* index);
+
 /* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 

@@ -58,6 +58,9 @@ def load():
         ("orcm_equation", c_int, [P, P, P, c_int]),
         ("orcm_run", c_int, [P, P, c_int, c_int]),
         ("orc_checkpoint_read", c_int, [P, ctypes.c_char_p, P, POINTER(c_int)]),
+        ("orc_plot_text", c_int64, [P, P, c_int, P, P, c_size_t]),
+        ("orc_xdr_write", c_int64, [P, P, c_int, P, P, c_size_t]),
+        ("orc_xdr_read", c_int, [P, ctypes.c_char_p, c_size_t, P, POINTER(c_int)]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res
@@ -170,6 +173,41 @@ def checkpoint_read(desc, text):
     rc = load().orc_checkpoint_read(_d(desc), text, _p(g), ctypes.byref(idx))
     if rc:
         raise RuntimeError(f"orc_checkpoint_read failed ({rc})")
+    return idx.value, g
+
+
+def plot_text(desc, padded, fmt, origin=None):
+    """The reference's plot data text (fmt 1: GNU matrix, 2: CSV) of the
+    interior (bytes)."""
+    o = None if origin is None else _p(np.asarray(origin, dtype=np.float64))
+    n = load().orc_plot_text(_d(desc), _p(padded), int(fmt), o, None, 0)
+    if n < 0:
+        raise RuntimeError("orc_plot_text failed")
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    load().orc_plot_text(_d(desc), _p(padded), int(fmt), o, ctypes.cast(buf, c_void_p),
+                         int(n) + 1)
+    return buf.raw[:int(n)]
+
+
+def xdr_write(desc, padded, index=0, origin=None):
+    """The reference's XDR file of the interior (bytes)."""
+    o = None if origin is None else _p(np.asarray(origin, dtype=np.float64))
+    n = load().orc_xdr_write(_d(desc), _p(padded), int(index), o, None, 0)
+    if n < 0:
+        raise RuntimeError("orc_xdr_write failed")
+    buf = ctypes.create_string_buffer(int(n))
+    load().orc_xdr_write(_d(desc), _p(padded), int(index), o, ctypes.cast(buf, c_void_p),
+                         int(n))
+    return buf.raw[:int(n)]
+
+
+def xdr_read(desc, data):
+    """(index, padded field) read the way the reference reads an XDR file."""
+    g = np.zeros(padded_shape(desc))
+    idx = c_int(0)
+    rc = load().orc_xdr_read(_d(desc), data, len(data), _p(g), ctypes.byref(idx))
+    if rc:
+        raise RuntimeError(f"orc_xdr_read failed ({rc})")
     return idx.value, g
 
 

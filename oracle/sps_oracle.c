@@ -848,6 +848,138 @@ int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
   return 0;
 }
 
+int64_t orc_plot_text(const orc_desc* d, const double* psi, int format,
+                      const double* origin, char* out, size_t cap) {
+  if (orc_check(d) != 0 || (format != 1 && format != 2)) return -1;
+  size_t n = 0;
+  char tmp[128];
+#define EMIT(...)                                                     \
+  do {                                                                \
+    int k_ = snprintf(tmp, sizeof(tmp), __VA_ARGS__);                 \
+    for (int q_ = 0; q_ < k_; ++q_, ++n)                              \
+      if (n < cap) out[n] = tmp[q_];                                  \
+  } while (0)
+  const int64_t X = pdim(d, 0), Y = pdim(d, 1);
+  const int nz = d->dim == 3 ? d->n[2] : 1;
+  const double x0 = origin ? origin[0] : 0.0, y0 = origin ? origin[1] : 0.0;
+  for (int k = 0; k < nz; ++k) {
+    if (format == 1) {
+      EMIT("% 13d ", k);
+      for (int i = 0; i < d->n[0]; ++i) EMIT("% 13.2f ", x0 + i * d->h);
+      EMIT("\n");
+    }
+    for (int j = 0; j < d->n[1]; ++j) {
+      const int64_t row =
+          X * ((j + ORC_DEPTH) + Y * (d->dim == 3 ? k + ORC_DEPTH : 0));
+      if (format == 1) EMIT("% 13.2f ", y0 + j * d->h);
+      for (int i = 0; i < d->n[0]; ++i) {
+        if (format == 1) {
+          EMIT("% 10lE ", psi[row + i + ORC_DEPTH]);
+        } else {
+          EMIT("% 10lE", psi[row + i + ORC_DEPTH]);
+          if (i < d->n[0] - 1) EMIT(",");
+        }
+      }
+      EMIT("\n");
+    }
+    if (format == 1 && k < nz - 1) EMIT("\n");
+  }
+  if (format == 1) EMIT("\n");
+#undef EMIT
+  if (n < cap) out[n] = 0;
+  return (int64_t)n;
+}
+
+static void xdr_put_int(unsigned char* out, size_t cap, size_t* n, int v) {
+  const uint32_t u = (uint32_t)v;
+  for (int b = 3; b >= 0; --b, ++*n)
+    if (*n < cap) out[*n] = (unsigned char)(u >> (8 * b));
+}
+
+static void xdr_put_double(unsigned char* out, size_t cap, size_t* n,
+                           double v) {
+  uint64_t u;
+  memcpy(&u, &v, 8);
+  for (int b = 7; b >= 0; --b, ++*n)
+    if (*n < cap) out[*n] = (unsigned char)(u >> (8 * b));
+}
+
+int64_t orc_xdr_write(const orc_desc* d, const double* psi, int index,
+                      const double* origin, unsigned char* out, size_t cap) {
+  if (orc_check(d) != 0) return -1;
+  size_t n = 0;
+  xdr_put_int(out, cap, &n, d->dim);
+  for (int a = 0; a < d->dim; ++a) xdr_put_int(out, cap, &n, d->n[a]);
+  for (int a = 0; a < d->dim; ++a) {
+    const double left = origin ? origin[a] : 0.0;
+    xdr_put_double(out, cap, &n, left);
+    xdr_put_double(out, cap, &n, left + (d->n[a] - 1) * d->h);
+  }
+  xdr_put_int(out, cap, &n, index);
+  xdr_put_int(out, cap, &n, d->dim);
+  for (int a = 0; a < d->dim; ++a) {
+    const double left = origin ? origin[a] : 0.0;
+    xdr_put_double(out, cap, &n, left);
+    xdr_put_double(out, cap, &n, left + (d->n[a] - 1) * d->h);
+  }
+  const int64_t X = pdim(d, 0), Y = pdim(d, 1);
+  const int nz = d->dim == 3 ? d->n[2] : 1;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < d->n[1]; ++j) {
+      const int64_t row =
+          X * ((j + ORC_DEPTH) + Y * (d->dim == 3 ? k + ORC_DEPTH : 0));
+      for (int i = 0; i < d->n[0]; ++i)
+        xdr_put_double(out, cap, &n, psi[row + i + ORC_DEPTH]);
+    }
+  return (int64_t)n;
+}
+
+static int xdr_get_int(const unsigned char* in, size_t len, size_t* p,
+                       int* v) {
+  if (*p + 4 > len) return -1;
+  uint32_t u = 0;
+  for (int b = 0; b < 4; ++b) u = (u << 8) | in[(*p)++];
+  *v = (int)u;
+  return 0;
+}
+
+static int xdr_get_double(const unsigned char* in, size_t len, size_t* p,
+                          double* v) {
+  if (*p + 8 > len) return -1;
+  uint64_t u = 0;
+  for (int b = 0; b < 8; ++b) u = (u << 8) | in[(*p)++];
+  memcpy(v, &u, 8);
+  return 0;
+}
+
+int orc_xdr_read(const orc_desc* d, const unsigned char* in, size_t len,
+                 double* psi, int* index) {
+  if (orc_check(d) != 0) return -1;
+  size_t p = 0;
+  int dim = 0, c = 0, idx = 0, dim2 = 0;
+  double lr;
+  if (xdr_get_int(in, len, &p, &dim) || dim != d->dim) return -2;
+  for (int a = 0; a < dim; ++a)
+    if (xdr_get_int(in, len, &p, &c) || c != d->n[a]) return -3;
+  for (int a = 0; a < 2 * dim; ++a)
+    if (xdr_get_double(in, len, &p, &lr)) return -4;
+  if (xdr_get_int(in, len, &p, &idx)) return -5;
+  if (xdr_get_int(in, len, &p, &dim2) || dim2 != dim) return -6;
+  for (int a = 0; a < 2 * dim; ++a)
+    if (xdr_get_double(in, len, &p, &lr)) return -4;
+  const int64_t X = pdim(d, 0), Y = pdim(d, 1);
+  const int nz = d->dim == 3 ? d->n[2] : 1;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < d->n[1]; ++j) {
+      const int64_t row =
+          X * ((j + ORC_DEPTH) + Y * (d->dim == 3 ? k + ORC_DEPTH : 0));
+      for (int i = 0; i < d->n[0]; ++i)
+        if (xdr_get_double(in, len, &p, &psi[row + i + ORC_DEPTH])) return -7;
+    }
+  if (index) *index = idx;
+  return 0;
+}
+
 /* ------------------------------------------------------------- coupled fields */
 
 size_t orcm_desc_size(void) { return sizeof(orcm_desc); }

@@ -144,6 +144,32 @@ int64_t orc_checkpoint_text(const orc_desc* d, const double* psi, int index,
 int orc_checkpoint_read(const orc_desc* d, const char* text, double* psi,
                         int* index);
 
+/* The reference's plot data files of a real field.
+ * format 1, GNU matrix (io/src/writegp.cpp save_grid_plotting :470-507): per
+ *   z slice k a line "% 13d " k, then "% 13.2f " per x position, "\n"; then
+ *   per y row "% 13.2f " y position and "% 10lE " per value, "\n"; a blank
+ *   line between slices and a final "\n".  Positions are origin + i*h.
+ * format 2, CSV (io/src/writecsv.cpp save_grid_plotting): every x row as
+ *   "% 10lE" values joined by ',' and ended by "\n", rows in y then z order.
+ * Returns the full length (at most cap bytes written), or -1. */
+int64_t orc_plot_text(const orc_desc* d, const double* psi, int format,
+                      const double* origin, char* out, size_t cap);
+
+/* The reference's XDR file of a field (io/src/writexdr.cpp print_xdr_header
+ * :45-87, save_xdr :92-98; xdrfile's XDR encoding, RFC 4506: 4-byte
+ * big-endian ints, 8-byte big-endian IEEE doubles): int dim, int counts[dim],
+ * per axis double (left, right) of the domain, int index, int dim, per axis
+ * double (left, right) of the interval, then every interior value as a
+ * double in x-fastest order.  Returns the length (at most cap bytes
+ * written), or -1. */
+int64_t orc_xdr_write(const orc_desc* d, const double* psi, int index,
+                      const double* origin, unsigned char* out, size_t cap);
+/* Read such a file as io/src/readxdr.cpp does (read_header, read_block):
+ * checks dim and counts against d, stores the index and the values into the
+ * interior of psi (padded).  Returns 0, or a negative code. */
+int orc_xdr_read(const orc_desc* d, const unsigned char* in, size_t len,
+                 double* psi, int* index);
+
 /* ------------------------------------------------------------ coupled fields
  * Several fields stepped together (Model C, Gray-Scott, Turing:
  * examples/models/modeldefinitions.h:63-81,123-137).  Each field's

@@ -23,6 +23,7 @@ OP_ID, OP_LAP, OP_BILAP, OP_GRADLAP = 0, 1, 2, 3
 AXIS_X, AXIS_Y, AXIS_Z = 0, 1, 2
 LAYOUT_INTERIOR, LAYOUT_PADDED = 0, 1
 PATH_AUTO, PATH_REFERENCE, PATH_FUSED = 0, 1, 2
+PLOT_GNU, PLOT_CSV = 1, 2
 
 STATUS_NAMES = {OK: "SPS_OK", E_INVALID: "SPS_E_INVALID", E_UNSUPPORTED: "SPS_E_UNSUPPORTED",
                 E_HIP: "SPS_E_HIP", E_NOMEM: "SPS_E_NOMEM", E_STATE: "SPS_E_STATE"}
@@ -101,6 +102,9 @@ def load() -> ctypes.CDLL:
         "sps_fd_query": (c_int, [P, POINTER(c_int), POINTER(c_int)]),
         "sps_fd_checkpoint_text": (c_int, [P, c_int, P, P, c_size_t, POINTER(c_size_t)]),
         "sps_fd_checkpoint_read": (c_int, [P, ctypes.c_char_p, c_size_t, POINTER(c_int)]),
+        "sps_fd_plot_text": (c_int, [P, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sps_fd_checkpoint_xdr": (c_int, [P, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sps_fd_checkpoint_read_xdr": (c_int, [P, c_void_p, c_size_t, POINTER(c_int)]),
         "sps_fdm_create": (c_int, [P, POINTER(c_void_p)]),
         "sps_fdm_destroy": (c_int, [P]),
         "sps_fdm_upload": (c_int, [P, c_int, P, c_int]),

@@ -8,10 +8,11 @@
 // function of the value alone: 11 significant digits, round half to even on
 // the exact binary value.
 //
-// format_e10 produces that text without a big-integer conversion: with E the
-// decimal exponent, t = |v| * 10^(10-E) is computed in double arithmetic from
+// format_e<P> produces that text (and the "% 10lE" = "% .6E" text of the
+// plot and CSV files) without a big-integer conversion: with E the decimal
+// exponent, t = |v| * 10^(P-E) is computed in double arithmetic from
 // correctly rounded powers of ten (at most three roundings, so |t - t*| <=
-// 3.3e-16 * t < 3.3e-5 for t < 1e11).  The 11 digits are round(t*); when the
+// 3.3e-16 * t < 3.3e-5 for t < 1e11).  The P+1 digits are round(t*); when the
 // fraction of t is farther than 1e-4 from 1/2, round(t) = round(t*) and the
 // result is exact.  Values closer to a tie, subnormals, infinities and NaNs are
 // reported as not handled; the caller formats those few with the C library.
@@ -29,7 +30,7 @@
 namespace sps {
 namespace dec {
 
-__host__ __device__ inline double pow10_exact_small(int k) {
+__host__ __device__ constexpr double pow10_exact_small(int k) {
   // 10^0 .. 10^22 are exact doubles
   constexpr double p[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
                             1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
@@ -53,19 +54,31 @@ __host__ __device__ inline double pow10_approx(int k) {
   return big[q + 19] * pow10_exact_small(r);
 }
 
-// Text of v as fprintf("%.10E", v) writes it, into out (at most 18 chars, no
-// terminator).  Returns the length, or 0 when the value must be formatted by
-// the C library (near a rounding tie, subnormal, infinite or NaN).
-__host__ __device__ inline int format_e10(double v, char* out) {
+// Text of v as fprintf("%.<P>E", v) writes it (fprintf("% .<P>E") when
+// space is set: a blank where a non-negative value has no sign), into out (at
+// most P + 8 chars, no terminator).  Returns the length, or 0 when the value
+// must be formatted by the C library (near a rounding tie, subnormal,
+// infinite or NaN).  1 <= P <= 14.
+template <int P>
+__host__ __device__ inline int format_e(double v, char* out, bool space = false) {
+  static_assert(P >= 1 && P <= 14, "digits after the point");
+  constexpr double lo = pow10_exact_small(P) - 0.5 - 1e-4;       // t below: E too big
+  constexpr double hi = pow10_exact_small(P + 1) - 0.5 + 1e-4;   // t at or above: E too small
   uint64_t bits;
   __builtin_memcpy(&bits, &v, 8);
   const bool neg = (bits >> 63) != 0;
   const uint64_t mag = bits & 0x7fffffffffffffffull;
   int n = 0;
   if (neg) out[n++] = '-';
+  else if (space) out[n++] = ' ';
   if (mag == 0) {
-    const char z[] = "0.0000000000E+00";
-    for (int i = 0; i < 16; ++i) out[n++] = z[i];
+    out[n++] = '0';
+    out[n++] = '.';
+    for (int i = 0; i < P; ++i) out[n++] = '0';
+    out[n++] = 'E';
+    out[n++] = '+';
+    out[n++] = '0';
+    out[n++] = '0';
     return n;
   }
   const int bexp = (int)(mag >> 52);
@@ -73,44 +86,45 @@ __host__ __device__ inline int format_e10(double v, char* out) {
   double a;
   __builtin_memcpy(&a, &mag, 8);
   // decimal exponent estimate from the binary one, floor(log10(2) * e2): the
-  // true exponent is that or one more, and rounding to 11 digits can carry
+  // true exponent is that or one more, and rounding to P+1 digits can carry
   // once more, so two straight-line corrections suffice (no loop whose exit
   // values are used later)
   int E = (int)__builtin_floor((bexp - 1023) * 0.30102999566398120);
   auto scaled = [a](int Ex) {
-    const int j = 10 - Ex;
+    const int j = P - Ex;
     return j > 300 ? (a * 1e300) * pow10_approx(j - 300) : a * pow10_approx(j);
   };
   double t = scaled(E);
-  if (t < 9999999999.5 - 1e-4) {
+  if (t < lo) {
     E -= 1;
     t = scaled(E);
-  } else if (t >= 99999999999.5 + 1e-4) {
+  } else if (t >= hi) {
     E += 1;
     t = scaled(E);
   }
-  if (t < 9999999999.5 - 1e-4) {
+  if (t < lo) {
     E -= 1;
     t = scaled(E);
-  } else if (t >= 99999999999.5 + 1e-4) {
+  } else if (t >= hi) {
     E += 1;
     t = scaled(E);
   }
-  if (!(t >= 9999999999.5 - 1e-4 && t < 99999999999.5 + 1e-4)) return 0;
+  if (!(t >= lo && t < hi)) return 0;
   const double d0 = __builtin_floor(t);
-  const double f = t - d0;  // exact: t < 2^37
+  const double f = t - d0;  // exact: t < 2^53
   if (__builtin_fabs(f - 0.5) < 1e-4) return 0;  // too close to a tie
   uint64_t d = (uint64_t)d0 + (f > 0.5 ? 1 : 0);
-  if (d < 10000000000ull || d >= 100000000000ull) return 0;
+  const uint64_t dlo = (uint64_t)pow10_exact_small(P);
+  if (d < dlo || d >= 10 * dlo) return 0;
   // digits
-  char dig[11];
-  for (int i = 10; i >= 0; --i) {
+  char dig[P + 1];
+  for (int i = P; i >= 0; --i) {
     dig[i] = (char)('0' + (int)(d % 10));
     d /= 10;
   }
   out[n++] = dig[0];
   out[n++] = '.';
-  for (int i = 1; i < 11; ++i) out[n++] = dig[i];
+  for (int i = 1; i <= P; ++i) out[n++] = dig[i];
   out[n++] = 'E';
   out[n++] = E < 0 ? '-' : '+';
   int ae = E < 0 ? -E : E;
@@ -121,6 +135,11 @@ __host__ __device__ inline int format_e10(double v, char* out) {
   out[n++] = (char)('0' + ae / 10);
   out[n++] = (char)('0' + ae % 10);
   return n;
+}
+
+// The checkpoint number format, "%.10E".
+__host__ __device__ inline int format_e10(double v, char* out) {
+  return format_e<10>(v, out, false);
 }
 
 __host__ __device__ inline bool is_space(char c) {

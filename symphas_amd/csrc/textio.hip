@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -62,20 +63,42 @@ __device__ U block_exclusive_scan(U v, U* total) {
   return incl - v;
 }
 
-template <typename T>
+// How values are laid out in a text file: "%.<P>E" numbers (P a template
+// argument), optionally with the blank flag, each followed by sep (after the
+// last of a row too when sep_last), one row of X values per line.  Rows can
+// carry a prefix (position columns, slice headers): extra[r] is the number of
+// non-value bytes before the values of chunk row r (prefixes of rows <= r plus
+// the newlines of rows < r; r itself when extra is NULL), and row r's prefix
+// is pre_str[pre_off[r] .. pre_off[r+1]).
+struct TextLayout {
+  int space;
+  char sep;
+  int sep_last;
+  const uint64_t* extra;
+  const uint32_t* pre_off;
+  const char* pre_str;
+};
+
+__device__ __forceinline__ bool has_sep(const TextLayout& L, int64_t i,
+                                        int64_t X) {
+  return L.sep_last || (i + 1) % X != 0;
+}
+
+template <typename T, int P>
 __global__ __launch_bounds__(kThreads) void fmt_len(
-    const T* __restrict__ f, int64_t n, uint8_t* __restrict__ len,
-    uint32_t* __restrict__ flag_idx, unsigned* __restrict__ nflag) {
+    const T* __restrict__ f, int64_t n, int64_t X, TextLayout L,
+    uint8_t* __restrict__ len, uint32_t* __restrict__ flag_idx,
+    unsigned* __restrict__ nflag) {
   const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (i >= n) return;
   char buf[24];
-  const int k = dec::format_e10((double)f[i], buf);
+  const int k = dec::format_e<P>((double)f[i], buf, L.space != 0);
   if (k == 0) {
     const unsigned slot = atomicAdd(nflag, 1u);  // capacity n
     flag_idx[slot] = (uint32_t)i;
     len[i] = 0;
   } else {
-    len[i] = (uint8_t)(k + 1);  // the value and its trailing space
+    len[i] = (uint8_t)(k + (has_sep(L, i, X) ? 1 : 0));
   }
 }
 
@@ -107,12 +130,12 @@ __global__ __launch_bounds__(kThreads) void len_block_sum(
   if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// boff[b]: byte offset of value b*kValsPerBlock in the chunk's text, newlines
-// of the rows completed before it included.  Flagged values (sorted fidx)
-// take their text from fstr.
-template <typename T>
+// boff[b]: bytes of the values before value b*kValsPerBlock in the chunk's
+// text (the layout's non-value bytes not included).  Flagged values (sorted
+// fidx) take their text, separator included, from fstr.
+template <typename T, int P>
 __global__ __launch_bounds__(kThreads) void fmt_write(
-    const T* __restrict__ f, int64_t n, int64_t X,
+    const T* __restrict__ f, int64_t n, int64_t X, TextLayout L,
     const uint8_t* __restrict__ len, const uint64_t* __restrict__ boff,
     const uint32_t* __restrict__ fidx, const uint32_t* __restrict__ foff,
     const char* __restrict__ fstr, unsigned nf, char* __restrict__ out) {
@@ -128,17 +151,21 @@ __global__ __launch_bounds__(kThreads) void fmt_write(
   uint32_t total;
   const uint32_t excl = block_exclusive_scan<uint32_t>(s, &total);
   uint64_t off = boff[blockIdx.x] + excl;
-  const int64_t rows0 = base / X;
 #pragma unroll
   for (int q = 0; q < kValsPerThread; ++q) {
     const int64_t i = i0 + q;
     if (i >= n) break;
-    char* o = out + off + (uint64_t)(i / X - rows0);
+    const int64_t r = i / X;
+    char* o = out + off + (L.extra ? L.extra[r] : (uint64_t)r);
+    if (L.pre_off && i % X == 0) {
+      const uint32_t a = L.pre_off[r], b = L.pre_off[r + 1];
+      for (uint32_t c = a; c < b; ++c) o[(int64_t)c - (int64_t)b] = L.pre_str[c];
+    }
     char buf[24];
-    int k = dec::format_e10((double)f[i], buf);
+    int k = dec::format_e<P>((double)f[i], buf, L.space != 0);
     if (k > 0) {
       for (int c = 0; c < k; ++c) o[c] = buf[c];
-      o[k] = ' ';
+      if (has_sep(L, i, X)) o[k] = L.sep;
     } else {
       // binary search of the sorted flagged indices
       unsigned lo = 0, hi = nf;
@@ -151,6 +178,33 @@ __global__ __launch_bounds__(kThreads) void fmt_write(
     }
     if ((i + 1) % X == 0) o[l[q]] = '\n';
     off += l[q];
+  }
+}
+
+// XDR doubles (RFC 4506: big-endian IEEE 754) from the field, two per thread.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void xdr_pack(const T* __restrict__ f,
+                                                     int64_t n,
+                                                     uint64_t* __restrict__ out) {
+  const int64_t i = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * 2;
+  if (i + 1 < n) {
+    const double a = (double)f[i], b = (double)f[i + 1];
+    out[i] = __builtin_bswap64((uint64_t)__double_as_longlong(a));
+    out[i + 1] = __builtin_bswap64((uint64_t)__double_as_longlong(b));
+  } else if (i < n) {
+    out[i] = __builtin_bswap64((uint64_t)__double_as_longlong((double)f[i]));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void xdr_unpack(
+    const uint64_t* __restrict__ in, int64_t n, T* __restrict__ f) {
+  const int64_t i = ((int64_t)blockIdx.x * kThreads + threadIdx.x) * 2;
+  if (i + 1 < n) {
+    f[i] = (T)__longlong_as_double((long long)__builtin_bswap64(in[i]));
+    f[i + 1] = (T)__longlong_as_double((long long)__builtin_bswap64(in[i + 1]));
+  } else if (i < n) {
+    f[i] = (T)__longlong_as_double((long long)__builtin_bswap64(in[i]));
   }
 }
 
@@ -257,8 +311,38 @@ std::string gp_header(const sps_fd_desc& d, int index, const double* origin) {
 
 unsigned nblocks(int64_t n, int64_t per) { return (unsigned)((n + per - 1) / per); }
 
+// A text file layout on the host side: the number format, the separators,
+// the file header and tail, and the prefix of each row (global row index g,
+// appended to out), if any.
+struct TextFormat {
+  int prec = 10;  // 10 ("%.10E") or 6 ("% 10lE")
+  bool space = false;
+  char sep = ' ';
+  bool sep_last = true;
+  std::string header, tail;
+  std::function<void(int64_t g, std::string& out)> prefix;
+};
+
+template <typename T, int P>
+void launch_fmt_len(const T* f, int64_t cnt, int64_t X, const TextLayout& L,
+                    uint8_t* len, uint32_t* fidx, unsigned* nflag,
+                    hipStream_t st) {
+  fmt_len<T, P><<<nblocks(cnt, kThreads), kThreads, 0, st>>>(f, cnt, X, L, len,
+                                                             fidx, nflag);
+}
+
+template <typename T, int P>
+void launch_fmt_write(const T* f, int64_t cnt, int64_t X, const TextLayout& L,
+                      const uint8_t* len, const uint64_t* boff,
+                      const uint32_t* fidx, const uint32_t* foff,
+                      const char* fstr, unsigned nf, char* out,
+                      hipStream_t st) {
+  fmt_write<T, P><<<nblocks(cnt, kValsPerBlock), kThreads, 0, st>>>(
+      f, cnt, X, L, len, boff, fidx, foff, fstr, nf, out);
+}
+
 template <typename T>
-int write_text(sps_fd_plan* p, const std::string& header, char* text,
+int write_text(sps_fd_plan* p, const TextFormat& fm, char* text,
                size_t capacity, size_t* length) {
   hipStream_t st = p->stream;
   const T* field = (const T*)p->buf[p->cur];
@@ -267,6 +351,7 @@ int write_text(sps_fd_plan* p, const std::string& header, char* text,
   const int64_t rows_per_chunk = std::max<int64_t>(1, kChunkValues / X);
   const int64_t chunk_cap = std::min<int64_t>(nsites, rows_per_chunk * X);
   const unsigned nb_cap = nblocks(chunk_cap, kValsPerBlock);
+  const bool p6 = fm.prec == 6;
 
   DevBufs db;
   uint8_t* d_len;
@@ -279,30 +364,70 @@ int write_text(sps_fd_plan* p, const std::string& header, char* text,
   SPS_TRY(db.alloc(&d_bsum, nb_cap * sizeof(uint32_t)));
   SPS_TRY(db.alloc(&d_boff, nb_cap * sizeof(uint64_t)));
   char* d_out = nullptr;
-  if (text)
-    SPS_TRY(db.alloc(&d_out, (size_t)chunk_cap * kMaxText + rows_per_chunk + 16));
+  const size_t value_room = (size_t)chunk_cap * kMaxText;
 
-  size_t total = header.size();
+  size_t total = fm.header.size();
   if (text) {
     if (capacity < total) {
-      set_error("checkpoint text: capacity %zu is below the header alone",
-                capacity);
+      set_error("text file: capacity %zu is below the header alone", capacity);
       return SPS_E_INVALID;
     }
-    std::memcpy(text, header.data(), header.size());
+    std::memcpy(text, fm.header.data(), fm.header.size());
   }
   std::vector<uint32_t> hidx;
   std::vector<double> hval;
   std::vector<uint32_t> hbsum(nb_cap);
   std::vector<uint64_t> hboff(nb_cap);
+  std::string pre;
+  std::vector<uint32_t> pre_off;
+  std::vector<uint64_t> extra;
+  size_t out_cap = 0;
   for (int64_t start = 0; start < nsites; start += chunk_cap) {
     const int64_t cnt = std::min<int64_t>(chunk_cap, nsites - start);
     const int64_t rows = cnt / X;
     const T* f = field + start;
     const unsigned nb = nblocks(cnt, kValsPerBlock);
+    DevBufs chunkbufs;
+    TextLayout L{fm.space ? 1 : 0, fm.sep, fm.sep_last ? 1 : 0, nullptr, nullptr,
+                 nullptr};
+    uint64_t extra_total = (uint64_t)rows;  // the newlines
+    if (fm.prefix) {
+      pre.clear();
+      pre_off.assign((size_t)rows + 1, 0);
+      extra.assign((size_t)rows, 0);
+      uint64_t acc = 0;
+      for (int64_t r = 0; r < rows; ++r) {
+        pre_off[r] = (uint32_t)pre.size();
+        fm.prefix(start / X + r, pre);
+        const uint64_t pl = pre.size() - pre_off[r];
+        extra[r] = acc + pl;
+        acc += pl + 1;
+      }
+      pre_off[rows] = (uint32_t)pre.size();
+      extra_total = acc;
+      uint64_t* d_extra;
+      uint32_t* d_preoff;
+      char* d_pre;
+      SPS_TRY(chunkbufs.alloc(&d_extra, rows * sizeof(uint64_t)));
+      SPS_TRY(chunkbufs.alloc(&d_preoff, (rows + 1) * sizeof(uint32_t)));
+      SPS_TRY(chunkbufs.alloc(&d_pre, pre.size()));
+      SPS_HIP_TRY(hipMemcpyAsync(d_extra, extra.data(), rows * sizeof(uint64_t),
+                                 hipMemcpyHostToDevice, st));
+      SPS_HIP_TRY(hipMemcpyAsync(d_preoff, pre_off.data(),
+                                 (rows + 1) * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, st));
+      if (!pre.empty())
+        SPS_HIP_TRY(hipMemcpyAsync(d_pre, pre.data(), pre.size(),
+                                   hipMemcpyHostToDevice, st));
+      L.extra = d_extra;
+      L.pre_off = d_preoff;
+      L.pre_str = d_pre;
+    }
     SPS_HIP_TRY(hipMemsetAsync(d_nflag, 0, sizeof(unsigned), st));
-    fmt_len<T><<<nblocks(cnt, kThreads), kThreads, 0, st>>>(f, cnt, d_len,
-                                                            d_fidx, d_nflag);
+    if (p6)
+      launch_fmt_len<T, 6>(f, cnt, X, L, d_len, d_fidx, d_nflag, st);
+    else
+      launch_fmt_len<T, 10>(f, cnt, X, L, d_len, d_fidx, d_nflag, st);
     unsigned nf = 0;
     SPS_HIP_TRY(hipMemcpyAsync(&nf, d_nflag, sizeof(unsigned),
                                hipMemcpyDeviceToHost, st));
@@ -331,7 +456,9 @@ int write_text(sps_fd_plan* p, const std::string& header, char* text,
       std::string sstr;
       char buf[64];
       for (unsigned k = 0; k < nf; ++k) {
-        const int w = std::snprintf(buf, sizeof(buf), "%.10E ", pv[k].second);
+        int w = p6 ? std::snprintf(buf, sizeof(buf), "% 10lE", pv[k].second)
+                   : std::snprintf(buf, sizeof(buf), "%.10E", pv[k].second);
+        if (fm.sep_last || (pv[k].first + 1) % X != 0) buf[w++] = fm.sep;
         sidx[k] = pv[k].first;
         soff[k] = (uint32_t)sstr.size();
         slen[k] = (uint8_t)w;
@@ -361,19 +488,34 @@ int write_text(sps_fd_plan* p, const std::string& header, char* text,
     SPS_HIP_TRY(hipStreamSynchronize(st));
     uint64_t acc = 0;
     for (unsigned b = 0; b < nb; ++b) {
-      hboff[b] = acc + (uint64_t)((int64_t)b * kValsPerBlock / X);
+      hboff[b] = acc;
       acc += hbsum[b];
     }
-    const uint64_t chunk_bytes = acc + (uint64_t)rows;
+    const uint64_t chunk_bytes = acc + extra_total;
     if (text) {
       if (total + chunk_bytes > capacity) {
-        set_error("checkpoint text: capacity %zu too small", capacity);
+        set_error("text file: capacity %zu too small", capacity);
         return SPS_E_INVALID;
+      }
+      const size_t need = std::max<size_t>(value_room + (size_t)extra_total,
+                                           (size_t)chunk_bytes) + 16;
+      if (need > out_cap) {
+        if (d_out) {
+          hipFree(d_out);
+          db.ptrs.erase(std::find(db.ptrs.begin(), db.ptrs.end(), (void*)d_out));
+          d_out = nullptr;
+        }
+        SPS_TRY(db.alloc(&d_out, need));
+        out_cap = need;
       }
       SPS_HIP_TRY(hipMemcpyAsync(d_boff, hboff.data(), nb * sizeof(uint64_t),
                                  hipMemcpyHostToDevice, st));
-      fmt_write<T><<<nb, kThreads, 0, st>>>(f, cnt, X, d_len, d_boff, d_sidx,
-                                            d_soff, d_sstr, nf, d_out);
+      if (p6)
+        launch_fmt_write<T, 6>(f, cnt, X, L, d_len, d_boff, d_sidx, d_soff,
+                               d_sstr, nf, d_out, st);
+      else
+        launch_fmt_write<T, 10>(f, cnt, X, L, d_len, d_boff, d_sidx, d_soff,
+                                d_sstr, nf, d_out, st);
       SPS_HIP_TRY(hipMemcpyAsync(text + total, d_out, chunk_bytes,
                                  hipMemcpyDeviceToHost, st));
       SPS_HIP_TRY(hipStreamSynchronize(st));
@@ -382,12 +524,59 @@ int write_text(sps_fd_plan* p, const std::string& header, char* text,
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    set_error("checkpoint text kernels failed: %s", hipGetErrorString(e));
+    set_error("text file kernels failed: %s", hipGetErrorString(e));
     return SPS_E_HIP;
   }
+  if (text) {
+    if (total + fm.tail.size() > capacity) {
+      set_error("text file: capacity %zu too small", capacity);
+      return SPS_E_INVALID;
+    }
+    std::memcpy(text + total, fm.tail.data(), fm.tail.size());
+  }
+  total += fm.tail.size();
   if (text && total < capacity) text[total] = 0;
   *length = total;
   return SPS_OK;
+}
+
+// The plot-file layouts (io/src/writegp.cpp save_grid_plotting :470-507,
+// io/src/writecsv.cpp save_grid_plotting): "% 10lE" values; GNU matrix rows
+// start with the y position "% 13.2f " and every z slice with a line of its
+// index "% 13d " and the x positions, slices separated by a blank line, and
+// the file ends with an empty line.
+TextFormat plot_format(const sps_fd_desc& d, int format, const double* origin) {
+  TextFormat fm;
+  fm.prec = 6;
+  fm.space = true;
+  if (format == SPS_PLOT_CSV) {
+    fm.sep = ',';
+    fm.sep_last = false;
+    return fm;
+  }
+  fm.sep = ' ';
+  fm.sep_last = true;
+  fm.tail = "\n";
+  const double x0 = origin ? origin[0] : 0.0, y0 = origin ? origin[1] : 0.0;
+  const double h = d.h;
+  const int X = d.n[0], Y = d.n[1];
+  fm.prefix = [x0, y0, h, X, Y](int64_t g, std::string& out) {
+    char buf[64];
+    const int64_t j = g % Y, k = g / Y;
+    if (j == 0) {
+      if (k > 0) out += '\n';
+      std::snprintf(buf, sizeof(buf), "% 13d ", (int)k);
+      out += buf;
+      for (int i = 0; i < X; ++i) {
+        std::snprintf(buf, sizeof(buf), "% 13.2f ", x0 + i * h);
+        out += buf;
+      }
+      out += '\n';
+    }
+    std::snprintf(buf, sizeof(buf), "% 13.2f ", y0 + (double)j * h);
+    out += buf;
+  };
+  return fm;
 }
 
 // Header as read_header (io/src/readgp.cpp) reads it; returns the bytes
@@ -528,10 +717,163 @@ int read_text(sps_fd_plan* p, const char* body, int64_t m) {
   return SPS_OK;
 }
 
+// XDR file header (io/src/writexdr.cpp print_xdr_header :45-87): int dim,
+// int counts, per axis the domain (left, right), int index, int dim, per axis
+// the interval (left, right); ints as 4 big-endian bytes, doubles as 8.
+std::string xdr_header(const sps_fd_desc& d, int index, const double* origin) {
+  std::string h;
+  auto put_int = [&h](int v) {
+    const uint32_t u = (uint32_t)v;
+    for (int b = 3; b >= 0; --b) h += (char)(u >> (8 * b));
+  };
+  auto put_double = [&h](double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    for (int b = 7; b >= 0; --b) h += (char)(u >> (8 * b));
+  };
+  auto bounds = [&](int a) {
+    const double left = origin ? origin[a] : 0.0;
+    put_double(left);
+    put_double(left + (d.n[a] - 1) * d.h);
+  };
+  put_int(d.dim);
+  for (int a = 0; a < d.dim; ++a) put_int(d.n[a]);
+  for (int a = 0; a < d.dim; ++a) bounds(a);
+  put_int(index);
+  put_int(d.dim);
+  for (int a = 0; a < d.dim; ++a) bounds(a);
+  return h;
+}
+
+constexpr int64_t kXdrChunk = int64_t(1) << 25;  // values per staged copy
+
+template <typename T>
+int write_xdr(sps_fd_plan* p, const std::string& header, unsigned char* data,
+              size_t capacity) {
+  hipStream_t st = p->stream;
+  const T* field = (const T*)p->buf[p->cur];
+  const int64_t n = (int64_t)p->nsites;
+  std::memcpy(data, header.data(), header.size());
+  DevBufs db;
+  uint64_t* d_buf;
+  const int64_t chunk = std::min<int64_t>(n, kXdrChunk);
+  SPS_TRY(db.alloc(&d_buf, (size_t)chunk * 8));
+  for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+    const int64_t cnt = std::min<int64_t>(chunk, n - s0);
+    xdr_pack<T><<<nblocks((cnt + 1) / 2, kThreads), kThreads, 0, st>>>(
+        field + s0, cnt, d_buf);
+    SPS_HIP_TRY(hipMemcpyAsync(data + header.size() + (size_t)s0 * 8, d_buf,
+                               (size_t)cnt * 8, hipMemcpyDeviceToHost, st));
+    SPS_HIP_TRY(hipStreamSynchronize(st));
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("XDR kernels failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  (void)capacity;
+  return SPS_OK;
+}
+
+template <typename T>
+int read_xdr(sps_fd_plan* p, const unsigned char* body) {
+  hipStream_t st = p->stream;
+  T* field = (T*)p->buf[p->cur];
+  const int64_t n = (int64_t)p->nsites;
+  DevBufs db;
+  uint64_t* d_buf;
+  const int64_t chunk = std::min<int64_t>(n, kXdrChunk);
+  SPS_TRY(db.alloc(&d_buf, (size_t)chunk * 8));
+  for (int64_t s0 = 0; s0 < n; s0 += chunk) {
+    const int64_t cnt = std::min<int64_t>(chunk, n - s0);
+    SPS_HIP_TRY(hipMemcpyAsync(d_buf, body + (size_t)s0 * 8, (size_t)cnt * 8,
+                               hipMemcpyHostToDevice, st));
+    xdr_unpack<T><<<nblocks((cnt + 1) / 2, kThreads), kThreads, 0, st>>>(
+        d_buf, cnt, field + s0);
+    SPS_HIP_TRY(hipStreamSynchronize(st));
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("XDR kernels failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
 }  // namespace
 }  // namespace sps
 
 using namespace sps;
+
+extern "C" int sps_fd_checkpoint_xdr(sps_fd_plan* p, int index,
+                                     const double* origin, void* data,
+                                     size_t capacity, size_t* length) {
+  if (!p || !length) {
+    set_error("plan or length pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  const std::string header = xdr_header(p->d, index, origin);
+  const size_t total = header.size() + 8 * p->nsites;
+  *length = total;
+  if (!data) return SPS_OK;
+  if (capacity < total) {
+    set_error("XDR file: capacity %zu below the %zu bytes of the file", capacity,
+              total);
+    return SPS_E_INVALID;
+  }
+  return p->d.dtype == SPS_F32
+             ? write_xdr<float>(p, header, (unsigned char*)data, capacity)
+             : write_xdr<double>(p, header, (unsigned char*)data, capacity);
+}
+
+extern "C" int sps_fd_checkpoint_read_xdr(sps_fd_plan* p, const void* data,
+                                          size_t length, int* index) {
+  if (!p || !data) {
+    set_error("plan or data is NULL");
+    return SPS_E_INVALID;
+  }
+  // read_header (io/src/readxdr.cpp): counts must match the plan
+  const unsigned char* b = (const unsigned char*)data;
+  size_t pos = 0;
+  auto get_int = [&](int* v) {
+    if (pos + 4 > length) return false;
+    uint32_t u = 0;
+    for (int q = 0; q < 4; ++q) u = (u << 8) | b[pos++];
+    *v = (int)u;
+    return true;
+  };
+  int dim = 0, c = 0, idx = 0, dim2 = 0;
+  if (!get_int(&dim) || dim != p->d.dim) {
+    set_error("XDR file: dimension does not match the plan");
+    return SPS_E_INVALID;
+  }
+  for (int a = 0; a < dim; ++a)
+    if (!get_int(&c) || c != p->d.n[a]) {
+      set_error("XDR file: count of axis %d does not match the plan", a);
+      return SPS_E_INVALID;
+    }
+  pos += 16 * (size_t)dim;  // domain bounds
+  if (!get_int(&idx) || !get_int(&dim2) || dim2 != dim) {
+    set_error("XDR file: block header unreadable");
+    return SPS_E_INVALID;
+  }
+  pos += 16 * (size_t)dim;  // interval bounds
+  if (pos > length || length - pos < 8 * p->nsites) {
+    set_error("XDR file: %zu bytes of values for a field of %zu sites",
+              pos > length ? (size_t)0 : length - pos, p->nsites);
+    return SPS_E_INVALID;
+  }
+  const int rc = p->d.dtype == SPS_F32 ? read_xdr<float>(p, b + pos)
+                                       : read_xdr<double>(p, b + pos);
+  if (rc != SPS_OK) return rc;
+  p->has_data = true;
+  if (index) *index = idx;
+  return SPS_OK;
+}
 
 extern "C" int sps_fd_checkpoint_text(sps_fd_plan* p, int index,
                                       const double* origin, char* text,
@@ -548,10 +890,36 @@ extern "C" int sps_fd_checkpoint_text(sps_fd_plan* p, int index,
     set_error("checkpoint text: fields above 2^32 sites are not supported");
     return SPS_E_UNSUPPORTED;
   }
-  const std::string header = gp_header(p->d, index, origin);
+  TextFormat fm;
+  fm.header = gp_header(p->d, index, origin);
   return p->d.dtype == SPS_F32
-             ? write_text<float>(p, header, text, capacity, length)
-             : write_text<double>(p, header, text, capacity, length);
+             ? write_text<float>(p, fm, text, capacity, length)
+             : write_text<double>(p, fm, text, capacity, length);
+}
+
+extern "C" int sps_fd_plot_text(sps_fd_plan* p, int format, const double* origin,
+                                char* text, size_t capacity, size_t* length) {
+  if (!p || !length) {
+    set_error("plan or length pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (format != SPS_PLOT_GNU && format != SPS_PLOT_CSV) {
+    set_error("plot text: format %d is neither SPS_PLOT_GNU nor SPS_PLOT_CSV",
+              format);
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  if (p->nsites > 0xffffffffull) {
+    set_error("plot text: fields above 2^32 sites are not supported");
+    return SPS_E_UNSUPPORTED;
+  }
+  const TextFormat fm = plot_format(p->d, format, origin);
+  return p->d.dtype == SPS_F32
+             ? write_text<float>(p, fm, text, capacity, length)
+             : write_text<double>(p, fm, text, capacity, length);
 }
 
 extern "C" int sps_fd_checkpoint_read(sps_fd_plan* p, const char* text,

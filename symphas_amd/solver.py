@@ -137,6 +137,49 @@ class FDPlan:
         abi.check(self.lib.sps_fd_checkpoint_read(self._h, text, len(text), byref(idx)))
         return idx.value
 
+    def plot_text(self, fmt: int = abi.PLOT_GNU, origin=None) -> bytes:
+        """The field as the reference's plot data file: abi.PLOT_GNU (GNU
+        matrix with positions) or abi.PLOT_CSV, formatted on the device."""
+        o = None
+        if origin is not None:
+            o_arr = np.ascontiguousarray(origin, dtype=np.float64)
+            o = o_arr.ctypes.data_as(c_void_p)
+        # upper bound: 15 bytes per value ("-d.ddddddE+ddd "), 15 per row
+        # (position column and newline), a header line of 14 bytes per x
+        # position per z slice, blank lines
+        n_ = [int(self.desc.n[a]) for a in range(self.desc.dim)]
+        sites = int(np.prod(n_))
+        rows = sites // n_[0]
+        slices = n_[2] if self.desc.dim == 3 else 1
+        cap = 15 * sites + 15 * rows + slices * (14 * (n_[0] + 1) + 2) + 16
+        buf = np.empty(cap, dtype=np.uint8)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fd_plot_text(self._h, int(fmt), o, buf.ctypes.data_as(c_void_p),
+                                            cap, byref(n)))
+        return buf[:n.value].tobytes()
+
+    def checkpoint_xdr(self, index: int = 0, origin=None) -> bytes:
+        """The field as the reference's XDR file (big-endian doubles)."""
+        o = None
+        if origin is not None:
+            o_arr = np.ascontiguousarray(origin, dtype=np.float64)
+            o = o_arr.ctypes.data_as(c_void_p)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fd_checkpoint_xdr(self._h, int(index), o, None, 0, byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        abi.check(self.lib.sps_fd_checkpoint_xdr(self._h, int(index), o,
+                                                 buf.ctypes.data_as(c_void_p), buf.size,
+                                                 byref(n)))
+        return buf.tobytes()
+
+    def read_checkpoint_xdr(self, data: bytes) -> int:
+        """Load an XDR file into the field; returns its index."""
+        idx = c_int(0)
+        arr = np.frombuffer(data, dtype=np.uint8)
+        abi.check(self.lib.sps_fd_checkpoint_read_xdr(self._h, arr.ctypes.data_as(c_void_p),
+                                                      arr.size, byref(idx)))
+        return idx.value
+
     def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0,
              values_per_lane: int = 0):
         abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk,

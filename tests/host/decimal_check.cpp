@@ -1,5 +1,7 @@
 // Host check of symphas_amd/csrc/decimal.hpp against the C library:
-// format_e10 against snprintf("%.10E") and parse_fast against strtod.
+// format_e10 against snprintf("%.10E"), format_e<6> with the blank flag
+// against snprintf("% 10lE") (the plot and CSV number format) and parse_fast
+// against strtod.
 //   decimal_check <count> <seed>
 // Prints "checked N formatted F (fallback B) parsed P" and exits non-zero
 // on the first mismatch.
@@ -34,6 +36,14 @@ static bool check(double v) {
       std::printf("FORMAT MISMATCH %a: want %s got %.*s\n", v, want, n, got);
       return false;
     }
+  }
+  char want6[64], got6[32];
+  std::snprintf(want6, sizeof(want6), "% 10lE", v);
+  const int n6 = sps::dec::format_e<6>(v, got6, true);
+  if (n6 != 0 &&
+      ((size_t)n6 != std::strlen(want6) || std::memcmp(got6, want6, n6) != 0)) {
+    std::printf("FORMAT6 MISMATCH %a: want '%s' got '%.*s'\n", v, want6, n6, got6);
+    return false;
   }
   double r;
   if (sps::dec::parse_fast(want, (int)std::strlen(want), &r)) {
