@@ -21,6 +21,10 @@
 //   SolverSystemFD_MI355X<D>                    the system: values, dframe, update
 //   Model<D, Solver>, run_model, find_solution  modules-models.h:436-492,
 //                                               modules-none.h:43-53
+//   op(1) .. op(4), MultiModel<D, Solver, F>    coupled fields (MC, GRAYSCOTT,
+//                                               Turing: examples/models/
+//                                               modeldefinitions.h:63-81,
+//                                               123-137), through sps_fdm_*
 //
 // Errors throw sps::symphas::solver_error carrying the C ABI's status and
 // message (the reference exits; see sol/inc/stencil.h:671-679).
@@ -36,6 +40,7 @@
 #include <string>
 #include <tuple>
 #include <utility>
+#include <vector>
 #include <vector>
 
 #include "../../include/sps_fd.h"
@@ -101,14 +106,16 @@ inline Expr make_const(double v) {
   return {p};
 }
 
-// op(1): the single order parameter of the model
+// op(k): the k-th order parameter of the model (1-based).  A single-field
+// Model takes op(1) only; a MultiModel takes op(1) .. op(SPS_MAX_FIELDS).
 inline Expr op(int index) {
-  if (index != 1)
+  if (index < 1 || index > SPS_MAX_FIELDS)
     throw solver_error(SPS_E_UNSUPPORTED,
-                       "op(" + std::to_string(index) +
-                           "): the MI355X step advances single-field models");
+                       "op(" + std::to_string(index) + "): at most " +
+                           std::to_string(SPS_MAX_FIELDS) + " fields");
   auto p = std::make_shared<Node>();
   p->kind = Node::Field;
+  p->index = index;
   return {p};
 }
 
@@ -258,6 +265,11 @@ inline Value lower_node(const NodeP& n, const std::vector<double>& coeffs) {
                  : 1.0;
       return v;
     case Node::Field:
+      if (n->index != 1)
+        throw solver_error(SPS_E_UNSUPPORTED,
+                           "op(" + std::to_string(n->index) +
+                               ") in a single-field model (MultiModel steps "
+                               "several fields)");
       v.poly[0].k[1] = 1.0;
       return v;
     case Node::Var:
@@ -383,6 +395,155 @@ inline int lower_terms(const Expr& rhs, sps_fd_term* terms, int max_terms,
   push(SPS_OP_ID, 0, 0, id0);
   for (int operand = 1; operand <= SPS_MAX_PROV; ++operand)
     if (v.poly.count(operand)) push(SPS_OP_ID, 0, operand, v.poly.at(operand));
+  return nt;
+}
+
+// ------------------------------------------------ several coupled fields
+
+// Lowering of the right-hand side of one field of a multi-field model into
+// sps_fdm_term's coeff * D[ prod_f psi_f^pow[f] ]: products are expanded into
+// monomials of the fields, derivatives distribute over the monomials (one term
+// per derivative of a monomial, its coefficient outside), lap(lap) is the
+// bilaplacian.  Terms come out by category (bilaplacian, laplacian,
+// gradlaplacian by axis, pointwise) and, within one, in the order in which
+// their monomials first appear in the expression; equal monomials merge.
+namespace detail {
+
+using Mono = std::array<int, SPS_MAX_FIELDS>;
+
+struct MValue {
+  std::vector<std::pair<Mono, double>> poly;  // pointwise, constant included
+  std::vector<std::pair<std::tuple<int, int, Mono>, double>> derivs;
+
+  bool is_const() const {
+    for (auto& kv : poly)
+      if (kv.second != 0.0 && kv.first != Mono{}) return false;
+    return derivs.empty();
+  }
+  double constant() const {
+    for (auto& kv : poly)
+      if (kv.first == Mono{}) return kv.second;
+    return 0.0;
+  }
+};
+
+template <typename K>
+inline void madd(std::vector<std::pair<K, double>>& v, const K& k, double c) {
+  for (auto& kv : v)
+    if (kv.first == k) {
+      kv.second += c;
+      return;
+    }
+  v.emplace_back(k, c);
+}
+
+inline MValue mscale(MValue v, double s) {
+  for (auto& kv : v.poly) kv.second *= s;
+  for (auto& kv : v.derivs) kv.second *= s;
+  return v;
+}
+
+inline MValue lower_mnode(const NodeP& n, const std::vector<double>& coeffs) {
+  MValue v;
+  switch (n->kind) {
+    case Node::Const:
+      madd(v.poly, Mono{}, n->value);
+      return v;
+    case Node::Coeff:
+      madd(v.poly, Mono{},
+           (n->index >= 1 && (size_t)n->index <= coeffs.size())
+               ? coeffs[n->index - 1]
+               : 1.0);
+      return v;
+    case Node::Field: {
+      Mono m{};
+      m[n->index - 1] = 1;
+      madd(v.poly, m, 1.0);
+      return v;
+    }
+    case Node::Var:
+      throw solver_error(SPS_E_UNSUPPORTED,
+                         "provisional variables in a multi-field model");
+    case Node::Neg:
+      return mscale(lower_mnode(n->a, coeffs), -1.0);
+    case Node::Add: {
+      MValue a = lower_mnode(n->a, coeffs), b = lower_mnode(n->b, coeffs);
+      for (auto& kv : b.poly) madd(a.poly, kv.first, kv.second);
+      for (auto& kv : b.derivs) madd(a.derivs, kv.first, kv.second);
+      return a;
+    }
+    case Node::Mul: {
+      MValue a = lower_mnode(n->a, coeffs), b = lower_mnode(n->b, coeffs);
+      if (b.is_const()) return mscale(a, b.constant());
+      if (a.is_const()) return mscale(b, a.constant());
+      if (!a.derivs.empty() || !b.derivs.empty())
+        throw solver_error(SPS_E_UNSUPPORTED,
+                           "derivative multiplied by a non-constant");
+      for (auto& x : a.poly)
+        for (auto& y : b.poly) {
+          if (x.second == 0.0 || y.second == 0.0) continue;
+          Mono m{};
+          int deg = 0;
+          for (int f = 0; f < SPS_MAX_FIELDS; ++f) {
+            m[f] = x.first[f] + y.first[f];
+            deg += m[f];
+          }
+          if (deg > SPS_MAX_DEGREE)
+            throw solver_error(SPS_E_UNSUPPORTED, "monomial of degree above 6");
+          madd(v.poly, m, x.second * y.second);
+        }
+      return v;
+    }
+    case Node::Deriv: {
+      MValue in = lower_mnode(n->a, coeffs);
+      for (auto& kv : in.poly)
+        if (kv.second != 0.0 && kv.first != Mono{})
+          madd(v.derivs, std::make_tuple(n->order, n->axis, kv.first),
+               kv.second);
+      for (auto& kv : in.derivs) {
+        const int inner = std::get<0>(kv.first);
+        if (!(n->order == 2 && inner == 2))
+          throw solver_error(SPS_E_UNSUPPORTED,
+                             "derivative of order " +
+                                 std::to_string(n->order + inner) +
+                                 " other than lap(lap)");
+        madd(v.derivs, std::make_tuple(4, 0, std::get<2>(kv.first)),
+             kv.second);
+      }
+      return v;
+    }
+  }
+  return v;
+}
+
+}  // namespace detail
+
+inline int lower_mterms(const Expr& rhs, sps_fdm_term* terms, int max_terms,
+                        const std::vector<double>& coeffs = {}) {
+  detail::MValue v = detail::lower_mnode(rhs.n, coeffs);
+  int nt = 0;
+  auto push = [&](int op, int axis, const detail::Mono& m, double c) {
+    if (c == 0.0) return;
+    if (nt >= max_terms)
+      throw solver_error(SPS_E_UNSUPPORTED,
+                         "more than " + std::to_string(max_terms) + " terms");
+    sps_fdm_term& t = terms[nt++];
+    std::memset(&t, 0, sizeof(t));
+    t.op = op;
+    t.axis = axis;
+    t.coeff = c;
+    for (int f = 0; f < SPS_MAX_FIELDS; ++f) t.pow[f] = m[f];
+  };
+  for (int order : {4, 2})
+    for (auto& kv : v.derivs)
+      if (std::get<0>(kv.first) == order)
+        push(order == 4 ? SPS_OP_BILAP : SPS_OP_LAP, 0, std::get<2>(kv.first),
+             kv.second);
+  for (int axis = 0; axis < 3; ++axis)
+    for (auto& kv : v.derivs)
+      if (std::get<0>(kv.first) == 3 && std::get<1>(kv.first) == axis)
+        push(SPS_OP_GRADLAP, axis, std::get<2>(kv.first), kv.second);
+  for (auto& kv : v.poly) push(SPS_OP_ID, 0, kv.first, kv.second);
   return nt;
 }
 
@@ -636,6 +797,133 @@ struct Model {
       eq = solver.form_expr_one(system0, rhs);
       eq_dt = dt;
     }
+  }
+};
+
+// MultiModel<D, Sp, F>: a model of F coupled fields (MODEL(MC, (SCALARS(2)),
+// ...), GRAYSCOTT, Turing: examples/models/modeldefinitions.h:63-81,123-137),
+// one EVOLUTION right-hand side per field, every field stepped from the same
+// current fields (modelspecialized.h:383-385 then model.h:931-936).  The
+// fields live in one device plan (sps_fdm_*); system(i) is field i's padded
+// host mirror, refreshed on demand.
+template <size_t D, typename Sp, size_t F>
+struct MultiModel {
+  static_assert(F >= 1 && F <= SPS_MAX_FIELDS, "1 .. SPS_MAX_FIELDS fields");
+  Sp solver;
+  std::array<int, D> dims{};
+  std::array<int, D> padded{};
+  std::array<std::vector<double>, F> values;  // padded host mirrors
+  std::array<Expr, F> rhs;
+  sps_fdm_plan* plan = nullptr;
+  bool host_stale = false;
+  long index = 0;
+  double time = 0;
+  double eq_dt = -1;
+
+  // initial[f]: interior values of field f, x fastest (empty = zeros)
+  MultiModel(const problem_parameters<D>& p, std::array<Expr, F> evolution,
+             std::array<std::vector<double>, F> initial = {},
+             int dtype = SPS_F64, int path = SPS_PATH_AUTO,
+             std::vector<double> coeffs = {})
+      : solver(Sp::make_solver(p)), dims(p.dims), rhs(std::move(evolution)) {
+    solver.dtype = dtype;
+    solver.path = path;
+    solver.coeffs = std::move(coeffs);
+    size_t len = 1, inner = 1;
+    for (size_t a = 0; a < D; ++a) {
+      padded[a] = dims[a] + 2 * SPS_BOUNDARY_DEPTH;
+      len *= (size_t)padded[a];
+      inner *= (size_t)dims[a];
+    }
+    for (size_t f = 0; f < F; ++f) {
+      values[f].assign(len, 0.0);
+      if (initial[f].empty()) continue;
+      if (initial[f].size() != inner)
+        throw solver_error(SPS_E_INVALID, "initial field size mismatch");
+      const int B = SPS_BOUNDARY_DEPTH;
+      const int nz = D == 3 ? dims[2] : 1;
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < dims[1]; ++y)
+          for (int x = 0; x < dims[0]; ++x)
+            values[f][(x + B) + (size_t)padded[0] *
+                                    ((y + B) + (size_t)padded[1] *
+                                                   (D == 3 ? z + B : 0))] =
+                initial[f][x + (size_t)dims[0] * (y + (size_t)dims[1] * z)];
+    }
+  }
+  MultiModel(const MultiModel&) = delete;
+  MultiModel& operator=(const MultiModel&) = delete;
+  ~MultiModel() {
+    if (plan) sps_fdm_destroy(plan);
+  }
+
+  // field f's padded values with fresh ghosts
+  const std::vector<double>& system(size_t f) {
+    if (host_stale && plan) {
+      for (size_t g = 0; g < F; ++g) {
+        if (solver.dtype == SPS_F64) {
+          check(sps_fdm_download(plan, (int)g, values[g].data(),
+                                 SPS_LAYOUT_PADDED),
+                "download");
+        } else {
+          std::vector<float> tmp(values[g].size());
+          check(sps_fdm_download(plan, (int)g, tmp.data(), SPS_LAYOUT_PADDED),
+                "download");
+          for (size_t i = 0; i < tmp.size(); ++i) values[g][i] = tmp[i];
+        }
+      }
+      host_stale = false;
+    }
+    return values[f];
+  }
+  long get_index() const { return index; }
+
+  void equation() {}
+  void step(double dt) { step_n(dt, 1); }
+  void step_n(double dt, int n) {
+    ensure_plan(dt);
+    check(sps_fdm_step(plan, n), "step");
+    host_stale = true;
+    index += n;
+  }
+  void update(double t) { time = t; }
+
+ private:
+  // form_expr_one for every field into one descriptor; the plan is
+  // (re)built when dt changes, carrying the current fields over
+  void ensure_plan(double dt) {
+    if (plan && dt == eq_dt) return;
+    if (plan && host_stale) system(0);
+    sps_fdm_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.dim = (int)D;
+    d.dtype = solver.dtype;
+    for (size_t a = 0; a < 3; ++a) d.n[a] = a < D ? dims[a] : 1;
+    d.lap_points = Sp::lap_points;
+    d.gradlap_points = Sp::gradlap_points;
+    d.bilap_points = Sp::bilap_points;
+    d.nfields = (int)F;
+    d.path = solver.path;
+    d.h = solver.h;
+    d.dt = dt;
+    for (size_t f = 0; f < F; ++f)
+      d.nterms[f] = lower_mterms(rhs[f], d.terms[f], SPS_MAX_MTERMS,
+                                 solver.coeffs);
+    if (plan) sps_fdm_destroy(plan);
+    plan = nullptr;
+    check(sps_fdm_create(&d, &plan), "sps_fdm_create");
+    for (size_t f = 0; f < F; ++f) {
+      if (solver.dtype == SPS_F64) {
+        check(sps_fdm_upload(plan, (int)f, values[f].data(), SPS_LAYOUT_PADDED),
+              "upload");
+      } else {
+        std::vector<float> tmp(values[f].begin(), values[f].end());
+        check(sps_fdm_upload(plan, (int)f, tmp.data(), SPS_LAYOUT_PADDED),
+              "upload");
+      }
+    }
+    host_stale = true;
+    eq_dt = dt;
   }
 };
 

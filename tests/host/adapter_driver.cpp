@@ -25,6 +25,59 @@ static void print_terms(const char* name, const Expr& e,
   std::printf("\n");
 }
 
+static void print_mterms(const char* name, const Expr& e,
+                         const std::vector<double>& coeffs = {}) {
+  sps_fdm_term t[SPS_MAX_MTERMS];
+  int n = lower_mterms(e, t, SPS_MAX_MTERMS, coeffs);
+  std::printf("%s", name);
+  for (int i = 0; i < n; ++i) {
+    std::printf(" | op=%d axis=%d coeff=%.17g pow=", t[i].op, t[i].axis, t[i].coeff);
+    for (int f = 0; f < SPS_MAX_FIELDS; ++f) std::printf("%s%d", f ? "," : "", t[i].pow[f]);
+  }
+  std::printf("\n");
+}
+
+// The right-hand sides of MC, GRAYSCOTT and Turing as written in
+// examples/models/modeldefinitions.h, with c(i) the model coefficients.
+static std::vector<std::pair<std::string, Expr>> coupled_rhs() {
+  auto psi = op(1), rho = op(2);
+  return {
+      {"model_c_psi", -bilap(psi) - lap((c(1) - c(2) * psi * psi) * psi + c(5) * rho * rho)},
+      {"model_c_rho", lap(rho) + (c(3) - c(4) * rho * rho) * rho + 2.0 * c(5) * psi * rho},
+      {"gray_scott_psi", c(1) * lap(psi) - psi * rho * rho + c(3) * (1.0 - psi)},
+      {"gray_scott_rho", c(2) * lap(rho) + psi * rho * rho - (c(3) + c(4)) * rho},
+      {"turing_psi", c(1) * lap(psi) + c(2) * (psi + c(3) * rho - psi * rho * rho -
+                                                 c(4) * psi * rho)},
+      {"turing_rho", lap(rho) + c(2) * (c(5) * rho + c(6) * psi + psi * rho * rho +
+                                          c(4) * psi * rho)},
+  };
+}
+
+static Expr rhs_of(const std::string& name) {
+  for (const auto& [n, e] : coupled_rhs())
+    if (n == name) return e;
+  throw solver_error(SPS_E_INVALID, "no right-hand side " + name);
+}
+
+template <size_t D, typename M>
+static void write_fields(M& model, const std::array<int, D>& dims, size_t nf,
+                         const char* path) {
+  std::vector<double> out;
+  const int B = SPS_BOUNDARY_DEPTH;
+  const int nz = D == 3 ? dims[2] : 1;
+  const int px = dims[0] + 2 * B, py = dims[1] + 2 * B;
+  for (size_t f = 0; f < nf; ++f) {
+    const auto& g = model.system(f);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < dims[1]; ++y)
+        for (int x = 0; x < dims[0]; ++x)
+          out.push_back(g[(x + B) + (size_t)px * ((y + B) + (size_t)py * (D == 3 ? z + B : 0))]);
+  }
+  FILE* f = std::fopen(path, "wb");
+  std::fwrite(out.data(), sizeof(double), out.size(), f);
+  std::fclose(f);
+}
+
 static void print_error(const char* name, Expr (*make)()) {
   try {
     Expr e = make();
@@ -95,6 +148,12 @@ int main(int argc, char** argv) {
     print_terms("mixed_operands", 0.5 * var(2) + psi - 2.0 + lap(var(1) * var(1)));
     print_error("field_times_var", [] { auto p = op(1); return p * var(1); });
     print_error("var3", [] { return var(3); });
+    // coupled fields (examples/models/modeldefinitions.h:74-80,123-136)
+    const std::vector<double> mc = {1.1, 0.8, 0.6, 1.3, 0.4};
+    for (const auto& [name, e] : coupled_rhs())
+      print_mterms(name.c_str(), e, mc);
+    print_error("second_field_single", [] { return op(2); });
+    print_error("field5", [] { return op(5); });
     return 0;
   }
   if (argc < 4 || std::string(argv[1]) != "run") {
@@ -161,6 +220,27 @@ int main(int argc, char** argv) {
         find_solution(model, 0.002, 30);
         write_interior(model.system(), argv[3]);
       }
+    } else if (which == "model_c" || which == "gray_scott" || which == "turing" ||
+               which == "gray_scott_fused") {
+      // coupled fields, FP64: the reference-order path (bit for bit against
+      // the oracle), and Gray-Scott on the fused two-field path
+      const bool fused = which == "gray_scott_fused";
+      const std::string base = fused ? "gray_scott" : which;
+      problem_parameters<2> p;
+      p.dims = {40, 28};
+      p.h = 1.0;
+      std::vector<double> cf;
+      double dt = 0.002;
+      if (base == "model_c") cf = {1.1, 0.8, 0.6, 1.3, 0.4};
+      if (base == "gray_scott") cf = {0.16, 0.08, 0.035, 0.065}, dt = 0.5;
+      if (base == "turing") cf = {1.0, 0.7, 1.2, 0.5, -0.4, 0.9}, dt = 0.01;
+      std::array<std::vector<double>, 2> init = {uniform(40 * 28, 5, -0.5, 0.5),
+                                                 uniform(40 * 28, 6, -0.5, 0.5)};
+      MultiModel<2, SolverFT_MI355X<Stencil2d2h<5, 6, 13>>, 2> model(
+          p, {rhs_of(base + "_psi"), rhs_of(base + "_rho")}, init, SPS_F64,
+          fused ? SPS_PATH_FUSED : SPS_PATH_REFERENCE, cf);
+      find_solution(model, dt, 30);
+      write_fields<2>(model, p.dims, 2, argv[3]);
     } else {
       std::fprintf(stderr, "unknown case %s\n", which.c_str());
       return 2;

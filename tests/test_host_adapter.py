@@ -37,6 +37,10 @@ def _parse(text):
         terms = []
         for t in rest:
             kv = dict(x.split("=", 1) for x in t.split())
+            if "pow" in kv:  # a coupled-field term
+                terms.append((int(kv["op"]), int(kv["axis"]), float(kv["coeff"]),
+                              [int(v) for v in kv["pow"].split(",")]))
+                continue
             terms.append((int(kv["op"]), int(kv["axis"]), float(kv["coeff"]),
                           [float(v) for v in kv["poly"].split(",")], int(kv["operand"])))
         out[name] = terms
@@ -76,6 +80,18 @@ def test_lowering_matches_model_definitions(driver):
     assert got["mixed_operands"] == [(abi.OP_LAP, 0, 1.0, [0, 0, 1, 0, 0, 0, 0], 1),
                                      (abi.OP_ID, 0, 1.0, [-2, 1, 0, 0, 0, 0, 0], 0),
                                      (abi.OP_ID, 0, 1.0, [0, 0.5, 0, 0, 0, 0, 0], 2)]
+    # coupled fields: the right-hand sides of MC, GRAYSCOTT and Turing as
+    # written in the reference lower to models.py's term lists
+    mc = (1.1, 0.8, 0.6, 1.3, 0.4)
+    for name, terms in (("model_c", models.model_c_terms(mc)),
+                        ("gray_scott", models.gray_scott_terms(mc[:4])),
+                        ("turing", models.turing_terms(mc + (1.0,)))):
+        for f, field in enumerate(("psi", "rho")):
+            want = [(t.op, t.axis, t.coeff, list(t.pow) + [0] * (abi.MAX_FIELDS - len(t.pow)))
+                    for t in terms[f]]
+            assert got[f"{name}_{field}"] == want, (name, field)
+    for name in ("second_field_single", "field5"):
+        assert got[name].startswith("error -2"), name
 
 
 def _run(driver, case, tmp_path):
@@ -128,3 +144,33 @@ def test_adapter_bb_provisional_fused(gpu_lib, orc, driver, tmp_path):
     d = models.model_bb((40, 28), h=0.9, dt=0.002, a=1.1, b=0.4)
     want = orc.unpad(d, orc.run(d, inputs.uniform((28, 40), seed=4, lo=-0.5, hi=0.5), 30, 8))
     assert np.linalg.norm(got - want) / np.linalg.norm(want) <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["model_c", "gray_scott", "turing"])
+def test_adapter_coupled_bitwise(gpu_lib, orc, driver, tmp_path, case):
+    """MultiModel on the reference-order path against the oracle, bit for
+    bit, from the right-hand sides as written in the reference."""
+    got = _run(driver, case, tmp_path).reshape(2, 28, 40)
+    cf, dt = {"model_c": ((1.1, 0.8, 0.6, 1.3, 0.4), 0.002),
+              "gray_scott": ((0.16, 0.08, 0.035, 0.065), 0.5),
+              "turing": ((1.0, 0.7, 1.2, 0.5, -0.4, 0.9), 0.01)}[case]
+    d = getattr(models, case)((40, 28), h=1.0, dt=dt, c=cf, dtype=abi.F64,
+                              path=abi.PATH_REFERENCE)
+    init = [inputs.uniform((28, 40), seed=5, lo=-0.5, hi=0.5),
+            inputs.uniform((28, 40), seed=6, lo=-0.5, hi=0.5)]
+    want = orc.mrun(d, init, 30, 8)
+    for f in range(2):
+        assert np.array_equal(got[f], want[f]), f
+
+
+@pytest.mark.gpu
+def test_adapter_gray_scott_fused(gpu_lib, orc, driver, tmp_path):
+    got = _run(driver, "gray_scott_fused", tmp_path).reshape(2, 28, 40)
+    d = models.gray_scott((40, 28), h=1.0, dt=0.5, c=(0.16, 0.08, 0.035, 0.065),
+                          dtype=abi.F64, path=abi.PATH_FUSED)
+    init = [inputs.uniform((28, 40), seed=5, lo=-0.5, hi=0.5),
+            inputs.uniform((28, 40), seed=6, lo=-0.5, hi=0.5)]
+    want = orc.mrun(d, init, 30, 8)
+    for f in range(2):
+        assert np.linalg.norm(got[f] - want[f]) / np.linalg.norm(want[f]) <= 1e-12
