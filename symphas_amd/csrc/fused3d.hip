@@ -34,8 +34,10 @@ template <int S0, int S1, int S2, typename T, int NI>
 __device__ __forceinline__ T d7(const T (&a)[3][NI], int r) {
   const T ctr = a[S1][r + 1];
   const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+  // each lane move pairs with a register operand, so both fold into
+  // v_add_f32_dpp
   return fmad(T(-6), ctr,
-              ((xl + xr) + (a[S1][r] + a[S1][r + 2])) +
+              ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +
                   (a[S0][r + 1] + a[S2][r + 1]));
 }
 
@@ -196,13 +198,18 @@ int launch_variant(int K, int rows, const T* in, T* out, int X, int Y, int Z,
   if (K == 1) {
     switch (rows) {
       case 4: return launch3s<T, 4, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      // 6 rows also serve the one-step remainder launch of a (2, 6) shape
+      case 6: return launch3s<T, 6, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
       case 8: return launch3s<T, 8, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      case 12: return launch3s<T, 12, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
       case 16: return launch3s<T, 16, R, 1, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
       default: break;
     }
   } else if (K == 2) {
     if (rows == 4) return launch3s<T, 4, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
     if constexpr (sizeof(T) == 4) {
+      // 6 rows: the rings of four radius-one stages fit three waves per SIMD
+      if (rows == 6) return launch3s<T, 6, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
       if (rows == 8) return launch3s<T, 8, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
     }
   }
@@ -225,8 +232,9 @@ int dispatch3(int K, int rows, const T* in, T* out, int X, int Y, int Z, int TZ,
 }  // namespace
 
 bool fused3d_shape_ok(int dtype_size, int K, int rows) {
-  if (K == 1) return rows == 4 || rows == 8 || rows == 16;
-  if (K == 2) return rows == 4 || (rows == 8 && dtype_size == 4);
+  if (K == 1)
+    return rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16;
+  if (K == 2) return rows == 4 || ((rows == 6 || rows == 8) && dtype_size == 4);
   return false;
 }
 

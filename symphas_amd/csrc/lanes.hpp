@@ -15,20 +15,22 @@ constexpr int kDppWaveRol1 = 0x134;
 constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveRor1 = 0x13C;
 
+// bound_ctrl: a lane whose source lies outside the wave reads 0, so no "old"
+// value has to be set up before the move
 template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
   return __builtin_bit_cast(
       float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
-                                         0xF, 0xF, false));
+                                         0xF, 0xF, true));
 }
 
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
   const long long b = __builtin_bit_cast(long long, v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL,
-                                             0xF, 0xF, false);
+                                             0xF, 0xF, true);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF,
-                                             0xF, false);
+                                             0xF, true);
   return __builtin_bit_cast(
       double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }

@@ -131,9 +131,10 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
         planes.push_back(z);
     }
     for (int k : {1, 2})
-      for (int r : {4, 8, 16})
+      for (int r : {4, 6, 8, 12, 16})
         for (int z : planes)
-          if ((k == 1 && r > 4) || (k == 2 && sps::fused3d_shape_ok((int)p->esize, k, r)))
+          if ((k == 1 && r > 4 && sps::fused3d_shape_ok((int)p->esize, k, r)) ||
+              (k == 2 && sps::fused3d_shape_ok((int)p->esize, k, r)))
             cands.push_back({k, r, z, 1});
   }
   const size_t bytes = p->nsites * p->esize;

@@ -260,7 +260,7 @@ def test_3d_chunking_and_fused_steps_are_bitwise(gpu_lib, dtype, model):
     psi = field(desc, seed=12)
     say("3D z-chunking, rows and fused steps", model, dtype)
     base = gpu_run(desc, psi, 11, K=1, TY=8, TZ=32)
-    shapes = [(1, 4), (1, 8), (1, 16), (2, 4)] + ([(2, 8)] if dtype == F32 else [])
+    shapes = [(1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4)] + ([(2, 6), (2, 8)] if dtype == F32 else [])
     for K, rows in shapes:
         for TZ in (1, 4, 8, 20, 64):
             got = gpu_run(desc, psi, 11, K=K, TY=rows, TZ=TZ)
@@ -410,8 +410,8 @@ def test_state_and_argument_errors(gpu_lib):
         assert lib.sps_fd_apply(p._h, abi.OP_ID, 0, 0, out.ctypes.data_as(ctypes.c_void_p), 0) == abi.E_INVALID
     d3 = models.model_b((16, 16, 16), dtype=F64, stencil=Stencil3d2h(7, 10, 25), path=FUSED)
     with FDPlan(d3) as p:
-        for K, rows, planes, vec in ((2, 16, 0, 0), (2, 8, 0, 0), (3, 4, 0, 0), (1, 12, 0, 0),
-                                     (1, 8, 2000, 0), (1, 8, 0, 2)):
+        for K, rows, planes, vec in ((2, 16, 0, 0), (2, 8, 0, 0), (2, 6, 0, 0), (3, 4, 0, 0),
+                                     (1, 10, 0, 0), (1, 8, 2000, 0), (1, 8, 0, 2)):
             with pytest.raises(abi.SpsError) as ei:
                 p.tune(K, rows, planes, vec)
             assert ei.value.status == abi.E_INVALID, (K, rows, planes, vec)
@@ -462,7 +462,7 @@ def test_autotune_3d(gpu_lib):
     with FDPlan(desc) as p:
         p.upload(psi)
         k, rows, planes, vec = p.autotune(use_cache=False)
-        assert (k, rows) in ((1, 4), (1, 8), (1, 16), (2, 4), (2, 8))
+        assert (k, rows) in ((1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4), (2, 6), (2, 8))
         assert planes in (8, 16, 32, 40) and vec == 1
         assert np.array_equal(p.download(), psi)
         p.step(6)
