@@ -127,8 +127,7 @@ __global__ __launch_bounds__(256) void fused3d_stream(
       // fill the ring with garbage that no valid output reads
       if (i < 2 * (S - 1)) return;
     }
-    const int zo = z0 + i - 2 * NS;
-    T* dst = out + (int64_t)zo * XY;
+    T res[NO];
 #pragma unroll
     for (int r = 0; r < NO; ++r) {
       const T d = d7<S0, S1, S2>(a, r);
@@ -144,10 +143,24 @@ __global__ __launch_bounds__(256) void fused3d_stream(
         if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
         v = fmad(R == 2 ? c.kd : c.k1, d, base);
       }
-      if constexpr (LAST) {
-        if (store_lane && y0 + r < Y) (dst + rowoff[r + RT])[col] = v;
-      } else {
+      if constexpr (LAST)
+        res[r] = v;
+      else
         o[S2][r] = v;
+    }
+    if constexpr (LAST) {
+      // every row computed by all lanes first, then one masked region of
+      // stores; the row checks are wave-uniform
+      if (store_lane) {
+        T* dst = out + (int64_t)(z0 + i - 2 * NS) * XY + col;
+        if (y0 + NO <= Y) {
+#pragma unroll
+          for (int r = 0; r < NO; ++r) dst[rowoff[r + RT]] = res[r];
+        } else {
+#pragma unroll
+          for (int r = 0; r < NO; ++r)
+            if (y0 + r < Y) dst[rowoff[r + RT]] = res[r];
+        }
       }
     }
   };
