@@ -306,9 +306,10 @@ int sps_fd_autotune(sps_fd_plan* plan, int use_cache, int* steps_per_launch,
 /* ---------------------------------------------------------------- batched */
 
 /* Many independent 2D fields of uneven sizes and coefficients, advanced by one
- * persistent launch per step, captured once in a hipGraph and replayed.
- * Every descriptor must be 2D, share dtype, lower to the fused form and have
- * n[0] = n[1] = 64*v for v in {2,3,4,6,8}.
+ * launch per step group (one wave per strip x row chunk of any instance),
+ * captured once in a hipGraph and replayed.
+ * Every descriptor must be 2D FP32, lower to the fused form (polynomials of
+ * degree <= 3) and have an even n[0]; extents may differ per instance.
  * Replaces: find_solution over an array of models, which steps each model in
  * turn (modules-io.h:368-474, loop at :462-465). */
 int sps_fd_batch_create(const sps_fd_desc* descs, int count,

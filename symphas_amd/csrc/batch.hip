@@ -1,18 +1,15 @@
-// batch.hip - many independent 2D fields in one persistent launch per step
-// group (the batched counterpart of find_solution over an array of models,
+// batch.hip - many independent 2D fields in one launch per step group (the
+// batched counterpart of find_solution over an array of models,
 // modules-io.h:368-474, which steps the models one after another).
 //
 // Every instance is stored unpadded in one device pool with two buffers
-// (ping-pong shared by all instances).  A work item is a chunk of rows of one
-// instance; items are sorted largest first and handed out through a device
-// counter, so waves that draw small instances take more items.  Each wave
-// holds whole rows (width 64*v, v in {2,3,4,6,8}): the x wrap is the DPP
-// rotate and every lane stores.  Per-instance coefficients (mobility,
-// gradient coefficient, dt, h, model) live in a descriptor table read with
-// scalar loads.  The counter of launch parity p is used by that launch, which
-// clears the other parity's counter for the next launch; kernels on one stream
-// run in order, so the cleared counter is never in use.  Captured once in a
-// hipGraph per parity and replayed.
+// (ping-pong shared by all instances).  A work item is one strip (64 lanes x 2
+// values, halo lanes at both sides, as the single-field kernel) by a chunk of
+// rows of one instance, so every instance runs the same register-light stream
+// whatever its width, and one wave takes one item.  Per-instance coefficients
+// (mobility, gradient coefficient, dt, h, model) live in a descriptor table
+// read with scalar loads.  Captured once in a hipGraph per parity and
+// replayed.
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -27,63 +24,63 @@ namespace {
 
 struct BatchInst {
   float* buf[2];
-  int X, Y, vec, kind;  // kind: 0 radius 1 with A, 1 radius 2 without A,
+  int X, Y, pad, kind;  // kind: 0 radius 1 with A, 1 radius 2 without A,
                         //       2 radius 2 with A, 3 radius 1 without A
   double h, divh;
   FusedCoef<float> c;
 };
 
 struct BatchItem {
-  int inst, y0, ty, pad;
+  int inst, y0, ty, strip;
 };
 
-template <int VEC, int K>
-__device__ __forceinline__ void run_item(const BatchInst& I, int parity,
-                                         int y0, int ty, int lane) {
-  const float* in = I.buf[parity];
-  float* out = I.buf[1 - parity];
-  const FusedCoef<float> c = I.c;
-  switch (I.kind) {
-    case 0:
-      stream_tile<float, VEC, K, 1, true, 3, true>(in, out, 64 * VEC, I.Y, y0,
-                                                  ty, lane * VEC, true, c);
-      break;
-    case 1:
-      stream_tile<float, VEC, K, 2, false, 3, true>(in, out, 64 * VEC, I.Y,
-                                                   y0, ty, lane * VEC, true, c);
-      break;
-    case 2:
-      stream_tile<float, VEC, K, 2, true, 3, true>(in, out, 64 * VEC, I.Y, y0,
-                                                  ty, lane * VEC, true, c);
-      break;
-    default:
-      stream_tile<float, VEC, K, 1, false, 3, true>(in, out, 64 * VEC, I.Y,
-                                                   y0, ty, lane * VEC, true, c);
-      break;
-  }
+// Strip geometry of the 2-values-per-lane stream (fused_tile.hpp): a wave
+// covers (64 - 2*HL)*2 columns and HL halo lanes per side carry the K*R
+// columns the K fused steps reach.
+constexpr int kBatchVec = 2;
+__host__ __device__ constexpr int batch_halo_lanes(int K, int R) {
+  return (K * R + kBatchVec - 1) / kBatchVec;
+}
+__host__ __device__ constexpr int batch_strip_width(int K, int R) {
+  return (64 - 2 * batch_halo_lanes(K, R)) * kBatchVec;
 }
 
+template <int K, int R, bool HASA>
+__device__ __forceinline__ void run_strip(const BatchInst& I, int parity,
+                                          const BatchItem& it, int lane) {
+  constexpr int HL = batch_halo_lanes(K, R);
+  constexpr int W = batch_strip_width(K, R);
+  // the descriptor is read once into registers: the stream's stores could
+  // alias it, which would otherwise force a reload per row
+  const FusedCoef<float> c = I.c;
+  const int X = I.X, Y = I.Y;
+  const float* in = I.buf[parity];
+  float* out = I.buf[1 - parity];
+  const int cb = it.strip * W + (lane - HL) * kBatchVec;
+  int col = cb % X;
+  if (col < 0) col += X;
+  const bool store_lane = lane >= HL && lane < 64 - HL && cb < X;
+  stream_tile<float, kBatchVec, K, R, HASA, 3, false>(in, out, X, Y, it.y0,
+                                                      it.ty, col, store_lane, c);
+}
+
+// One wave per item (strip x row chunk); items are near-uniform in cost, so
+// the hardware's dispatch balances them.  Consecutive blocks go to one XCD.
 template <int K>
 __global__ __launch_bounds__(256) void batch_kernel(
     const BatchInst* __restrict__ insts, const BatchItem* __restrict__ items,
-    int nitems, int parity, unsigned* __restrict__ counters) {
+    int nitems, int parity) {
   const int lane = threadIdx.x & 63;
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[1 - parity] = 0u;
-  unsigned* ctr = counters + parity;
-  while (true) {
-    unsigned it = 0;
-    if (lane == 0) it = atomicAdd(ctr, 1u);
-    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0, 64));
-    if (it >= (unsigned)nitems) break;  // every wave reaches this
-    const BatchItem item = items[it];
-    const BatchInst& I = insts[item.inst];
-    switch (I.vec) {
-      case 2: run_item<2, K>(I, parity, item.y0, item.ty, lane); break;
-      case 3: run_item<3, K>(I, parity, item.y0, item.ty, lane); break;
-      case 4: run_item<4, K>(I, parity, item.y0, item.ty, lane); break;
-      case 6: run_item<6, K>(I, parity, item.y0, item.ty, lane); break;
-      default: run_item<8, K>(I, parity, item.y0, item.ty, lane); break;
-    }
+  const int it = __builtin_amdgcn_readfirstlane(
+      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (it >= nitems) return;
+  const BatchItem item = items[it];
+  const BatchInst& I = insts[item.inst];
+  switch (I.kind) {
+    case 0: run_strip<K, 1, true>(I, parity, item, lane); break;
+    case 1: run_strip<K, 2, false>(I, parity, item, lane); break;
+    case 2: run_strip<K, 2, true>(I, parity, item, lane); break;
+    default: run_strip<K, 1, false>(I, parity, item, lane); break;
   }
 }
 
@@ -145,13 +142,11 @@ struct sps_fd_batch {
   hipStream_t stream = nullptr;
   float* pool = nullptr;  // 2 * total floats
   BatchInst* d_insts = nullptr;
-  BatchItem* d_items = nullptr;
-  unsigned* d_counters = nullptr;
+  BatchItem* d_items[4] = {};  // item table per steps-per-launch (strip widths differ)
   double* d_red = nullptr;
-  int nitems = 0;
+  int nitems[4] = {};
   int parity = 0;
   int K = 4, TY = 32;  // best of a 1..4 x 16..128 sweep on 256 instances
-  int grid[4] = {1, 1, 1, 1};  // persistent grid per steps-per-launch
   bool has_data = false;
   GraphSlot graphs[2];
   std::vector<BatchInst> h_insts;
@@ -165,34 +160,40 @@ struct sps_fd_batch {
   }
 
   int build_items() {
-    std::vector<BatchItem> items;
-    for (int i = 0; i < count; ++i) {
-      const int Y = descs[i].n[1];
-      for (int y0 = 0; y0 < Y; y0 += TY)
-        items.push_back(BatchItem{i, y0, std::min(TY, Y - y0), 0});
+    for (int k = 1; k <= 4; ++k) {
+      std::vector<BatchItem> items;
+      for (int i = 0; i < count; ++i) {
+        const int R = (h_insts[i].kind == 1 || h_insts[i].kind == 2) ? 2 : 1;
+        const int W = batch_strip_width(k, R);
+        const int X = descs[i].n[0], Y = descs[i].n[1];
+        for (int st = 0; st * W < X; ++st)
+          for (int y0 = 0; y0 < Y; y0 += TY)
+            items.push_back(BatchItem{i, y0, std::min(TY, Y - y0), st});
+      }
+      // largest first: cost ~ streamed rows (every strip is equally wide)
+      std::stable_sort(items.begin(), items.end(),
+                       [](const BatchItem& a, const BatchItem& b) {
+                         return a.ty > b.ty;
+                       });
+      if (d_items[k - 1]) hipFree(d_items[k - 1]);
+      d_items[k - 1] = nullptr;
+      nitems[k - 1] = (int)items.size();
+      SPS_HIP_TRY(hipMalloc(&d_items[k - 1], sizeof(BatchItem) * items.size()));
+      SPS_HIP_TRY(hipMemcpy(d_items[k - 1], items.data(),
+                            sizeof(BatchItem) * items.size(),
+                            hipMemcpyHostToDevice));
     }
-    // largest first: cost ~ row width x streamed rows
-    std::stable_sort(items.begin(), items.end(),
-                     [&](const BatchItem& a, const BatchItem& b) {
-                       const double ca = (double)descs[a.inst].n[0] * a.ty;
-                       const double cb = (double)descs[b.inst].n[0] * b.ty;
-                       return ca > cb;
-                     });
-    if (d_items) hipFree(d_items);
-    d_items = nullptr;
-    nitems = (int)items.size();
-    SPS_HIP_TRY(hipMalloc(&d_items, sizeof(BatchItem) * items.size()));
-    SPS_HIP_TRY(hipMemcpy(d_items, items.data(), sizeof(BatchItem) * items.size(),
-                          hipMemcpyHostToDevice));
     return SPS_OK;
   }
 
+  unsigned blocks(int k) const { return (unsigned)((nitems[k] + 3) / 4); }
+
   int launch_one(int k) {
     switch (k) {
-      case 1: batch_kernel<1><<<grid[0], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
-      case 2: batch_kernel<2><<<grid[1], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
-      case 3: batch_kernel<3><<<grid[2], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
-      default: batch_kernel<4><<<grid[3], 256, 0, stream>>>(d_insts, d_items, nitems, parity, d_counters); break;
+      case 1: batch_kernel<1><<<blocks(0), 256, 0, stream>>>(d_insts, d_items[0], nitems[0], parity); break;
+      case 2: batch_kernel<2><<<blocks(1), 256, 0, stream>>>(d_insts, d_items[1], nitems[1], parity); break;
+      case 3: batch_kernel<3><<<blocks(2), 256, 0, stream>>>(d_insts, d_items[2], nitems[2], parity); break;
+      default: batch_kernel<4><<<blocks(3), 256, 0, stream>>>(d_insts, d_items[3], nitems[3], parity); break;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
@@ -246,9 +247,7 @@ struct sps_fd_batch {
   }
 };
 
-static int batch_width_ok(int n) {
-  return n == 128 || n == 192 || n == 256 || n == 384 || n == 512;
-}
+static int batch_width_ok(int n) { return n >= 2 && n % kBatchVec == 0; }
 
 extern "C" {
 
@@ -269,7 +268,7 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
       return SPS_E_UNSUPPORTED;
     }
     if (!batch_width_ok(d.n[0])) {
-      set_error("batch instance %d: x extent %d not in {128,192,256,384,512}",
+      set_error("batch instance %d: x extent %d is odd (lanes hold 2 values)",
                 i, d.n[0]);
       return SPS_E_UNSUPPORTED;
     }
@@ -324,7 +323,7 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
     I.buf[1] = b->pool + b->total + b->offset[i];
     I.X = descs[i].n[0];
     I.Y = descs[i].n[1];
-    I.vec = I.X / 64;
+    I.pad = 0;
     I.h = descs[i].h;
     I.divh = 1.0 / descs[i].h;
     make_fused_coef(descs[i], lower(descs[i]), I.c);
@@ -333,31 +332,12 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
   if (hipMalloc(&b->d_insts, sizeof(BatchInst) * count) != hipSuccess ||
       hipMemcpy(b->d_insts, b->h_insts.data(), sizeof(BatchInst) * count,
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMalloc(&b->d_counters, 2 * sizeof(unsigned)) != hipSuccess ||
-      hipMemset(b->d_counters, 0, 2 * sizeof(unsigned)) != hipSuccess ||
       hipMalloc(&b->d_red, 2 * sizeof(double) * count) != hipSuccess) {
     set_error("batch device set-up failed");
     return fail(SPS_E_HIP);
   }
   int rc = b->build_items();
   if (rc != SPS_OK) return fail(rc);
-  // persistent grid: as many workgroups as fit on the device at once
-  int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-          hipSuccess) {
-    set_error("device query failed");
-    return fail(SPS_E_HIP);
-  }
-  const void* kernels[4] = {(const void*)batch_kernel<1>, (const void*)batch_kernel<2>,
-                            (const void*)batch_kernel<3>, (const void*)batch_kernel<4>};
-  for (int k = 0; k < 4; ++k) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[k], 256,
-                                                     0) != hipSuccess)
-      per_cu = 1;
-    b->grid[k] = std::max(1, cus * std::max(1, per_cu));
-  }
   *out = b;
   return SPS_OK;
 }
@@ -368,8 +348,8 @@ int sps_fd_batch_destroy(sps_fd_batch* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->pool) hipFree(b->pool);
   if (b->d_insts) hipFree(b->d_insts);
-  if (b->d_items) hipFree(b->d_items);
-  if (b->d_counters) hipFree(b->d_counters);
+  for (BatchItem* p : b->d_items)
+    if (p) hipFree(p);
   if (b->d_red) hipFree(b->d_red);
   if (b->own_stream) hipStreamDestroy(b->own_stream);
   delete b;

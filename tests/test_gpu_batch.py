@@ -84,8 +84,28 @@ def test_batch_against_oracle(gpu_lib, orc):
         assert err <= 1e-5, err
 
 
+def test_batch_uneven_extents_bitwise(gpu_lib):
+    """Widths that are not multiples of the strip width, rectangular
+    instances and a width narrower than one strip, against single-field
+    runs bit for bit."""
+    shapes = [(70, 50), (130, 90), (8, 12), (250, 34), (514, 40)]
+    descs, fields = [], []
+    for i, (x, y) in enumerate(shapes):
+        mk = models.model_b if i % 2 == 0 else models.allen_cahn
+        descs.append(mk((x, y), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED))
+        fields.append(inputs.uniform((y, x), seed=30 + i, dtype=np.float32))
+    print("[case] batch uneven extents", shapes, flush=True)
+    with FDBatch(descs) as b:
+        for i, f in enumerate(fields):
+            b.upload(i, f)
+        b.step(37)
+        got = [b.download(i) for i in range(len(descs))]
+    for d, f, g in zip(descs, fields, got):
+        assert np.array_equal(g, _single(d, f, 37)), (d.n[0], d.n[1])
+
+
 def test_batch_rejects_unsupported(gpu_lib):
-    d = models.model_b((100, 128), dtype=abi.F32)
+    d = models.model_b((101, 128), dtype=abi.F32)
     with pytest.raises(abi.SpsError) as ei:
         FDBatch([d])
     assert ei.value.status == abi.E_UNSUPPORTED
