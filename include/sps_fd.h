@@ -306,8 +306,8 @@ int sps_fd_autotune(sps_fd_plan* plan, int use_cache, int* steps_per_launch,
 /* ---------------------------------------------------------------- batched */
 
 /* Many independent 2D fields of uneven sizes and coefficients, advanced by one
- * launch per step group (one wave per strip x row chunk of any instance),
- * captured once in a hipGraph and replayed.
+ * persistent launch per step group (resident waves walk the strip x row-chunk
+ * items of every instance), captured once in a hipGraph and replayed.
  * Every descriptor must be 2D FP32, lower to the fused form (polynomials of
  * degree <= 3) and have an even n[0]; extents may differ per instance.
  * Replaces: find_solution over an array of models, which steps each model in
