@@ -119,17 +119,39 @@ def extra_cfg1(torch, stream):
     from symphas_amd import abi, inputs, models
     from symphas_amd.solver import FDPlan
     d = models.allen_cahn((64, 64), h=0.5, dt=0.01, dtype=abi.F64, path=abi.PATH_REFERENCE)
+    psi0 = inputs.uniform((64, 64), seed=1)
     with FDPlan(d) as p:
         p.set_stream(stream.cuda_stream)
-        p.upload(inputs.uniform((64, 64), seed=1))
+        p.upload(psi0)
         p.step(100)
         p.sync()
         a, b = _events(torch)
         a.record(stream); p.step(1000); b.record(stream); stream.synchronize()
         ms = a.elapsed_time(b)
+        p.upload(psi0)
+        p.step(1000)
+        ref = p.download()
+    # the fused FP64 kernel against the reference-order one (which the tests
+    # hold bit-identical to the CPU SolverFT restatement) after 1000 steps
+    df = models.allen_cahn((64, 64), h=0.5, dt=0.01, dtype=abi.F64, path=abi.PATH_FUSED)
+    with FDPlan(df) as p:
+        p.set_stream(stream.cuda_stream)
+        p.upload(psi0)
+        p.step(1000)
+        fused = p.download()
     return {"workload": "Allen-Cahn 64^2 FP64 reference-order kernel, 1000 steps",
             "ms_total": ms, "Gsite_updates_per_s": 64 * 64 * 1000 / (ms / 1e3) / 1e9,
-            "note": "launch-bound (4096 sites per launch); graph-replayed"}
+            "error_vs_fp64_reference_order": _errors(fused, ref),
+            "note": "launch-bound (4096 sites per launch); graph-replayed; the reference-order "
+                    "kernel equals the CPU oracle bit for bit (tests/test_gpu_parity.py)"}
+
+
+def _errors(got, want):
+    import numpy as np
+    g = np.asarray(got, dtype=np.float64)
+    w = np.asarray(want, dtype=np.float64)
+    return {"max_abs_err": float(np.max(np.abs(g - w))),
+            "rel_l2_err": float(np.linalg.norm((g - w).ravel()) / np.linalg.norm(w.ravel()))}
 
 
 def extra_cfg3(torch, stream, steps=200):
@@ -152,13 +174,26 @@ def extra_cfg3(torch, stream, steps=200):
         a.record(stream); p.step(steps); b.record(stream); stream.synchronize()
         ms = a.elapsed_time(b)
         m1, _ = p.reduce()
+        got = p.download()
+    # the same steps in FP64 from the same FP32-rounded field
+    d64 = models.model_b((n, n, n), h=1.0, dt=0.005, stencil=models.Stencil3d2h(7, 10, 25),
+                         dtype=abi.F64, path=abi.PATH_FUSED)
+    with FDPlan(d64) as p:
+        p.set_stream(stream.cuda_stream)
+        p.upload(psi.astype(np.float64))
+        p.step(steps)
+        want = p.download()
+    err = _errors(got, want)
+    del got, want
     g = n ** 3 * steps / (ms / 1e3) / 1e9
     return {"workload": "Model B 3D 512^3 FP32, Stencil3d2h<7,10,25>, dt=0.005",
             "steps": steps, "Gsite_updates_per_s": g,
             "launch_shape": {"steps_per_launch": shape[0], "rows_per_wave": shape[1],
                              "planes_per_chunk": shape[2]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
-            "mass_drift_rel": abs(m1 - m0) / float(np.abs(psi.astype(np.float64)).sum())}
+            "mass_drift_rel": abs(m1 - m0) / float(np.abs(psi.astype(np.float64)).sum()),
+            "accuracy_vs_fp64": {"reference": "FP64 GPU run of the same steps from the same "
+                                              "FP32-rounded field", "steps": steps, **err}}
 
 
 def extra_cfg4(torch, stream, steps=480):
@@ -167,7 +202,7 @@ def extra_cfg4(torch, stream, steps=480):
     step group, hipGraph-replayed; against the same instances stepped one
     after another (the reference's find_solution pattern)."""
     import numpy as np
-    from symphas_amd import inputs, models
+    from symphas_amd import abi, inputs, models
     from symphas_amd.solver import FDBatch, FDPlan
     inst = models.batched_instances(256, seed=7)
     sites = sum(d.n[0] * d.n[1] for d, _ in inst)
@@ -182,7 +217,19 @@ def extra_cfg4(torch, stream, steps=480):
         a, b = _events(torch)
         a.record(stream); bt.step(steps); b.record(stream); stream.synchronize()
         ms = a.elapsed_time(b)
+        got = [bt.download(i) for i in range(len(inst))]
     g = sites * steps / (ms / 1e3) / 1e9
+    # every instance in FP64 (single plans) for the same 48 + steps steps
+    worst = {"max_abs_err": 0.0, "rel_l2_err": 0.0}
+    for (d, _), f, gf in zip(inst, fields, got):
+        d64 = type(d).from_buffer_copy(d)  # same descriptor, FP64 storage
+        d64.dtype = abi.F64
+        with FDPlan(d64) as pl:
+            pl.set_stream(stream.cuda_stream)
+            pl.upload(f.astype(np.float64))
+            pl.step(48 + steps)
+            e = _errors(gf, pl.download())
+        worst = {k: max(worst[k], e[k]) for k in worst}
     plans = []
     for (d, _), f in zip(inst, fields):
         pl = FDPlan(d)
@@ -207,7 +254,10 @@ def extra_cfg4(torch, stream, steps=480):
             "launch_shape": {"steps_per_launch": bshape[0], "rows_per_item": bshape[1]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
             "sequential_plans_Gsite_updates_per_s": seq,
-            "speedup_vs_sequential": g / seq}
+            "speedup_vs_sequential": g / seq,
+            "accuracy_vs_fp64": {"reference": "each instance as an FP64 single-field GPU run of "
+                                              "the same steps from the same FP32 field",
+                                 "steps": 48 + steps, "worst_instance": worst}}
 
 
 def extra_cfg5(torch, stream, iters):
