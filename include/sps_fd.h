@@ -402,6 +402,24 @@ int sps_fdm_tune(sps_fdm_plan* plan, int steps_per_launch, int rows_per_tile,
 int sps_fdm_autotune(sps_fdm_plan* plan, int* steps_per_launch,
                      int* rows_per_tile, int* values_per_lane);
 int sps_fdm_sync(sps_fdm_plan* plan);
+/* The data files of field `field` (0-based; the reference writes one file per
+ * field of a multi-field model, its id being the field's index:
+ * OUTPUT_CHECKPOINT_FMT, io/inc/io.h:185-188), with the formats and behaviour
+ * of sps_fd_checkpoint_text / _read, sps_fd_plot_text and
+ * sps_fd_checkpoint_xdr / _read_xdr. */
+int sps_fdm_checkpoint_text(sps_fdm_plan* plan, int field, int index,
+                            const double* origin, char* text, size_t capacity,
+                            size_t* length);
+int sps_fdm_checkpoint_read(sps_fdm_plan* plan, int field, const char* text,
+                            size_t length, int* index);
+int sps_fdm_plot_text(sps_fdm_plan* plan, int field, int format,
+                      const double* origin, char* text, size_t capacity,
+                      size_t* length);
+int sps_fdm_checkpoint_xdr(sps_fdm_plan* plan, int field, int index,
+                           const double* origin, void* data, size_t capacity,
+                           size_t* length);
+int sps_fdm_checkpoint_read_xdr(sps_fdm_plan* plan, int field,
+                                const void* data, size_t length, int* index);
 /* Caller-owned hipStream_t for later work (NULL: the HIP null stream). */
 int sps_fdm_set_stream(sps_fdm_plan* plan, void* hip_stream);
 

@@ -114,6 +114,11 @@ def load() -> ctypes.CDLL:
         "sps_fdm_sync": (c_int, [P]),
         "sps_fdm_tune": (c_int, [P, c_int, c_int, c_int]),
         "sps_fdm_autotune": (c_int, [P, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "sps_fdm_checkpoint_text": (c_int, [P, c_int, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sps_fdm_checkpoint_read": (c_int, [P, c_int, ctypes.c_char_p, c_size_t, POINTER(c_int)]),
+        "sps_fdm_plot_text": (c_int, [P, c_int, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sps_fdm_checkpoint_xdr": (c_int, [P, c_int, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "sps_fdm_checkpoint_read_xdr": (c_int, [P, c_int, c_void_p, c_size_t, POINTER(c_int)]),
         "sps_fdm_set_stream": (c_int, [P, P]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
         "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),

@@ -15,6 +15,7 @@
 #include "lanes.hpp"
 #include "multi.hpp"
 #include "plan.hpp"
+#include "textio.hpp"
 
 namespace sps {
 namespace {
@@ -769,6 +770,78 @@ int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile,
   if (steps_per_launch) p->K = steps_per_launch;
   if (rows_per_tile) p->TY = rows_per_tile;
   if (values_per_lane) p->VEC = values_per_lane;
+  return SPS_OK;
+}
+
+}  // extern "C"
+
+// field f of a coupled plan as a file-writer view (dim, n, h, dtype)
+static int field_view(sps_fdm_plan* p, int field, bool need_data,
+                      FieldView* v) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (field < 0 || field >= p->d.nfields) {
+    set_error("field %d not in [0, %d)", field, p->d.nfields);
+    return SPS_E_INVALID;
+  }
+  if (need_data && !p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  std::memset(&v->d, 0, sizeof(v->d));
+  v->d.dim = p->d.dim;
+  v->d.dtype = p->d.dtype;
+  for (int a = 0; a < 3; ++a) v->d.n[a] = p->d.n[a];
+  v->d.h = p->d.h;
+  v->data = p->buf[field][p->cur];
+  v->nsites = p->nsites;
+  v->stream = p->stream;
+  return SPS_OK;
+}
+
+extern "C" {
+
+int sps_fdm_checkpoint_text(sps_fdm_plan* p, int field, int index,
+                            const double* origin, char* text, size_t capacity,
+                            size_t* length) {
+  FieldView v;
+  SPS_TRY(field_view(p, field, true, &v));
+  return file_checkpoint_text(v, index, origin, text, capacity, length);
+}
+
+int sps_fdm_checkpoint_read(sps_fdm_plan* p, int field, const char* text,
+                            size_t length, int* index) {
+  FieldView v;
+  SPS_TRY(field_view(p, field, false, &v));
+  SPS_TRY(file_checkpoint_read(v, text, length, index));
+  p->has_data = true;
+  return SPS_OK;
+}
+
+int sps_fdm_plot_text(sps_fdm_plan* p, int field, int format,
+                      const double* origin, char* text, size_t capacity,
+                      size_t* length) {
+  FieldView v;
+  SPS_TRY(field_view(p, field, true, &v));
+  return file_plot_text(v, format, origin, text, capacity, length);
+}
+
+int sps_fdm_checkpoint_xdr(sps_fdm_plan* p, int field, int index,
+                           const double* origin, void* data, size_t capacity,
+                           size_t* length) {
+  FieldView v;
+  SPS_TRY(field_view(p, field, true, &v));
+  return file_xdr_write(v, index, origin, data, capacity, length);
+}
+
+int sps_fdm_checkpoint_read_xdr(sps_fdm_plan* p, int field, const void* data,
+                                size_t length, int* index) {
+  FieldView v;
+  SPS_TRY(field_view(p, field, false, &v));
+  SPS_TRY(file_xdr_read(v, data, length, index));
+  p->has_data = true;
   return SPS_OK;
 }
 

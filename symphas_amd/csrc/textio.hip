@@ -31,6 +31,7 @@
 #include "common.hpp"
 #include "decimal.hpp"
 #include "plan.hpp"
+#include "textio.hpp"
 
 namespace sps {
 namespace {
@@ -342,12 +343,12 @@ void launch_fmt_write(const T* f, int64_t cnt, int64_t X, const TextLayout& L,
 }
 
 template <typename T>
-int write_text(sps_fd_plan* p, const TextFormat& fm, char* text,
+int write_text(const FieldView& v, const TextFormat& fm, char* text,
                size_t capacity, size_t* length) {
-  hipStream_t st = p->stream;
-  const T* field = (const T*)p->buf[p->cur];
-  const int64_t X = p->d.n[0];
-  const int64_t nsites = (int64_t)p->nsites;
+  hipStream_t st = v.stream;
+  const T* field = (const T*)v.data;
+  const int64_t X = v.d.n[0];
+  const int64_t nsites = (int64_t)v.nsites;
   const int64_t rows_per_chunk = std::max<int64_t>(1, kChunkValues / X);
   const int64_t chunk_cap = std::min<int64_t>(nsites, rows_per_chunk * X);
   const unsigned nb_cap = nblocks(chunk_cap, kValsPerBlock);
@@ -631,9 +632,9 @@ long parse_header(const sps_fd_desc& d, const char* text, size_t length,
 }
 
 template <typename T>
-int read_text(sps_fd_plan* p, const char* body, int64_t m) {
-  hipStream_t st = p->stream;
-  const int64_t nvals = (int64_t)p->nsites;
+int read_text(const FieldView& v, const char* body, int64_t m) {
+  hipStream_t st = v.stream;
+  const int64_t nvals = (int64_t)v.nsites;
   const unsigned nb = nblocks(std::max<int64_t>(m, 1), kBytesPerBlock);
   DevBufs db;
   char* d_s;
@@ -667,7 +668,7 @@ int read_text(sps_fd_plan* p, const char* body, int64_t m) {
   SPS_HIP_TRY(hipMemcpyAsync(d_boff, bo.data(), nb * sizeof(uint64_t),
                              hipMemcpyHostToDevice, st));
   SPS_HIP_TRY(hipMemsetAsync(d_nflag, 0, sizeof(unsigned), st));
-  T* field = (T*)p->buf[p->cur];
+  T* field = (T*)v.data;
   tok_parse<T><<<nb, kThreads, 0, st>>>(d_s, m, d_boff, nvals, field, d_ftok,
                                         d_fpos, d_nflag);
   unsigned nf = 0;
@@ -748,11 +749,11 @@ std::string xdr_header(const sps_fd_desc& d, int index, const double* origin) {
 constexpr int64_t kXdrChunk = int64_t(1) << 25;  // values per staged copy
 
 template <typename T>
-int write_xdr(sps_fd_plan* p, const std::string& header, unsigned char* data,
-              size_t capacity) {
-  hipStream_t st = p->stream;
-  const T* field = (const T*)p->buf[p->cur];
-  const int64_t n = (int64_t)p->nsites;
+int write_xdr(const FieldView& v, const std::string& header,
+              unsigned char* data) {
+  hipStream_t st = v.stream;
+  const T* field = (const T*)v.data;
+  const int64_t n = (int64_t)v.nsites;
   std::memcpy(data, header.data(), header.size());
   DevBufs db;
   uint64_t* d_buf;
@@ -771,15 +772,14 @@ int write_xdr(sps_fd_plan* p, const std::string& header, unsigned char* data,
     set_error("XDR kernels failed: %s", hipGetErrorString(e));
     return SPS_E_HIP;
   }
-  (void)capacity;
   return SPS_OK;
 }
 
 template <typename T>
-int read_xdr(sps_fd_plan* p, const unsigned char* body) {
-  hipStream_t st = p->stream;
-  T* field = (T*)p->buf[p->cur];
-  const int64_t n = (int64_t)p->nsites;
+int read_xdr(const FieldView& v, const unsigned char* body) {
+  hipStream_t st = v.stream;
+  T* field = (T*)v.data;
+  const int64_t n = (int64_t)v.nsites;
   DevBufs db;
   uint64_t* d_buf;
   const int64_t chunk = std::min<int64_t>(n, kXdrChunk);
@@ -801,23 +801,68 @@ int read_xdr(sps_fd_plan* p, const unsigned char* body) {
 }
 
 }  // namespace
-}  // namespace sps
 
-using namespace sps;
-
-extern "C" int sps_fd_checkpoint_xdr(sps_fd_plan* p, int index,
-                                     const double* origin, void* data,
-                                     size_t capacity, size_t* length) {
-  if (!p || !length) {
-    set_error("plan or length pointer is NULL");
+int file_checkpoint_text(const FieldView& v, int index, const double* origin,
+                         char* text, size_t capacity, size_t* length) {
+  if (!length) {
+    set_error("length pointer is NULL");
     return SPS_E_INVALID;
   }
-  if (!p->has_data) {
-    set_error("no field uploaded");
-    return SPS_E_STATE;
+  if (v.nsites > 0xffffffffull) {
+    set_error("checkpoint text: fields above 2^32 sites are not supported");
+    return SPS_E_UNSUPPORTED;
   }
-  const std::string header = xdr_header(p->d, index, origin);
-  const size_t total = header.size() + 8 * p->nsites;
+  TextFormat fm;
+  fm.header = gp_header(v.d, index, origin);
+  return v.d.dtype == SPS_F32 ? write_text<float>(v, fm, text, capacity, length)
+                              : write_text<double>(v, fm, text, capacity, length);
+}
+
+int file_plot_text(const FieldView& v, int format, const double* origin,
+                   char* text, size_t capacity, size_t* length) {
+  if (!length) {
+    set_error("length pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (format != SPS_PLOT_GNU && format != SPS_PLOT_CSV) {
+    set_error("plot text: format %d is neither SPS_PLOT_GNU nor SPS_PLOT_CSV",
+              format);
+    return SPS_E_INVALID;
+  }
+  if (v.nsites > 0xffffffffull) {
+    set_error("plot text: fields above 2^32 sites are not supported");
+    return SPS_E_UNSUPPORTED;
+  }
+  const TextFormat fm = plot_format(v.d, format, origin);
+  return v.d.dtype == SPS_F32 ? write_text<float>(v, fm, text, capacity, length)
+                              : write_text<double>(v, fm, text, capacity, length);
+}
+
+int file_checkpoint_read(const FieldView& v, const char* text, size_t length,
+                         int* index) {
+  if (!text) {
+    set_error("text is NULL");
+    return SPS_E_INVALID;
+  }
+  int idx = 0;
+  const long hl = parse_header(v.d, text, length, &idx);
+  if (hl < 0) return SPS_E_INVALID;
+  const int rc = v.d.dtype == SPS_F32
+                     ? read_text<float>(v, text + hl, (int64_t)length - hl)
+                     : read_text<double>(v, text + hl, (int64_t)length - hl);
+  if (rc != SPS_OK) return rc;
+  if (index) *index = idx;
+  return SPS_OK;
+}
+
+int file_xdr_write(const FieldView& v, int index, const double* origin,
+                   void* data, size_t capacity, size_t* length) {
+  if (!length) {
+    set_error("length pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  const std::string header = xdr_header(v.d, index, origin);
+  const size_t total = header.size() + 8 * v.nsites;
   *length = total;
   if (!data) return SPS_OK;
   if (capacity < total) {
@@ -825,34 +870,33 @@ extern "C" int sps_fd_checkpoint_xdr(sps_fd_plan* p, int index,
               total);
     return SPS_E_INVALID;
   }
-  return p->d.dtype == SPS_F32
-             ? write_xdr<float>(p, header, (unsigned char*)data, capacity)
-             : write_xdr<double>(p, header, (unsigned char*)data, capacity);
+  return v.d.dtype == SPS_F32 ? write_xdr<float>(v, header, (unsigned char*)data)
+                              : write_xdr<double>(v, header, (unsigned char*)data);
 }
 
-extern "C" int sps_fd_checkpoint_read_xdr(sps_fd_plan* p, const void* data,
-                                          size_t length, int* index) {
-  if (!p || !data) {
-    set_error("plan or data is NULL");
+int file_xdr_read(const FieldView& v, const void* data, size_t length,
+                  int* index) {
+  if (!data) {
+    set_error("data is NULL");
     return SPS_E_INVALID;
   }
   // read_header (io/src/readxdr.cpp): counts must match the plan
   const unsigned char* b = (const unsigned char*)data;
   size_t pos = 0;
-  auto get_int = [&](int* v) {
+  auto get_int = [&](int* x) {
     if (pos + 4 > length) return false;
     uint32_t u = 0;
     for (int q = 0; q < 4; ++q) u = (u << 8) | b[pos++];
-    *v = (int)u;
+    *x = (int)u;
     return true;
   };
   int dim = 0, c = 0, idx = 0, dim2 = 0;
-  if (!get_int(&dim) || dim != p->d.dim) {
+  if (!get_int(&dim) || dim != v.d.dim) {
     set_error("XDR file: dimension does not match the plan");
     return SPS_E_INVALID;
   }
   for (int a = 0; a < dim; ++a)
-    if (!get_int(&c) || c != p->d.n[a]) {
+    if (!get_int(&c) || c != v.d.n[a]) {
       set_error("XDR file: count of axis %d does not match the plan", a);
       return SPS_E_INVALID;
     }
@@ -862,80 +906,80 @@ extern "C" int sps_fd_checkpoint_read_xdr(sps_fd_plan* p, const void* data,
     return SPS_E_INVALID;
   }
   pos += 16 * (size_t)dim;  // interval bounds
-  if (pos > length || length - pos < 8 * p->nsites) {
+  if (pos > length || length - pos < 8 * v.nsites) {
     set_error("XDR file: %zu bytes of values for a field of %zu sites",
-              pos > length ? (size_t)0 : length - pos, p->nsites);
+              pos > length ? (size_t)0 : length - pos, v.nsites);
     return SPS_E_INVALID;
   }
-  const int rc = p->d.dtype == SPS_F32 ? read_xdr<float>(p, b + pos)
-                                       : read_xdr<double>(p, b + pos);
+  const int rc = v.d.dtype == SPS_F32 ? read_xdr<float>(v, b + pos)
+                                      : read_xdr<double>(v, b + pos);
   if (rc != SPS_OK) return rc;
-  p->has_data = true;
   if (index) *index = idx;
   return SPS_OK;
 }
 
-extern "C" int sps_fd_checkpoint_text(sps_fd_plan* p, int index,
-                                      const double* origin, char* text,
-                                      size_t capacity, size_t* length) {
-  if (!p || !length) {
-    set_error("plan or length pointer is NULL");
+}  // namespace sps
+
+using namespace sps;
+
+namespace {
+
+FieldView view_of(sps_fd_plan* p) {
+  return FieldView{p->buf[p->cur], p->d, p->nsites, p->stream};
+}
+
+int plan_ready(sps_fd_plan* p) {
+  if (!p) {
+    set_error("plan is NULL");
     return SPS_E_INVALID;
   }
   if (!p->has_data) {
     set_error("no field uploaded");
     return SPS_E_STATE;
   }
-  if (p->nsites > 0xffffffffull) {
-    set_error("checkpoint text: fields above 2^32 sites are not supported");
-    return SPS_E_UNSUPPORTED;
-  }
-  TextFormat fm;
-  fm.header = gp_header(p->d, index, origin);
-  return p->d.dtype == SPS_F32
-             ? write_text<float>(p, fm, text, capacity, length)
-             : write_text<double>(p, fm, text, capacity, length);
+  return SPS_OK;
+}
+
+}  // namespace
+
+extern "C" int sps_fd_checkpoint_text(sps_fd_plan* p, int index,
+                                      const double* origin, char* text,
+                                      size_t capacity, size_t* length) {
+  SPS_TRY(plan_ready(p));
+  return file_checkpoint_text(view_of(p), index, origin, text, capacity, length);
 }
 
 extern "C" int sps_fd_plot_text(sps_fd_plan* p, int format, const double* origin,
                                 char* text, size_t capacity, size_t* length) {
-  if (!p || !length) {
-    set_error("plan or length pointer is NULL");
-    return SPS_E_INVALID;
-  }
-  if (format != SPS_PLOT_GNU && format != SPS_PLOT_CSV) {
-    set_error("plot text: format %d is neither SPS_PLOT_GNU nor SPS_PLOT_CSV",
-              format);
-    return SPS_E_INVALID;
-  }
-  if (!p->has_data) {
-    set_error("no field uploaded");
-    return SPS_E_STATE;
-  }
-  if (p->nsites > 0xffffffffull) {
-    set_error("plot text: fields above 2^32 sites are not supported");
-    return SPS_E_UNSUPPORTED;
-  }
-  const TextFormat fm = plot_format(p->d, format, origin);
-  return p->d.dtype == SPS_F32
-             ? write_text<float>(p, fm, text, capacity, length)
-             : write_text<double>(p, fm, text, capacity, length);
+  SPS_TRY(plan_ready(p));
+  return file_plot_text(view_of(p), format, origin, text, capacity, length);
 }
 
 extern "C" int sps_fd_checkpoint_read(sps_fd_plan* p, const char* text,
                                       size_t length, int* index) {
-  if (!p || !text) {
-    set_error("plan or text is NULL");
+  if (!p) {
+    set_error("plan is NULL");
     return SPS_E_INVALID;
   }
-  int idx = 0;
-  const long hl = parse_header(p->d, text, length, &idx);
-  if (hl < 0) return SPS_E_INVALID;
-  const int rc = p->d.dtype == SPS_F32
-                     ? read_text<float>(p, text + hl, (int64_t)length - hl)
-                     : read_text<double>(p, text + hl, (int64_t)length - hl);
-  if (rc != SPS_OK) return rc;
+  SPS_TRY(file_checkpoint_read(view_of(p), text, length, index));
   p->has_data = true;
-  if (index) *index = idx;
+  return SPS_OK;
+}
+
+extern "C" int sps_fd_checkpoint_xdr(sps_fd_plan* p, int index,
+                                     const double* origin, void* data,
+                                     size_t capacity, size_t* length) {
+  SPS_TRY(plan_ready(p));
+  return file_xdr_write(view_of(p), index, origin, data, capacity, length);
+}
+
+extern "C" int sps_fd_checkpoint_read_xdr(sps_fd_plan* p, const void* data,
+                                          size_t length, int* index) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_TRY(file_xdr_read(view_of(p), data, length, index));
+  p->has_data = true;
   return SPS_OK;
 }

@@ -195,6 +195,21 @@ class FDPlan:
         return k.value, r.value, z.value, v.value
 
 
+def _origin_arg(origin):
+    if origin is None:
+        return None, None
+    arr = np.ascontiguousarray(origin, dtype=np.float64)
+    return arr, arr.ctypes.data_as(c_void_p)
+
+
+def _text_cap(desc, per_value: int, per_row: int, per_slice_header: int = 0) -> int:
+    n = [int(desc.n[a]) for a in range(desc.dim)]
+    sites = int(np.prod(n))
+    rows = sites // n[0]
+    slices = n[2] if desc.dim == 3 else 1
+    return per_value * sites + per_row * rows + slices * per_slice_header + 4096
+
+
 def device_count() -> int:
     return int(abi.load().sps_fd_device_count())
 
@@ -317,6 +332,50 @@ class MultiPlan:
         abi.check(self.lib.sps_fdm_download(self._h, int(field), out.ctypes.data_as(c_void_p), layout))
         return out
 
+
+    def checkpoint_text(self, field: int, index: int = 0, origin=None) -> bytes:
+        """Field `field` as the reference's GNU checkpoint file (device-formatted)."""
+        keep, o = _origin_arg(origin)
+        cap = _text_cap(self.desc, 19, 1)
+        buf = np.empty(cap, dtype=np.uint8)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fdm_checkpoint_text(self._h, int(field), int(index), o,
+                                                   buf.ctypes.data_as(c_void_p), cap, byref(n)))
+        return buf[:n.value].tobytes()
+
+    def read_checkpoint(self, field: int, text: bytes) -> int:
+        idx = c_int(0)
+        abi.check(self.lib.sps_fdm_checkpoint_read(self._h, int(field), text, len(text),
+                                                   byref(idx)))
+        return idx.value
+
+    def plot_text(self, field: int, fmt: int = abi.PLOT_GNU, origin=None) -> bytes:
+        keep, o = _origin_arg(origin)
+        cap = _text_cap(self.desc, 15, 15, 14 * (int(self.desc.n[0]) + 1) + 2)
+        buf = np.empty(cap, dtype=np.uint8)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fdm_plot_text(self._h, int(field), int(fmt), o,
+                                             buf.ctypes.data_as(c_void_p), cap, byref(n)))
+        return buf[:n.value].tobytes()
+
+    def checkpoint_xdr(self, field: int, index: int = 0, origin=None) -> bytes:
+        keep, o = _origin_arg(origin)
+        n = c_size_t(0)
+        abi.check(self.lib.sps_fdm_checkpoint_xdr(self._h, int(field), int(index), o, None, 0,
+                                                  byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        abi.check(self.lib.sps_fdm_checkpoint_xdr(self._h, int(field), int(index), o,
+                                                  buf.ctypes.data_as(c_void_p), buf.size,
+                                                  byref(n)))
+        return buf.tobytes()
+
+    def read_checkpoint_xdr(self, field: int, data: bytes) -> int:
+        idx = c_int(0)
+        arr = np.frombuffer(data, dtype=np.uint8)
+        abi.check(self.lib.sps_fdm_checkpoint_read_xdr(self._h, int(field),
+                                                       arr.ctypes.data_as(c_void_p), arr.size,
+                                                       byref(idx)))
+        return idx.value
     def step(self, nsteps: int = 1):
         abi.check(self.lib.sps_fdm_step(self._h, int(nsteps)))
 

@@ -158,3 +158,43 @@ def test_autotune_keeps_fields(gpu_lib):
         p.step(12)
         for f in range(2):
             assert np.array_equal(p.download(f), want[f])
+
+
+def test_coupled_field_files(gpu_lib, orc):
+    """Each field of a coupled plan written as the reference's checkpoint,
+    plot and XDR files (one file per field), byte-identical to the oracle's
+    restatements; reading the checkpoints back into a fresh plan resumes the
+    run bit for bit."""
+    d = models.model_c((40, 28), h=0.7, dt=0.002, stencil=Stencil2d2h(5, 6, 13), dtype=F32,
+                       path=FUSED, c=(1.1, 0.8, 0.6, 1.3, 0.4))
+    fields = [a.astype(np.float32) for a in fields_for(d, 81)]
+    single = models.make_desc((40, 28), Stencil2d2h(5, 6, 13), 0.7, 0.002, [])
+    say("coupled checkpoint files")
+    with MultiPlan(d) as p:
+        for f, a in enumerate(fields):
+            p.upload(f, a)
+        p.step(10)
+        now = [p.download(f) for f in range(2)]
+        texts = [p.checkpoint_text(f, index=10, origin=[0.5, -1.0]) for f in range(2)]
+        xdrs = [p.checkpoint_xdr(f, index=10) for f in range(2)]
+        plots = [p.plot_text(f, abi.PLOT_GNU) for f in range(2)]
+        csvs = [p.plot_text(f, abi.PLOT_CSV) for f in range(2)]
+        p.step(7)
+        want = [p.download(f) for f in range(2)]
+    for f in range(2):
+        g = orc.pad(single, now[f].astype(np.float64))
+        assert texts[f] == orc.checkpoint_text(single, g, 10, origin=[0.5, -1.0])
+        assert xdrs[f] == orc.xdr_write(single, g, 10)
+        assert plots[f] == orc.plot_text(single, g, abi.PLOT_GNU)
+        assert csvs[f] == orc.plot_text(single, g, abi.PLOT_CSV)
+    with MultiPlan(d) as q:
+        for f in range(2):
+            assert q.read_checkpoint(f, texts[f]) == 10
+        q.step(7)
+        assert all(np.array_equal(q.download(f), want[f]) for f in range(2))
+    with MultiPlan(d) as q:
+        for f in range(2):
+            assert q.read_checkpoint_xdr(f, xdrs[f]) == 10
+        assert all(np.array_equal(q.download(f), now[f]) for f in range(2))
+        with pytest.raises(abi.SpsError):
+            q.checkpoint_text(2)
