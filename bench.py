@@ -52,6 +52,8 @@ def parse():
                          "launch 1..4 x [4 values per lane x rows 8..64, 2 values per lane "
                          "x rows 16..128], 6 launches each, in that order) on the 4096^2 "
                          "Model B field, for per-shape profiles")
+    ap.add_argument("--peak-probe", action="store_true",
+                    help="measure this GPU's copy bandwidth and BF16 GEMM rate, then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device 5 times (the known-size "
                          "transfer that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -68,6 +70,39 @@ def copy_probe():
         y.copy_(x)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": 4 * n, "repeats": 5}), flush=True)
+
+
+def peak_probe():
+    """Measured peaks of this GPU: a 1 GiB device copy (HBM read + write) and a
+    BF16 GEMM through the vendor BLAS (torch.matmul -> hipBLASLt, dense MFMA),
+    each the best of several timed repeats."""
+    import torch
+    torch.cuda.set_device(0)
+    out = {}
+    n = 128 * 1024 * 1024  # 512 MiB of FP32 each way
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    a, b = _events(torch)
+    best = 1e9
+    for _ in range(10):
+        a.record(); y.copy_(x); b.record(); torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    out["copy_GBs"] = 2 * 4 * n / (best / 1e3) / 1e9
+    del x, y
+    m = 8192
+    p = torch.randn(m, m, device="cuda", dtype=torch.bfloat16)
+    q = torch.randn(m, m, device="cuda", dtype=torch.bfloat16)
+    for _ in range(3):
+        torch.matmul(p, q)
+    torch.cuda.synchronize()
+    best = 1e9
+    for _ in range(10):
+        a.record(); torch.matmul(p, q); b.record(); torch.cuda.synchronize()
+        best = min(best, a.elapsed_time(b))
+    out["bf16_gemm_TFLOPs"] = 2 * m ** 3 / (best / 1e3) / 1e12
+    out["note"] = ("1 GiB device copy (read + write counted); 8192^3 BF16 GEMM via torch.matmul "
+                   "(hipBLASLt); vendor peaks: HBM 8 TB/s, dense BF16 2.5 PFLOP/s")
+    print(json.dumps({"peak_probe": out}), flush=True)
 
 
 def shape_sweep():
@@ -515,6 +550,9 @@ def main():
     if args.extra_only:
         extra_only(args)
         return
+    if args.peak_probe:
+        peak_probe()
+        return
     if args.copy_probe:
         copy_probe()
         return
@@ -576,6 +614,7 @@ def main():
     avg_launch_s = ms / 1e3 / launches
     alg_bytes_per_launch = 8.0 * sites * spl
     achieved = alg_bytes_per_launch / avg_launch_s / 1e9
+    traffic = profiled_traffic(shape[0], shape[1], shape[3])
 
     m1, e1 = plan.reduce()
     out32 = plan.download()
@@ -642,10 +681,16 @@ def main():
                    "steps_per_launch": spl, "launch_shape": tuned},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(shape[0], shape[1], shape[3]),
+                     "traffic": traffic,
+                     "dram_bytes_per_site_update": (None if traffic is None
+                                                    else traffic / (sites * spl)),
+                     "dram_GBs": (None if traffic is None
+                                  else round(traffic / avg_launch_s / 1e9, 1)),
                      "note": ("algorithmic 8 B per site-update x sites x steps per launch / "
                               "average launch time; with several steps fused per launch the "
-                              "fraction can exceed 1")},
+                              "fraction can exceed 1. traffic = FETCH_SIZE + WRITE_SIZE bytes "
+                              "per launch of this launch shape (rocprofv3, calibrated on a "
+                              "device copy), dram_GBs = traffic / average launch time")},
         "cpu_baseline": cpu,
         "accuracy_vs_fp64": accuracy,
         "mass_drift_rel": drift,
