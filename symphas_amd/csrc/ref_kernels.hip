@@ -143,14 +143,12 @@ __device__ __forceinline__ int wrap(int c, int n) {
   return c < 0 ? c + n : c;
 }
 
+// The value one site takes: mode 0 psi + dt*rhs, 1 rhs, 2 one derivative of
+// psi.  `in` may point to global memory or to LDS (flat addressing); the
+// arithmetic is the same either way.
 template <typename T, int DIM>
-__global__ __launch_bounds__(256) void ref_kernel(const T* __restrict__ in,
-                                                  T* __restrict__ out,
-                                                  RefParams<T> p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= p.n[0] || y >= p.n[1]) return;
+__device__ __forceinline__ T site_value(const RefParams<T>& p, const T* in,
+                                        int x, int y, int z) {
   const int64_t nx = p.n[0], nxy = (int64_t)p.n[0] * p.n[1];
   Window<T, DIM> w;
   w.f = in;
@@ -161,35 +159,43 @@ __global__ __launch_bounds__(256) void ref_kernel(const T* __restrict__ in,
     w.zo[d + 2] = DIM == 3 ? (int64_t)wrap(z + d, p.n[2]) * nxy : 0;
   }
   const int64_t idx = x + nx * y + nxy * z;
-  T result;
-  if (p.mode == 2) {
-    result = derivative<T, DIM>(p, w, IdentityOp<T>{}, p.apply_op,
-                                p.apply_axis, p.apply_component);
-  } else {
-    T acc = T(0);
-    for (int t = 0; t < p.nterms; ++t) {
-      const DevTerm<T>& tm = p.terms[t];
-      // the operand's grid: the field, or a provisional variable's grid,
-      // which the reference refreshes periodically like the field
-      // (sol/inc/modelspecialized.h:430-434); same offsets either way
-      Window<T, DIM> wo = w;
-      if (tm.operand > 0) wo.f = p.vars[tm.operand];
-      T val;
-      if (tm.op == SPS_OP_ID) {
-        val = tm.coeff * poly_eval(tm.poly, wo.f[idx]);
-      } else if (tm.identity) {
-        val = tm.coeff * derivative<T, DIM>(p, wo, IdentityOp<T>{}, tm.op,
-                                            tm.axis, tm.component);
-      } else {
-        val = tm.coeff * derivative<T, DIM>(p, wo, PolyOp<T>{tm.poly}, tm.op,
-                                            tm.axis, tm.component);
-      }
-      acc = t == 0 ? val : acc + val;
+  if (p.mode == 2)
+    return derivative<T, DIM>(p, w, IdentityOp<T>{}, p.apply_op, p.apply_axis,
+                              p.apply_component);
+  T acc = T(0);
+  for (int t = 0; t < p.nterms; ++t) {
+    const DevTerm<T>& tm = p.terms[t];
+    // the operand's grid: the field, or a provisional variable's grid,
+    // which the reference refreshes periodically like the field
+    // (sol/inc/modelspecialized.h:430-434); same offsets either way
+    Window<T, DIM> wo = w;
+    if (tm.operand > 0) wo.f = p.vars[tm.operand];
+    T val;
+    if (tm.op == SPS_OP_ID) {
+      val = tm.coeff * poly_eval(tm.poly, wo.f[idx]);
+    } else if (tm.identity) {
+      val = tm.coeff * derivative<T, DIM>(p, wo, IdentityOp<T>{}, tm.op,
+                                          tm.axis, tm.component);
+    } else {
+      val = tm.coeff * derivative<T, DIM>(p, wo, PolyOp<T>{tm.poly}, tm.op,
+                                          tm.axis, tm.component);
     }
-    // forward Euler psi + dt*dframe (examples/solvers/solverft.h:46-50)
-    result = p.mode == 0 ? in[idx] + p.dt * acc : acc;
+    acc = t == 0 ? val : acc + val;
   }
-  out[idx] = result;
+  // forward Euler psi + dt*dframe (examples/solvers/solverft.h:46-50)
+  return p.mode == 0 ? in[idx] + p.dt * acc : acc;
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void ref_kernel(const T* __restrict__ in,
+                                                  T* __restrict__ out,
+                                                  RefParams<T> p) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= p.n[0] || y >= p.n[1]) return;
+  const int64_t idx = x + (int64_t)p.n[0] * (y + (int64_t)p.n[1] * z);
+  out[idx] = site_value<T, DIM>(p, in, x, y, z);
 }
 
 }  // namespace
