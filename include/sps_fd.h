@@ -281,7 +281,7 @@ int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
  *     FP64: 2 or 1), dividing n[0].
  * 3D: (steps_per_launch, rows_per_tile) = steps fused per launch and y rows
  *     per wave, one of (1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4) and,
- *     FP32 only, (2, 6), (2, 8);
+ *     FP32 only, (2, 6), (2, 8), (2, 12);
  *     planes_per_chunk = z planes per streamed chunk, 1..1024;
  *     values_per_lane must be 0 or 1. */
 int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile,

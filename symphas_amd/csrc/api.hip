@@ -779,7 +779,7 @@ int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile,
         planes_per_chunk > 1024 || (values_per_lane != 0 && values_per_lane != 1) ||
         !sps::fused3d_shape_ok((int)p->esize, k, rows)) {
       set_error("tune (3D): (steps_per_launch, rows_per_tile) in "
-                "(1, 4|6|8|12|16), (2, 4) or, FP32 only, (2, 6|8); "
+                "(1, 4|6|8|12|16), (2, 4) or, FP32 only, (2, 6|8|12); "
                 "planes_per_chunk in 0..1024; values_per_lane 0 or 1");
       return SPS_E_INVALID;
     }

@@ -224,6 +224,7 @@ int launch_variant(int K, int rows, const T* in, T* out, int X, int Y, int Z,
       // 6 rows: the rings of four radius-one stages fit three waves per SIMD
       if (rows == 6) return launch3s<T, 6, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
       if (rows == 8) return launch3s<T, 8, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
+      if (rows == 12) return launch3s<T, 12, R, 2, HASA, POLY>(in, out, X, Y, Z, TZ, c, s);
     }
   }
   set_error("fused 3D step: unsupported launch shape (%d steps, %d rows)", K, rows);
@@ -247,7 +248,8 @@ int dispatch3(int K, int rows, const T* in, T* out, int X, int Y, int Z, int TZ,
 bool fused3d_shape_ok(int dtype_size, int K, int rows) {
   if (K == 1)
     return rows == 4 || rows == 6 || rows == 8 || rows == 12 || rows == 16;
-  if (K == 2) return rows == 4 || ((rows == 6 || rows == 8) && dtype_size == 4);
+  if (K == 2)
+    return rows == 4 || ((rows == 6 || rows == 8 || rows == 12) && dtype_size == 4);
   return false;
 }
 

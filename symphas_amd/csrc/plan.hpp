@@ -61,15 +61,15 @@ struct sps_fd_plan {
   void drop_graphs();
 
   // Launch shape before any tuning.  2D: the best of a sweep of steps per
-  // launch {1..4} x rows per chunk {16..256} on Model B 4096^2 (FP32: 3 x 32,
-  // FP64: 4 x 64).  3D: two steps per launch (FP32: 8 rows per wave, two
-  // waves per SIMD; FP64: 4 rows, one wave per SIMD) with the z-chunk height
-  // that splits the grid into as few rounds of resident waves as possible
-  // (on Model B 512^3 FP32 that is 171 planes, 3 chunks, 1920 waves).
+  // launch {1..4} x rows per chunk x lane width on Model B 4096^2 (FP32:
+  // 4 x 32 rows, 2 values per lane; FP64: 4 x 64).  3D: two steps per launch
+  // (FP32: 12 rows per wave, two waves per SIMD at 254 VGPRs; FP64: 4 rows,
+  // one wave per SIMD) with the z-chunk height that splits the grid into as
+  // few rounds of resident waves as possible.
   void choose_default_shape() {
     if (d.dim == 3) {
       K = 2;
-      TY = d.dtype == SPS_F32 ? 8 : 4;
+      TY = d.dtype == SPS_F32 ? 12 : 4;
       const int slots = d.dtype == SPS_F32 ? 2048 : 1024;  // resident waves
       const int halo = 2 * K;  // lanes per side at radius two
       const int64_t per_chunk = (int64_t)((d.n[0] + 63 - 2 * halo) / (64 - 2 * halo)) *
@@ -77,13 +77,14 @@ struct sps_fd_plan {
       const int64_t m = std::max<int64_t>(1, slots / per_chunk);
       TZ = (int)std::min<int64_t>(1024, std::max<int64_t>(16, (d.n[2] + m - 1) / m));
     } else if (d.dtype == SPS_F32) {
-      K = 3;
+      K = 4;
       TY = 32;
     } else {
       K = 4;
       TY = 64;
     }
-    VEC = 16 / (int)esize;
+    // FP32 2D: 8-byte lanes (2 values) won the shape sweep; FP64: 16-byte
+    VEC = (d.dim == 2 && d.dtype == SPS_F32) ? 8 / (int)esize : 16 / (int)esize;
     if (d.dim == 2 && d.n[0] % VEC != 0) VEC = 8 / (int)esize;
   }
 

@@ -260,7 +260,7 @@ def test_3d_chunking_and_fused_steps_are_bitwise(gpu_lib, dtype, model):
     psi = field(desc, seed=12)
     say("3D z-chunking, rows and fused steps", model, dtype)
     base = gpu_run(desc, psi, 11, K=1, TY=8, TZ=32)
-    shapes = [(1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4)] + ([(2, 6), (2, 8)] if dtype == F32 else [])
+    shapes = [(1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4)] + ([(2, 6), (2, 8), (2, 12)] if dtype == F32 else [])
     for K, rows in shapes:
         for TZ in (1, 4, 8, 20, 64):
             got = gpu_run(desc, psi, 11, K=K, TY=rows, TZ=TZ)
@@ -462,7 +462,8 @@ def test_autotune_3d(gpu_lib):
     with FDPlan(desc) as p:
         p.upload(psi)
         k, rows, planes, vec = p.autotune(use_cache=False)
-        assert (k, rows) in ((1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4), (2, 6), (2, 8))
+        assert (k, rows) in ((1, 4), (1, 6), (1, 8), (1, 12), (1, 16), (2, 4), (2, 6), (2, 8),
+                             (2, 12))
         assert planes in (8, 16, 32, 40) and vec == 1
         assert np.array_equal(p.download(), psi)
         p.step(6)
