@@ -272,6 +272,242 @@ template int launch_fused_multi<double>(const double* const*, double* const*,
                                         int, int, int, int, int,
                                         const MFusedCoef<double>&, hipStream_t);
 
+namespace {
+
+// ---------------------------------------------------------------- 3D
+
+// radius-one 7-point difference (unscaled) at output row r of a ring whose
+// plane S1 is centred on row r + 1; each lane move pairs with a register
+// operand so both fold into v_add_f32_dpp
+template <int S0, int S1, int S2, typename T, int NI>
+__device__ __forceinline__ T d7m(const T (&a)[3][NI], int r) {
+  const T ctr = a[S1][r + 1];
+  const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+  return fmad(T(-6), ctr,
+              ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +
+                  (a[S0][r + 1] + a[S2][r + 1]));
+}
+
+template <int N>
+struct MRowsOf {
+  static constexpr int value = N > 0 ? N : 1;
+};
+
+// Two coupled fields in 3D, streamed along z like fused3d_stream: one wave
+// owns 64 columns (one per lane, R*K halo lanes per side) x TYW rows of a
+// z-chunk; K steps are a chain of radius-one stages (radius two: mu, then
+// update), every stage carrying both fields in rings of three planes.
+template <typename T, int TYW, int R, int K>
+__global__ __launch_bounds__(256) void fused3d_multi(
+    const T* __restrict__ in0, const T* __restrict__ in1, T* __restrict__ out0,
+    T* __restrict__ out1, int X, int Y, int Z, int TZ, int nstrips, int nrowblk,
+    int nitems, MFusedCoef<T> c) {
+  constexpr int RT = R * K;
+  constexpr int NS = (R == 2 ? 2 : 1) * K;
+  static_assert(NS >= 1 && NS <= 2, "at most two stages");
+  constexpr int HL = RT;
+  constexpr int W = 64 - 2 * HL;
+  constexpr int NP = TYW + 2 * RT;
+  const int lane = threadIdx.x & 63;
+  const int item = __builtin_amdgcn_readfirstlane(
+      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (item >= nitems) return;
+  const int strip = item % nstrips;
+  const int rb = (item / nstrips) % nrowblk;
+  const int zc = item / (nstrips * nrowblk);
+  const int x = strip * W + lane - HL;
+  int col = x % X;
+  if (col < 0) col += X;
+  const bool store_lane = lane >= HL && lane < 64 - HL && x < X;
+  const int y0 = rb * TYW;
+  const int z0 = zc * TZ;
+  const int tz = min(TZ, Z - z0);
+  const int64_t XY = (int64_t)X * Y;
+  int rowoff[NP];
+#pragma unroll
+  for (int r = 0; r < NP; ++r) {
+    int y = (y0 - RT + r) % Y;
+    if (y < 0) y += Y;
+    rowoff[r] = __builtin_amdgcn_readfirstlane(y * X);
+  }
+  const int NI = tz + 2 * RT;
+  int zl = (z0 - RT) % Z;
+  if (zl < 0) zl += Z;
+
+  T A0[2][3][NP];
+  T A1[2][3][MRowsOf<NS >= 2 ? NP - 2 : 0>::value];
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+      for (int r = 0; r < NP; ++r) A0[f][q][r] = T(0);
+#pragma unroll
+      for (int r = 0; r < MRowsOf<NS >= 2 ? NP - 2 : 0>::value; ++r)
+        A1[f][q][r] = T(0);
+    }
+
+  T NX[2][NP];
+  auto fetch = [&]() {
+    const int64_t po = (int64_t)zl * XY;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+      NX[0][r] = (in0 + po + rowoff[r])[col];
+      NX[1][r] = (in1 + po + rowoff[r])[col];
+    }
+    zl = zl + 1 == Z ? 0 : zl + 1;
+  };
+  fetch();
+
+  auto stage = [&](auto sc, auto PHc, auto& a, auto& b, auto& o, int i) {
+    constexpr int S = decltype(sc)::value;
+    constexpr int PH = decltype(PHc)::value;
+    constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;
+    constexpr int NO = NP - 2 * S;
+    constexpr bool MU = R == 2 && (S % 2 == 1);
+    constexpr bool LAST = S == NS;
+    if constexpr (LAST) {
+      if (i < 2 * NS) return;
+    } else {
+      if (i < 2 * (S - 1)) return;
+    }
+    T res[2][NO];
+#pragma unroll
+    for (int r = 0; r < NO; ++r) {
+      const T d0 = d7m<S0, S1, S2>(a[0], r), d1 = d7m<S0, S1, S2>(a[1], r);
+      if constexpr (MU) {
+        const T xv = a[0][S1][r + 1], yv = a[1][S1][r + 1];
+        T p0, p1;
+        eval_poly<T, 1>(c.b_class[0], c.b[0], &xv, &yv, &p0);
+        eval_poly<T, 1>(c.b_class[1], c.b[1], &xv, &yv, &p1);
+        res[0][r] = fmad(c.kmu[0], d0, p0);
+        res[1][r] = fmad(c.kmu[1], d1, p1);
+      } else {
+        T xv, yv;
+        if constexpr (R == 2) {
+          xv = b[0][S0][r + 2];
+          yv = b[1][S0][r + 2];
+        } else {
+          xv = a[0][S1][r + 1];
+          yv = a[1][S1][r + 1];
+        }
+        T p0, p1;
+        eval_poly<T, 1>(c.adt_class[0], c.adt[0], &xv, &yv, &p0);
+        eval_poly<T, 1>(c.adt_class[1], c.adt[1], &xv, &yv, &p1);
+        res[0][r] = fmad(R == 2 ? c.kd : c.k1[0], d0, xv + p0);
+        res[1][r] = fmad(R == 2 ? c.kd : c.k1[1], d1, yv + p1);
+      }
+    }
+    if constexpr (LAST) {
+      if (store_lane) {
+        const int64_t po = (int64_t)(z0 + i - 2 * NS) * XY + col;
+        if (y0 + NO <= Y) {
+#pragma unroll
+          for (int r = 0; r < NO; ++r) {
+            out0[po + rowoff[r + RT]] = res[0][r];
+            out1[po + rowoff[r + RT]] = res[1][r];
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < NO; ++r)
+            if (y0 + r < Y) {
+              out0[po + rowoff[r + RT]] = res[0][r];
+              out1[po + rowoff[r + RT]] = res[1][r];
+            }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < NO; ++r) {
+        o[0][S2][r] = res[0][r];
+        o[1][S2][r] = res[1][r];
+      }
+    }
+  };
+
+  auto phase = [&](auto PHc, int i) {
+    constexpr int PH = decltype(PHc)::value;
+    constexpr int S2 = (PH + 2) % 3;
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int r = 0; r < NP; ++r) A0[f][S2][r] = NX[f][r];
+    if (i + 1 < NI) fetch();
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    stage(I1{}, PHc, A0, A0, A1, i);
+    if constexpr (NS >= 2) stage(I2{}, PHc, A1, A0, A0, i);
+  };
+  for (int i = 0; i < NI; i += 3) {
+    phase(std::integral_constant<int, 0>{}, i);
+    if (i + 1 < NI) phase(std::integral_constant<int, 1>{}, i + 1);
+    if (i + 2 < NI) phase(std::integral_constant<int, 2>{}, i + 2);
+  }
+}
+
+template <typename T, int TYW, int R, int K>
+int launch3m(const T* const* in, T* const* out, int X, int Y, int Z, int TZ,
+             const MFusedCoef<T>& c, hipStream_t s) {
+  constexpr int W = 64 - 2 * R * K;
+  const int nstrips = (X + W - 1) / W;
+  const int nrowblk = (Y + TYW - 1) / TYW;
+  const int nzc = (Z + TZ - 1) / TZ;
+  const int64_t nitems = (int64_t)nstrips * nrowblk * nzc;
+  if (nitems > ((int64_t)1 << 31) - 4) {
+    set_error("fused coupled 3D step: too many work items");
+    return SPS_E_INVALID;
+  }
+  fused3d_multi<T, TYW, R, K><<<(unsigned)((nitems + 3) / 4), 256, 0, s>>>(
+      in[0], in[1], out[0], out[1], X, Y, Z, TZ, nstrips, nrowblk, (int)nitems,
+      c);
+  return SPS_OK;
+}
+
+}  // namespace
+
+// 3D shapes: radius one K in {1, 2}, radius two K = 1; 4 or 8 rows per wave
+bool fused3d_multi_shape_ok(int radius, int K, int rows) {
+  return (rows == 4 || rows == 8) && K >= 1 && K <= (radius == 1 ? 2 : 1);
+}
+
+template <typename T>
+int launch_fused3d_multi(const T* const* in, T* const* out, int X, int Y,
+                         int Z, int K, int TY, int TZ, const MFusedCoef<T>& c,
+                         hipStream_t s) {
+  if (!fused3d_multi_shape_ok(c.radius, K, TY) || TZ < 1) {
+    set_error("fused coupled 3D step: unsupported shape (%d steps, %d rows)", K,
+              TY);
+    return SPS_E_INVALID;
+  }
+  int rc;
+  if (c.radius == 1) {
+    if (K == 1)
+      rc = TY == 4 ? launch3m<T, 4, 1, 1>(in, out, X, Y, Z, TZ, c, s)
+                   : launch3m<T, 8, 1, 1>(in, out, X, Y, Z, TZ, c, s);
+    else
+      rc = TY == 4 ? launch3m<T, 4, 1, 2>(in, out, X, Y, Z, TZ, c, s)
+                   : launch3m<T, 8, 1, 2>(in, out, X, Y, Z, TZ, c, s);
+  } else {
+    rc = TY == 4 ? launch3m<T, 4, 2, 1>(in, out, X, Y, Z, TZ, c, s)
+                 : launch3m<T, 8, 2, 1>(in, out, X, Y, Z, TZ, c, s);
+  }
+  if (rc != SPS_OK) return rc;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("fused coupled 3D launch failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+template int launch_fused3d_multi<float>(const float* const*, float* const*,
+                                         int, int, int, int, int, int,
+                                         const MFusedCoef<float>&, hipStream_t);
+template int launch_fused3d_multi<double>(const double* const*, double* const*,
+                                          int, int, int, int, int, int,
+                                          const MFusedCoef<double>&,
+                                          hipStream_t);
+
 }  // namespace sps
 
 // ------------------------------------------------------------------ plans
@@ -286,6 +522,7 @@ struct sps_fdm_plan {
   int cur = 0;
   bool has_data = false;
   int K = 1, TY = 32, VEC = 1;
+  int TZ = 64;  // 3D: z planes per streamed chunk
   sps::MRefParams<float> rp32;
   sps::MRefParams<double> rp64;
   sps::MFusedCoef<float> fc32;
@@ -364,10 +601,12 @@ int validate_mdesc(const sps_fdm_desc* d) {
 
 MLowered lower_multi(const sps_fdm_desc& d) {
   MLowered lw;
-  if (d.dim != 2 || d.nfields != 2) {
-    lw.why_not = "the fused coupled kernel is 2D with two fields";
+  if (d.nfields != 2) {
+    lw.why_not = "the fused coupled kernels take two fields";
     return lw;
   }
+  // lap5 o lap5 = bilap13 in 2D, lap7 o lap7 = bilap25 in 3D
+  const int lap_pts = d.dim == 2 ? 5 : 7, bilap_pts = d.dim == 2 ? 13 : 25;
   bool any_lap = false, any_bilap = false;
   for (int f = 0; f < 2; ++f)
     for (int t = 0; t < d.nterms[f]; ++t) {
@@ -398,12 +637,12 @@ MLowered lower_multi(const sps_fdm_desc& d) {
           return lw;
       }
     }
-  if (any_lap && d.lap_points != 5) {
-    lw.why_not = "laplacian stencil is not the 5-point one";
+  if (any_lap && d.lap_points != lap_pts) {
+    lw.why_not = "laplacian stencil is not the 5-point (2D) / 7-point (3D) one";
     return lw;
   }
-  if (any_bilap && d.bilap_points != 13) {
-    lw.why_not = "bilaplacian stencil is not the 13-point one";
+  if (any_bilap && d.bilap_points != bilap_pts) {
+    lw.why_not = "bilaplacian stencil is not the 13-point (2D) / 25-point (3D) one";
     return lw;
   }
   // radius one when every B_f is b_f * psi_f and there is no bilaplacian
@@ -516,6 +755,19 @@ int sps_fdm_plan::launch_one(int k) {
         rc = SPS_E_HIP;
       }
     }
+  } else if (d.dim == 3) {
+    if (d.dtype == SPS_F32) {
+      const float* in[2] = {(const float*)buf[0][cur], (const float*)buf[1][cur]};
+      float* out[2] = {(float*)buf[0][1 - cur], (float*)buf[1][1 - cur]};
+      rc = launch_fused3d_multi<float>(in, out, d.n[0], d.n[1], d.n[2], k, TY,
+                                       TZ, fc32, stream);
+    } else {
+      const double* in[2] = {(const double*)buf[0][cur],
+                             (const double*)buf[1][cur]};
+      double* out[2] = {(double*)buf[0][1 - cur], (double*)buf[1][1 - cur]};
+      rc = launch_fused3d_multi<double>(in, out, d.n[0], d.n[1], d.n[2], k, TY,
+                                        TZ, fc64, stream);
+    }
   } else if (d.dtype == SPS_F32) {
     const float* in[2] = {(const float*)buf[0][cur], (const float*)buf[1][cur]};
     float* out[2] = {(float*)buf[0][1 - cur], (float*)buf[1][1 - cur]};
@@ -556,7 +808,7 @@ int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out) {
   p->esize = desc->dtype == SPS_F32 ? 4 : 8;
   p->nsites = (size_t)desc->n[0] * desc->n[1] * (desc->dim == 3 ? desc->n[2] : 1);
   p->low = lower_multi(*desc);
-  const int vec = 8 / (int)p->esize;
+  const int vec = desc->dim == 3 ? 1 : 8 / (int)p->esize;
   const bool fusable = p->low.fusable && desc->n[0] % vec == 0;
   if (desc->path == SPS_PATH_FUSED && !fusable) {
     set_error("fused coupled path unavailable: %s",
@@ -572,11 +824,26 @@ int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out) {
   if (p->path == SPS_PATH_FUSED) {
     make_mfused_coef(*desc, p->low, p->fc32);
     make_mfused_coef(*desc, p->low, p->fc64);
-    p->K = p->low.radius == 1 ? 4 : 2;
-    p->TY = 32;
-    // 8-byte lanes: the 16-byte variant holds twice the rows in registers
-    // and measured slower at every launch shape on 4096^2 FP32
-    p->VEC = 8 / (int)p->esize;
+    if (desc->dim == 3) {
+      // two fields per stage: radius one fuses 2 steps, radius two 1; the
+      // z-chunk splits the grid into about one round of resident waves
+      p->K = p->low.radius == 1 ? 2 : 1;
+      p->TY = p->esize == 8 ? 4 : 8;  // FP64 8-row rings exceed 256 VGPRs
+      p->VEC = 1;
+      const int halo = p->low.radius * p->K;
+      const int64_t per_chunk =
+          (int64_t)((desc->n[0] + 63 - 2 * halo) / (64 - 2 * halo)) *
+          ((desc->n[1] + p->TY - 1) / p->TY);
+      const int64_t m = std::max<int64_t>(1, 2048 / per_chunk);
+      p->TZ = (int)std::min<int64_t>(
+          1024, std::max<int64_t>(16, (desc->n[2] + m - 1) / m));
+    } else {
+      p->K = p->low.radius == 1 ? 4 : 2;
+      p->TY = 32;
+      // 8-byte lanes: the 16-byte variant holds twice the rows in registers
+      // and measured slower at every launch shape on 4096^2 FP32
+      p->VEC = 8 / (int)p->esize;
+    }
   }
   hipError_t e = hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -758,6 +1025,20 @@ int sps_fdm_tune(sps_fdm_plan* p, int steps_per_launch, int rows_per_tile,
               "4..512");
     return SPS_E_INVALID;
   }
+  if (p->d.dim == 3) {
+    const int k = steps_per_launch ? steps_per_launch : p->K;
+    const int rows = rows_per_tile ? rows_per_tile : p->TY;
+    if ((values_per_lane != 0 && values_per_lane != 1) ||
+        (p->path == SPS_PATH_FUSED &&
+         !fused3d_multi_shape_ok(p->low.radius, k, rows))) {
+      set_error("tune (coupled 3D): steps_per_launch 1..2 (radius one) or 1 "
+                "(radius two), rows_per_tile 4 or 8, values_per_lane 0 or 1");
+      return SPS_E_INVALID;
+    }
+    p->K = k;
+    p->TY = rows;
+    return SPS_OK;
+  }
   if (values_per_lane != 0) {
     const int narrow = 8 / (int)p->esize, wide = 16 / (int)p->esize;
     if ((values_per_lane != narrow && values_per_lane != wide) ||
@@ -870,8 +1151,11 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
   const int oldK = p->K, oldTY = p->TY, oldVEC = p->VEC, cur0 = p->cur;
   int bestK = oldK, bestTY = oldTY, bestVEC = oldVEC;
   std::vector<int> vecs;
-  for (int v : {16 / (int)p->esize, 8 / (int)p->esize})
-    if (p->d.n[0] % v == 0) vecs.push_back(v);
+  if (p->d.dim == 3)
+    vecs.push_back(1);
+  else
+    for (int v : {16 / (int)p->esize, 8 / (int)p->esize})
+      if (p->d.n[0] % v == 0) vecs.push_back(v);
   double best = 1e300;
   do {
     bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
@@ -886,7 +1170,10 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
     }
     for (int vec : vecs)
     for (int k = 1; k <= 4 && rc == SPS_OK; ++k)
-      for (int rows : {16, 32, 64}) {
+      for (int rows : {4, 8, 16, 32, 64}) {
+        if (p->d.dim == 3 ? !fused3d_multi_shape_ok(p->low.radius, k, rows)
+                          : rows < 16)
+          continue;
         p->K = k;
         p->TY = rows;
         p->VEC = vec;

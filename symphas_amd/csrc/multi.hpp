@@ -68,4 +68,12 @@ template <typename T>
 int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
                        int TY, int vec, const MFusedCoef<T>& c, hipStream_t s);
 
+// 3D: radius one K in {1, 2}, radius two K = 1; TY rows per wave in {4, 8},
+// TZ planes per streamed chunk
+bool fused3d_multi_shape_ok(int radius, int K, int rows);
+template <typename T>
+int launch_fused3d_multi(const T* const* in, T* const* out, int X, int Y,
+                         int Z, int K, int TY, int TZ, const MFusedCoef<T>& c,
+                         hipStream_t s);
+
 }  // namespace sps

@@ -113,7 +113,10 @@ def test_paths_and_layouts(gpu_lib):
         p.upload(0, a)
         assert np.array_equal(p.download(1), a)
         assert np.array_equal(p.download(0, abi.LAYOUT_PADDED), np.pad(a, 3, mode="wrap"))
-    for d in (models.model_c((12, 10, 6), stencil=Stencil3d2h(7, 10, 25), path=AUTO),
+    with MultiPlan(models.model_c((12, 10, 6), stencil=Stencil3d2h(7, 10, 25),
+                                  path=AUTO)) as p:
+        assert p.query()[0] == FUSED
+    for d in (models.model_c((12, 10, 6), stencil=Stencil3d2h(15, 10, 25), path=AUTO),
               models.turing((12, 10), stencil=Stencil2d2h(9, 6, 13), path=AUTO)):
         with MultiPlan(d) as p:
             assert p.query()[0] == REF
@@ -134,10 +137,15 @@ def test_invalid_descriptors(gpu_lib):
     d.nfields = 0
     with pytest.raises(abi.SpsError):
         MultiPlan(d)
-    d = models.model_c((16, 16, 8), stencil=Stencil3d2h(7, 10, 25), path=FUSED)
+    d = models.model_c((16, 16, 8), stencil=Stencil3d2h(15, 10, 25), path=FUSED)
     with pytest.raises(abi.SpsError) as ei:
         MultiPlan(d)
     assert ei.value.status == abi.E_UNSUPPORTED
+    with MultiPlan(models.model_c((16, 16, 8), stencil=Stencil3d2h(7, 10, 25),
+                                  path=FUSED)) as p:
+        for K, TY in ((2, 8), (1, 6), (1, 16)):
+            with pytest.raises(abi.SpsError):
+                p.tune(K, TY)
 
 
 def test_autotune_keeps_fields(gpu_lib):
@@ -198,3 +206,37 @@ def test_coupled_field_files(gpu_lib, orc):
         assert all(np.array_equal(q.download(f), now[f]) for f in range(2))
         with pytest.raises(abi.SpsError):
             q.checkpoint_text(2)
+
+
+CASES3 = [
+    ("model_c", (40, 18, 12), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
+    ("gray_scott", (36, 20, 14), 0.5, (0.16, 0.08, 0.035, 0.065)),
+    ("turing", (30, 22, 9), 0.01, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+]
+
+
+@pytest.mark.parametrize("name,n,dt,c", CASES3)
+def test_fused_3d_coupled(gpu_lib, orc, name, n, dt, c):
+    """3D two-field fused kernel: FP64 within 1e-12 of the oracle, FP32
+    bitwise across steps per launch and rows per wave, and within FP32
+    rounding of the FP64 oracle."""
+    st = Stencil3d2h(7, 10, 25)
+    say("coupled fused 3D", name, n)
+    d64 = build(name, st, n, dt, c, F64, AUTO)
+    with MultiPlan(d64) as p:
+        assert p.query()[0] == FUSED
+    fields = fields_for(d64, 74)
+    want = orc.mrun(d64, fields, 12, 8)
+    for g, w in zip(gpu_run(d64, fields, 12), want):
+        assert rel_l2(g, w) <= 1e-12
+    d32 = build(name, st, n, dt, c, F32, FUSED)
+    f32 = [a.astype(np.float32) for a in fields]
+    base = gpu_run(d32, f32, 12, K=1, TY=4)
+    radius_one = name != "model_c"
+    for K, TY in ((1, 8),) + (((2, 4), (2, 8)) if radius_one else ()):
+        got = gpu_run(d32, f32, 12, K=K, TY=TY)
+        for g, b in zip(got, base):
+            assert np.array_equal(g, b), (K, TY)
+    want32 = orc.mrun(d64, [a.astype(np.float64) for a in f32], 12, 8)
+    for g, w in zip(base, want32):
+        assert rel_l2(g.astype(np.float64), w) <= 2e-6
