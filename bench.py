@@ -211,10 +211,18 @@ def extra_cfg3(torch, stream, steps=200):
     with FDPlan(d) as p:
         p.set_stream(stream.cuda_stream)
         p.upload(psi)
+        # a device copy of the initial field, made before autotuning and
+        # warmup: a host-to-device transfer of the field right before the
+        # timed steps slows the kernels of the next ~25 ms
+        psi_dev = torch.from_numpy(psi).to("cuda")
+        torch.cuda.synchronize()
         shape = p.autotune(use_cache=False)
         m0, _ = p.reduce()
         p.step(steps)  # warm: the same call captures the replayed launch graph
-        p.upload(psi)
+        p.copy_from_device(psi_dev.data_ptr())  # back to the initial field
+        p.sync()
+        del psi_dev
+        torch.cuda.synchronize()
         a, b = _events(torch)
         a.record(stream); p.step(steps); b.record(stream); stream.synchronize()
         ms = a.elapsed_time(b)
@@ -626,6 +634,11 @@ def main():
     plan = FDPlan(d32)
     plan.set_stream(stream.cuda_stream)
     plan.upload(psi0)
+    # the initial field also stays on the device, copied before autotuning
+    # and warmup: a host-to-device transfer right before the timed steps
+    # slows the kernels of the next tens of milliseconds
+    psi0_dev = torch.from_numpy(psi0).to("cuda")
+    torch.cuda.synchronize()
     if args.steps_per_launch or args.rows_per_tile or args.values_per_lane:
         plan.tune(args.steps_per_launch, args.rows_per_tile, 0, args.values_per_lane)
         shape = (args.steps_per_launch, args.rows_per_tile, 0, args.values_per_lane)
@@ -640,7 +653,10 @@ def main():
     # warmup (also captures the step graphs)
     plan.step(args.warmup)
     plan.sync()
-    plan.upload(psi0)
+    # back to the initial field, device to device
+    plan.copy_from_device(psi0_dev.data_ptr())
+    plan.sync()
+    del psi0_dev
     torch.cuda.synchronize()
 
     start = torch.cuda.Event(enable_timing=True)
