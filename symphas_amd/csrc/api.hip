@@ -231,19 +231,26 @@ template <typename T>
 void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c) {
   const double divh = 1.0 / d.h;
   const double divh2 = divh * divh;
+  const double w = 2.0 * d.dim;  // centre weight of the 5/7-point difference
   int dega = -1, degb = -1;
   for (int k = 0; k <= kMaxDeg; ++k) {
-    c.adt[k] = T(d.dt * lw.a[k]);
-    c.b[k] = T(lw.b[k]);
     if (lw.a[k] != 0.0) dega = k;
     if (lw.b[k] != 0.0) degb = k;
   }
-  c.kmu = T(lw.kc * divh2);
-  c.kd = T(d.dt * divh2);
-  c.k1 = T(d.dt * divh2 * lw.b[1]);
+  const double kmu = lw.kc * divh2, kd = d.dt * divh2, k1 = kd * lw.b[1];
   c.dega = dega;
   c.degb = degb;
   c.radius = (lw.kc == 0.0 && degb <= 1) ? 1 : 2;
+  // mu~ = sigma*mu: sigma = 1/kmu, or 1 when mu has no laplacian term
+  const double sigma = kmu != 0.0 ? 1.0 / kmu : 1.0;
+  for (int k = 0; k < kMaxDeg; ++k) c.p[k] = T(sigma * lw.b[k + 1]);
+  c.p[0] = T(sigma * (lw.b[1] - w * kmu));
+  for (int k = 0; k <= kMaxDeg; ++k) c.q[k] = T(d.dt * lw.a[k]);
+  c.ks = T(kmu != 0.0 ? 1.0 : 0.0);
+  c.k1 = T(k1);
+  c.k2 = T(kd / sigma);
+  const int deg = dega > degb ? dega : degb;
+  c.poly = (deg <= 3 && (c.radius == 1 || kmu != 0.0)) ? 3 : 6;
 }
 
 // Reference-order parameters for one term list: group 0 is the right-hand

@@ -292,6 +292,14 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
         set_error("batch instance %d: polynomial degree above 3", i);
         return SPS_E_UNSUPPORTED;
       }
+    FusedCoef<float> fc;
+    make_fused_coef(d, lw, fc);
+    if (fc.poly != 3) {
+      set_error("batch instance %d: laplacian of a non-linear operand without "
+                "a bilaplacian term",
+                i);
+      return SPS_E_UNSUPPORTED;
+    }
   }
   if (sps_fd_device_count() < 1) {
     set_error("no HIP device: this library runs the step on an MI355X only");

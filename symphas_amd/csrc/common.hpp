@@ -56,18 +56,32 @@ struct RefParams {
 };
 
 // Canonical fused form: dpsi/dt = A(psi) + lap(B(psi)) + kc*bilap(psi), with
-// the coefficients pre-scaled for the kernels:
-//   mu   = B(psi) + kmu * D(psi)          D = unscaled 5/7-point difference
-//   psi' = psi + Adt(psi) + kd * D(mu)
-// kmu = kc*divh2, kd = dt*divh2, Adt = dt*A.  In "radius one" form (B linear,
-// kc = 0) the update is psi' = psi + Adt(psi) + k1 * D(psi), k1 = kd*b1.
+// B free of a constant term (its laplacian is zero).  With S the sum of the
+// 2*dim axial neighbours, w = 2*dim and D = S - w*psi the unscaled 5/7-point
+// difference, a radius-two step is
+//   mu   = B(psi) + kmu*D(psi),          kmu = kc*divh2
+//   psi' = psi + Adt(psi) + kd*D(mu),    kd = dt*divh2, Adt = dt*A
+// which the kernels evaluate with the centre weight and kmu folded into the
+// polynomial of the first stage:
+//   mu~  = ks*S + psi*p(psi)    mu~ = mu/kmu (ks = 1), or mu if kmu = 0 (ks = 0)
+//   psi' = psi + q(psi) + k2*D(mu~)   q = Adt, k2 = kd*kmu (kd if kmu = 0)
+// In "radius one" form (B = b1*psi, kc = 0) a step is
+//   psi' = psi + q(psi) + k1*D(psi)   k1 = kd*b1.
+// Two things stay unfolded on purpose: psi's own coefficient 1, and D as
+// fma(-w, psi, S), which is exactly zero on a uniform field.  Folded into the
+// rounded polynomial coefficients, either would bias the uniform mode's rate
+// every step, an error that grows linearly with the step count.  The first
+// stage's fold only perturbs mu~ by a rounding, which the exact D of the
+// second stage does not turn into such a bias.
 template <typename T>
 struct FusedCoef {
-  T adt[kMaxDeg + 1];
-  T b[kMaxDeg + 1];
-  T kmu, kd, k1;
-  int dega, degb;  // -1 when the polynomial is identically zero
+  T q[kMaxDeg + 1];
+  T p[kMaxDeg];
+  T ks, k1, k2;
+  int dega, degb;  // degrees of A and B, -1 when identically zero
   int radius;      // 1 or 2
+  int poly;        // degree bound of the kernel instance: 3, or 6 (also when
+                   // kmu = 0, which needs ks)
 };
 
 // Host-side lowering result.

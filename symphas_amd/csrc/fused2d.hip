@@ -61,8 +61,7 @@ template <typename T, int K, int R>
 int dispatch_poly(const T* in, T* out, int X, int Y, int TY, int vec,
                   const FusedCoef<T>& c, hipStream_t s) {
   const bool hasa = c.dega >= 0;
-  const int deg = c.dega > c.degb ? c.dega : c.degb;
-  if (deg <= 3) {
+  if (c.poly == 3) {
     return hasa ? dispatch_layout<T, K, R, true, 3>(in, out, X, Y, TY, vec, c, s)
                 : dispatch_layout<T, K, R, false, 3>(in, out, X, Y, TY, vec, c, s);
   }

@@ -11,8 +11,12 @@
 //   mu = B(psi) + kmu*D7(psi),  psi' = psi + Adt(psi) + kd*D7(mu)
 // (two stages: mu, then update), which is the reference's
 // -bilap psi - lap psi + lap psi^3 up to rounding; a radius-one step is one
-// stage psi' = psi + Adt(psi) + k1*D7(psi).  Every stage is evaluated with
-// the same expression whatever K, so K fused steps are bitwise K launches.
+// stage psi' = psi + Adt(psi) + k1*D7(psi).  The stages use the folded
+// coefficients of FusedCoef (S6 = sum of the 6 neighbours):
+//   mu~ = S6 + psi*p(psi),  psi' = psi + q(psi) + k2*(S6[mu~] - 6*mu~),
+//   radius one: psi' = psi + q(psi) + k1*(S6 - 6*psi).
+// Every stage is evaluated with the same expression whatever K, so K fused
+// steps are bitwise K launches.
 //
 // Plane pipeline: while input plane i is loaded, stage s produces plane i - s
 // of its output from planes i-s-1 .. i-s+1 of its input.  Input plane 0 lies
@@ -28,17 +32,16 @@
 namespace sps {
 namespace {
 
-// radius-one 7-point difference (unscaled) at output row r of a ring whose
-// plane S1 is centred on row r + 1
+// sum of the 6 neighbours at output row r of a ring whose plane S1 is
+// centred on row r + 1
 template <int S0, int S1, int S2, typename T, int NI>
-__device__ __forceinline__ T d7(const T (&a)[3][NI], int r) {
+__device__ __forceinline__ T n6(const T (&a)[3][NI], int r) {
   const T ctr = a[S1][r + 1];
   const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
   // each lane move pairs with a register operand, so both fold into
   // v_add_f32_dpp
-  return fmad(T(-6), ctr,
-              ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +
-                  (a[S0][r + 1] + a[S2][r + 1]));
+  return ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +
+         (a[S0][r + 1] + a[S2][r + 1]);
 }
 
 template <int N>
@@ -130,18 +133,20 @@ __global__ __launch_bounds__(256) void fused3d_stream(
     T res[NO];
 #pragma unroll
     for (int r = 0; r < NO; ++r) {
-      const T d = d7<S0, S1, S2>(a, r);
+      const T sn = n6<S0, S1, S2>(a, r);
+      const T x = a[S1][r + 1];
       T v;
       if constexpr (MU) {
-        v = fmad(c.kmu, d, horner<POLY>(c.b, a[S1][r + 1]));
+        const T t = horner<POLY - 1>(c.p, x);
+        v = fmad(t, x, POLY == 3 ? sn : c.ks * sn);
+      } else if constexpr (R == 2) {
+        T base = b[S0][r + 2];
+        if constexpr (HASA) base = base + horner<POLY>(c.q, base);
+        v = fmad(c.k2, fmad(T(-6), x, sn), base);
       } else {
-        T base;
-        if constexpr (R == 2)
-          base = b[S0][r + 2];
-        else
-          base = a[S1][r + 1];
-        if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-        v = fmad(R == 2 ? c.kd : c.k1, d, base);
+        T base = x;
+        if constexpr (HASA) base = base + horner<POLY>(c.q, x);
+        v = fmad(c.k1, fmad(T(-6), x, sn), base);
       }
       if constexpr (LAST)
         res[r] = v;
@@ -235,8 +240,7 @@ template <typename T, int R>
 int dispatch3(int K, int rows, const T* in, T* out, int X, int Y, int Z, int TZ,
               const FusedCoef<T>& c, hipStream_t s) {
   const bool hasa = c.dega >= 0;
-  const int deg = c.dega > c.degb ? c.dega : c.degb;
-  if (deg <= 3)
+  if (c.poly == 3)
     return hasa ? launch_variant<T, R, true, 3>(K, rows, in, out, X, Y, Z, TZ, c, s)
                 : launch_variant<T, R, false, 3>(K, rows, in, out, X, Y, Z, TZ, c, s);
   return hasa ? launch_variant<T, R, true, 6>(K, rows, in, out, X, Y, Z, TZ, c, s)

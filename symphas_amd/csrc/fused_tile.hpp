@@ -4,12 +4,12 @@
 // streams the rows of one chunk of the field through K pipelined time steps.
 // Neighbours along x come from adjacent lanes through DPP; neighbours along y
 // are the previous/next rows kept in registers, rotated by unrolling the row
-// loop by three.  Each stage applies one step of the canonical form
-//   radius 2:  mu   = B(psi) + kmu*D(psi)
-//              psi' = psi + Adt(psi) + kd*D(mu)
-//   radius 1:  psi' = psi + Adt(psi) + k1*D(psi)
-// with D the unscaled 5-point difference (sum of 4 neighbours - 4*centre), so
-// stage s lags stage s-1 by R rows and loses R columns at each strip edge.
+// loop by three.  Each stage applies one step of the canonical form in the
+// folded coefficients of FusedCoef (S = sum of the 4 neighbours):
+//   radius 2:  mu~  = S + psi*p(psi)          (ks*S when p has degree > 2)
+//              psi' = psi + q(psi) + k2*(S[mu~] - 4*mu~)
+//   radius 1:  psi' = psi + q(psi) + k1*(S - 4*psi)
+// so stage s lags stage s-1 by R rows and loses R columns at each strip edge.
 //
 // Two strip layouts:
 //   halo  (ROT = false): HL lanes at each side only feed the neighbours; the
@@ -38,18 +38,18 @@ struct Stage {
       }
   }
 
-  // Unscaled 5-point difference at the row held in slot C, with rows U and D
-  // above and below.
+  // Sum of the 4 neighbours of the row held in slot C, with rows U and D
+  // above and below; each lane move pairs with a register operand.
   template <int C, int U, int D>
-  __device__ __forceinline__ void diff5(const T (&A)[3][VEC],
-                                        T (&d)[VEC]) const {
+  __device__ __forceinline__ void nsum(const T (&A)[3][VEC],
+                                       T (&s)[VEC]) const {
     const T left0 = from_prev_lane<ROT>(A[C][VEC - 1]);
     const T rightN = from_next_lane<ROT>(A[C][0]);
 #pragma unroll
     for (int j = 0; j < VEC; ++j) {
       const T l = j == 0 ? left0 : A[C][j - 1];
       const T r = j == VEC - 1 ? rightN : A[C][j + 1];
-      d[j] = fmad(T(-4), A[C][j], (l + r) + (A[U][j] + A[D][j]));
+      s[j] = (l + r) + (A[U][j] + A[D][j]);
     }
   }
 
@@ -59,27 +59,29 @@ struct Stage {
     constexpr int S0 = PH % 3, S1 = (PH + 1) % 3, S2 = (PH + 2) % 3;
 #pragma unroll
     for (int j = 0; j < VEC; ++j) P[S2][j] = v[j];
+    T s[VEC];
+    nsum<S1, S2, S0>(P, s);
     if constexpr (R == 1) {
-      T d[VEC];
-      diff5<S1, S2, S0>(P, d);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        T base = P[S1][j];
-        if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-        v[j] = fmad(c.k1, d[j], base);
+        const T x = P[S1][j];
+        T base = x;
+        if constexpr (HASA) base = base + horner<POLY>(c.q, x);
+        v[j] = fmad(c.k1, fmad(T(-4), x, s[j]), base);
       }
     } else {
-      T d[VEC];
-      diff5<S1, S2, S0>(P, d);
 #pragma unroll
-      for (int j = 0; j < VEC; ++j)
-        M[S2][j] = fmad(c.kmu, d[j], horner<POLY>(c.b, P[S1][j]));
-      diff5<S1, S2, S0>(M, d);
+      for (int j = 0; j < VEC; ++j) {
+        const T x = P[S1][j];
+        const T t = horner<POLY - 1>(c.p, x);
+        M[S2][j] = fmad(t, x, POLY == 3 ? s[j] : c.ks * s[j]);
+      }
+      nsum<S1, S2, S0>(M, s);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
         T base = P[S0][j];
-        if constexpr (HASA) base = base + horner<POLY>(c.adt, base);
-        v[j] = fmad(c.kd, d[j], base);
+        if constexpr (HASA) base = base + horner<POLY>(c.q, base);
+        v[j] = fmad(c.k2, fmad(T(-4), M[S1][j], s[j]), base);
       }
     }
   }
