@@ -50,8 +50,9 @@ def parse():
     ap.add_argument("--shape-sweep", action="store_true",
                     help="only run every launch shape the 2D autotuner tries (steps per "
                          "launch 1..4 x [4 values per lane x rows 8..64, 2 values per lane "
-                         "x rows 16..128], 6 launches each, in that order) on the 4096^2 "
-                         "Model B field, for per-shape profiles")
+                         "x rows 16..128, each plus the CU-balanced heights], 6 launches "
+                         "each, in that order) on the 4096^2 Model B field, for per-shape "
+                         "profiles")
     ap.add_argument("--peak-probe", action="store_true",
                     help="measure this GPU's copy bandwidth and BF16 GEMM rate, then exit")
     ap.add_argument("--copy-probe", action="store_true",
@@ -113,11 +114,20 @@ def shape_sweep():
     torch.cuda.set_device(0)
     psi0 = inputs.uniform((N, N), seed=SEED, dtype=np.float32)
     d32 = models.model_b((N, N), h=1.0, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
     order = []
     with FDPlan(d32) as plan:
         plan.upload(psi0)
         for k in (1, 2, 3, 4):
-            for vec, rows_set in ((4, (8, 16, 32, 64)), (2, (16, 32, 64, 128))):
+            for vec, rows_set in ((4, [8, 16, 32, 64]), (2, [16, 32, 64, 128])):
+                # the autotuner's extra heights: m 4-wave workgroups per CU
+                hl = (k * 2 + vec - 1) // vec
+                nstrips = -(-N // ((64 - 2 * hl) * vec))
+                for m in range(2, 7):
+                    nch = m * cus * 4 // nstrips
+                    ty = -(-N // nch) if nch >= 1 else 0
+                    if 4 <= ty <= 512 and ty not in rows_set:
+                        rows_set.append(ty)
                 for rows in rows_set:
                     plan.tune(k, rows, 0, vec)
                     plan.step(6 * k)  # six launches, no graph

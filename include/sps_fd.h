@@ -275,8 +275,8 @@ int sps_fd_checkpoint_read_xdr(sps_fd_plan* plan, const void* data,
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 
 /* Override the fused path's launch shape (0 keeps the current value).
- * 2D: steps_per_launch in 1..4, rows_per_tile = rows per streamed chunk, a
- *     multiple of 4 in 4..512; planes_per_chunk must be 0; values_per_lane =
+ * 2D: steps_per_launch in 1..4, rows_per_tile = rows per streamed chunk,
+ *     4..512; planes_per_chunk must be 0; values_per_lane =
  *     consecutive x values per lane, 16 or 8 bytes' worth (FP32: 4 or 2,
  *     FP64: 2 or 1), dividing n[0].
  * 3D: (steps_per_launch, rows_per_tile) = steps fused per launch and y rows
@@ -289,7 +289,9 @@ int sps_fd_tune(sps_fd_plan* plan, int steps_per_launch, int rows_per_tile,
 
 /* Pick the fused path's launch shape by timing candidates on the device:
  * 2D steps_per_launch {1,2,3,4} x (16-byte lanes, rows_per_tile
- * {8,16,32,64} | 8-byte lanes, rows {16,32,64,128}); 3D (steps, rows) as
+ * {8,16,32,64} | 8-byte lanes, rows {16,32,64,128}), each lane width and
+ * steps_per_launch also with the chunk heights that give every CU of the
+ * device the same number (2..6) of 4-wave workgroups; 3D (steps, rows) as
  * sps_fd_tune allows x planes {8, 16, 32} and n[2] split into 1..8 equal
  * chunks.  The field is saved before and restored after the trial launches,
  * so the plan's state is unchanged.  When the environment variable

@@ -768,12 +768,11 @@ int sps_fd_tune(sps_fd_plan* p, int steps_per_launch, int rows_per_tile,
     const int wide = 16 / (int)p->esize, narrow = 8 / (int)p->esize;
     const int vec = values_per_lane ? values_per_lane : p->VEC;
     if (steps_per_launch < 0 || steps_per_launch > 4 || planes_per_chunk != 0 ||
-        (rows_per_tile != 0 &&
-         (rows_per_tile < 4 || rows_per_tile > 512 || rows_per_tile % 4)) ||
+        (rows_per_tile != 0 && (rows_per_tile < 4 || rows_per_tile > 512)) ||
         (vec != wide && vec != narrow) || p->d.n[0] % vec != 0) {
-      set_error("tune (2D): steps_per_launch in 0..4, rows_per_tile 0 or a "
-                "multiple of 4 in 4..512, planes_per_chunk 0, values_per_lane "
-                "0, %d or %d dividing the x extent", wide, narrow);
+      set_error("tune (2D): steps_per_launch in 0..4, rows_per_tile 0 or "
+                "4..512, planes_per_chunk 0, values_per_lane 0, %d or %d "
+                "dividing the x extent", wide, narrow);
       return SPS_E_INVALID;
     }
     if (steps_per_launch) p->K = steps_per_launch;

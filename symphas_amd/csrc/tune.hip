@@ -112,12 +112,35 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
   std::vector<Cand> cands;
   if (p->d.dim == 2) {
     // wide lanes (16 bytes) with short chunks, narrow lanes (8 bytes, twice
-    // the strips) with tall ones
+    // the strips) with tall ones; plus, per lane width and K, the chunk
+    // heights that give every CU the same number m of 4-wave workgroups (a
+    // CU with one workgroup more than the others sets the launch's length)
     const int wide = 16 / (int)p->esize, narrow = 8 / (int)p->esize;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) {
+      set_error("device query failed");
+      return SPS_E_HIP;
+    }
+    const int X = p->d.n[0], Y = p->d.n[1], R = p->fc64.radius;
     for (int k = 1; k <= 4; ++k) {
-      if (p->d.n[0] % wide == 0)
-        for (int r : {8, 16, 32, 64}) cands.push_back({k, r, 0, wide});
-      for (int r : {16, 32, 64, 128}) cands.push_back({k, r, 0, narrow});
+      for (int vec : {wide, narrow}) {
+        if (X % vec != 0) continue;
+        std::vector<int> rows = vec == wide ? std::vector<int>{8, 16, 32, 64}
+                                            : std::vector<int>{16, 32, 64, 128};
+        const int hl = (k * R + vec - 1) / vec;
+        const int nstrips = (X + (64 - 2 * hl) * vec - 1) / ((64 - 2 * hl) * vec);
+        for (int m = 2; m <= 6; ++m) {
+          const long nch = (long)m * cus * 4 / nstrips;
+          if (nch < 1) continue;
+          const long ty = (Y + nch - 1) / nch;
+          if (ty >= 4 && ty <= 512 &&
+              std::find(rows.begin(), rows.end(), (int)ty) == rows.end())
+            rows.push_back((int)ty);
+        }
+        for (int r : rows) cands.push_back({k, r, 0, vec});
+      }
     }
   } else {
     // chunk heights: short chunks, and Z split evenly into m chunks, which
