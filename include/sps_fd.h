@@ -334,9 +334,10 @@ int sps_fd_batch_tune(sps_fd_batch* batch, int steps_per_launch,
  * counterpart, as sps_fd_copy_device). */
 int sps_fd_batch_copy_device(sps_fd_batch* batch, int index, void* dev,
                              int direction);
-/* Time steps_per_launch {1..4} x rows_per_item {16,32,64,128} on the
- * device (fields saved and restored) and apply the fastest; returns the
- * choice through the optional pointers. */
+/* Time steps_per_launch {1..4} x rows_per_item {16,32,64,128, and for m =
+ * 2..6 the lowest height whose item count fits m items per SIMD of the
+ * device} on the device (fields saved and restored) and apply the fastest;
+ * returns the choice through the optional pointers. */
 int sps_fd_batch_autotune(sps_fd_batch* batch, int* steps_per_launch,
                           int* rows_per_item);
 /* Caller-owned hipStream_t for later work (NULL: the HIP null stream, as

@@ -166,6 +166,17 @@ struct sps_fd_batch {
     }
   }
 
+  // work items of the K-step kernel at item height ty (strips x chunks)
+  long count_items(int k, int ty) const {
+    long n = 0;
+    for (int i = 0; i < count; ++i) {
+      const int R = (h_insts[i].kind == 1 || h_insts[i].kind == 2) ? 2 : 1;
+      const int W = batch_strip_width(k, R);
+      n += (long)((descs[i].n[0] + W - 1) / W) * ((descs[i].n[1] + ty - 1) / ty);
+    }
+    return n;
+  }
+
   int build_items() {
     for (int k = 1; k <= 4; ++k) {
       std::vector<BatchItem> items;
@@ -548,7 +559,30 @@ int sps_fd_batch_autotune(sps_fd_batch* b, int* out_k, int* out_rows) {
           hipSuccess ||
       hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     rc = SPS_E_HIP;
-  for (int rows : {16, 32, 64, 128}) {
+  // item heights: fixed ones, plus per m = 2..6 the lowest height whose
+  // 4-step item count fits m items per SIMD of the device (a SIMD with one
+  // item more than the others sets the launch's length)
+  std::vector<int> cand_rows = {16, 32, 64, 128};
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+          hipSuccess)
+    rc = SPS_E_HIP;
+  for (int m = 2; m <= 6 && rc == SPS_OK; ++m) {
+    const long target = (long)m * cus * 4;
+    if (b->count_items(4, 512) > target) continue;
+    int lo = 4, hi = 512;  // count_items is non-increasing in the height
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (b->count_items(4, mid) <= target)
+        hi = mid;
+      else
+        lo = mid + 1;
+    }
+    if (std::find(cand_rows.begin(), cand_rows.end(), lo) == cand_rows.end())
+      cand_rows.push_back(lo);
+  }
+  for (int rows : cand_rows) {
     if (rc != SPS_OK) break;
     b->TY = rows;
     if ((rc = b->build_items()) != SPS_OK) break;

@@ -121,7 +121,7 @@ def test_batch_autotune_keeps_fields(gpu_lib):
         for i, f in enumerate(fields):
             b.upload(i, f)
         k, rows = b.autotune()
-        assert k in (1, 2, 3, 4) and rows in (16, 32, 64, 128)
+        assert k in (1, 2, 3, 4) and 4 <= rows <= 512
         for i, f in enumerate(fields):
             assert np.array_equal(b.download(i), f)
         b.step(24)
