@@ -399,9 +399,11 @@ int sps_fdm_query(const sps_fdm_plan* plan, int* path, int* steps_per_launch);
  * extent). */
 int sps_fdm_tune(sps_fdm_plan* plan, int steps_per_launch, int rows_per_tile,
                  int values_per_lane);
-/* Time steps_per_launch {1..4} x rows_per_tile {16,32,64} x both lane widths
- * on the device (fields saved and restored) and keep the fastest (no
- * reference counterpart, as sps_fd_autotune). */
+/* Time steps_per_launch {1..4} x rows_per_tile {16,32,64, and the chunk
+ * heights that give every CU the same number (2..6) of 4-wave workgroups} x
+ * both lane widths in 2D, (steps, rows) as sps_fdm_tune allows in 3D, on the
+ * device (fields saved and restored) and keep the fastest (no reference
+ * counterpart, as sps_fd_autotune). */
 int sps_fdm_autotune(sps_fdm_plan* plan, int* steps_per_launch,
                      int* rows_per_tile, int* values_per_lane);
 int sps_fdm_sync(sps_fdm_plan* plan);

@@ -1168,12 +1168,37 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
       rc = SPS_E_HIP;
       break;
     }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess) {
+      set_error("device query failed");
+      rc = SPS_E_HIP;
+      break;
+    }
     for (int vec : vecs)
-    for (int k = 1; k <= 4 && rc == SPS_OK; ++k)
-      for (int rows : {4, 8, 16, 32, 64}) {
-        if (p->d.dim == 3 ? !fused3d_multi_shape_ok(p->low.radius, k, rows)
-                          : rows < 16)
-          continue;
+    for (int k = 1; k <= 4 && rc == SPS_OK; ++k) {
+      std::vector<int> cand_rows;
+      if (p->d.dim == 3) {
+        for (int rows : {4, 8})
+          if (fused3d_multi_shape_ok(p->low.radius, k, rows)) cand_rows.push_back(rows);
+      } else {
+        cand_rows = {16, 32, 64};
+        // chunk heights that give every CU the same number m of 4-wave
+        // workgroups (a CU with one more sets the launch's length)
+        const int hl = (k * p->low.radius + vec - 1) / vec;
+        const int w = (64 - 2 * hl) * vec;
+        const int nstrips = (p->d.n[0] + w - 1) / w;
+        for (int m = 2; m <= 6; ++m) {
+          const long nch = (long)m * cus * 4 / nstrips;
+          if (nch < 1) continue;
+          const long ty = (p->d.n[1] + nch - 1) / nch;
+          if (ty >= 8 && ty <= 512 &&
+              std::find(cand_rows.begin(), cand_rows.end(), (int)ty) == cand_rows.end())
+            cand_rows.push_back((int)ty);
+        }
+      }
+      for (int rows : cand_rows) {
         p->K = k;
         p->TY = rows;
         p->VEC = vec;
@@ -1199,6 +1224,7 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
           bestVEC = vec;
         }
       }
+    }
     // restore the caller's fields
     p->cur = cur0;
     for (int f = 0; f < nf; ++f)

@@ -162,7 +162,7 @@ def test_autotune_keeps_fields(gpu_lib):
         p.upload(0, u)
         p.upload(1, v)
         k, rows, vec = p.autotune()
-        assert 1 <= k <= 4 and rows in (16, 32, 64) and vec in (2, 4)
+        assert 1 <= k <= 4 and 8 <= rows <= 512 and vec in (2, 4)
         p.step(12)
         for f in range(2):
             assert np.array_equal(p.download(f), want[f])
