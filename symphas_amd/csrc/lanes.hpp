@@ -141,10 +141,12 @@ __device__ __forceinline__ void store_vec(double* p, const double (&v)[VEC]) {
 
 // Spread consecutive workgroup ids (which the dispatcher deals round-robin to
 // the 8 XCDs) so that each XCD receives a contiguous range of tiles and the
-// halo rows neighbouring tiles share stay in one L2.
+// halo rows neighbouring tiles share stay in one L2.  XCD x receives blocks
+// x, x+8, ...: nb/8 of them, one more when x < nb%8; the map is a bijection
+// of [0, nb) for any nb.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
-  if ((nb & 7) != 0) return b;
-  return (b & 7) * (nb >> 3) + (b >> 3);
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 }  // namespace sps
