@@ -404,8 +404,8 @@ int sps_fd_plan::steps_per_launch() const {
 
 void sps_fd_plan::drop_graphs() {
   for (auto& g : graphs) {
-    if (g.exec) hipGraphExecDestroy(g.exec);
-    if (g.graph) hipGraphDestroy(g.graph);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
     g = GraphSlot{};
   }
 }
@@ -427,7 +427,7 @@ int sps_fd_plan::step(int nsteps) {
       hipError_t e = hipStreamEndCapture(stream, &graph);
       cur = start;  // capture records work; nothing ran
       if (rc != SPS_OK) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc;
       }
       if (e != hipSuccess) {
@@ -437,7 +437,7 @@ int sps_fd_plan::step(int nsteps) {
       hipGraphExec_t exec = nullptr;
       e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (e != hipSuccess) {
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         set_error("graph instantiate failed: %s", hipGetErrorString(e));
         return SPS_E_HIP;
       }
@@ -527,17 +527,17 @@ int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
 int sps_fd_destroy(sps_fd_plan* p) {
   if (!p) return SPS_OK;
   p->drop_graphs();
-  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->own_stream && p->own_stream != p->stream)
-    hipStreamSynchronize(p->own_stream);
+    (void)hipStreamSynchronize(p->own_stream);
   for (void* b : p->buf)
-    if (b) hipFree(b);
-  if (p->work) hipFree(p->work);
+    if (b) (void)hipFree(b);
+  if (p->work) (void)hipFree(p->work);
   for (void* v : p->vars)
-    if (v) hipFree(v);
-  if (p->partials) hipFree(p->partials);
-  if (p->sums) hipFree(p->sums);
-  if (p->own_stream) hipStreamDestroy(p->own_stream);
+    if (v) (void)hipFree(v);
+  if (p->partials) (void)hipFree(p->partials);
+  if (p->sums) (void)hipFree(p->sums);
+  if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
   return SPS_OK;
 }

@@ -160,8 +160,8 @@ struct sps_fd_batch {
 
   void drop_graphs() {
     for (auto& g : graphs) {
-      if (g.exec) hipGraphExecDestroy(g.exec);
-      if (g.graph) hipGraphDestroy(g.graph);
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
       g = GraphSlot{};
     }
   }
@@ -193,7 +193,7 @@ struct sps_fd_batch {
                        [](const BatchItem& a, const BatchItem& b) {
                          return a.ty > b.ty;
                        });
-      if (d_items[k - 1]) hipFree(d_items[k - 1]);
+      if (d_items[k - 1]) (void)hipFree(d_items[k - 1]);
       d_items[k - 1] = nullptr;
       nitems[k - 1] = (int)items.size();
       SPS_HIP_TRY(hipMalloc(&d_items[k - 1], sizeof(BatchItem) * items.size()));
@@ -240,7 +240,7 @@ struct sps_fd_batch {
         hipError_t e = hipStreamEndCapture(stream, &graph);
         parity = start;
         if (rc != SPS_OK) {
-          if (graph) hipGraphDestroy(graph);
+          if (graph) (void)hipGraphDestroy(graph);
           return rc;
         }
         if (e != hipSuccess) {
@@ -250,7 +250,7 @@ struct sps_fd_batch {
         hipGraphExec_t exec = nullptr;
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-          hipGraphDestroy(graph);
+          (void)hipGraphDestroy(graph);
           set_error("batch graph instantiate failed: %s", hipGetErrorString(e));
           return SPS_E_HIP;
         }
@@ -391,13 +391,13 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
 int sps_fd_batch_destroy(sps_fd_batch* b) {
   if (!b) return SPS_OK;
   b->drop_graphs();
-  if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->pool) hipFree(b->pool);
-  if (b->d_insts) hipFree(b->d_insts);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->pool) (void)hipFree(b->pool);
+  if (b->d_insts) (void)hipFree(b->d_insts);
   for (BatchItem* p : b->d_items)
-    if (p) hipFree(p);
-  if (b->d_red) hipFree(b->d_red);
-  if (b->own_stream) hipStreamDestroy(b->own_stream);
+    if (p) (void)hipFree(p);
+  if (b->d_red) (void)hipFree(b->d_red);
+  if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
   delete b;
   return SPS_OK;
 }
@@ -598,7 +598,10 @@ int sps_fd_batch_autotune(sps_fd_batch* b, int* out_k, int* out_rows) {
           break;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = SPS_E_HIP;
+          break;
+        }
         if (rep > 0) ms_best = std::min(ms_best, (double)ms);
       }
       if (rc == SPS_OK && ms_best < best) {
@@ -614,9 +617,9 @@ int sps_fd_batch_autotune(sps_fd_batch* b, int* out_k, int* out_rows) {
       hipStreamSynchronize(b->stream) != hipSuccess) {
     if (rc == SPS_OK) rc = SPS_E_HIP;
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  hipFree(saved);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(saved);
   b->K = rc == SPS_OK ? bestK : oldK;
   b->TY = rc == SPS_OK ? bestTY : oldTY;
   int rc2 = b->build_items();

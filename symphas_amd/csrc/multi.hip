@@ -869,13 +869,13 @@ int sps_fdm_create(const sps_fdm_desc* desc, sps_fdm_plan** out) {
 
 int sps_fdm_destroy(sps_fdm_plan* p) {
   if (!p) return SPS_OK;
-  if (p->stream) hipStreamSynchronize(p->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->own_stream && p->own_stream != p->stream)
-    hipStreamSynchronize(p->own_stream);
+    (void)hipStreamSynchronize(p->own_stream);
   for (auto& fb : p->buf)
     for (void* b : fb)
-      if (b) hipFree(b);
-  if (p->own_stream) hipStreamDestroy(p->own_stream);
+      if (b) (void)hipFree(b);
+  if (p->own_stream) (void)hipStreamDestroy(p->own_stream);
   delete p;
   return SPS_OK;
 }
@@ -1205,15 +1205,18 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
         double us = 1e300;
         for (int rep = 0; rep < 3 && rc == SPS_OK; ++rep) {
           if ((rc = p->launch_one(k)) != SPS_OK) break;  // warm
-          hipEventRecord(e0, p->stream);
+          if (hipEventRecord(e0, p->stream) != hipSuccess) rc = SPS_E_HIP;
           for (int i = 0; i < 12 / k && rc == SPS_OK; ++i) rc = p->launch_one(k);
-          hipEventRecord(e1, p->stream);
-          if (rc != SPS_OK || hipEventSynchronize(e1) != hipSuccess) {
+          if (rc != SPS_OK || hipEventRecord(e1, p->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess) {
             if (rc == SPS_OK) rc = SPS_E_HIP;
             break;
           }
           float ms = 0;
-          hipEventElapsedTime(&ms, e0, e1);
+          if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = SPS_E_HIP;
+            break;
+          }
           if (rep > 0) us = std::min(us, 1e3 * ms / ((12 / k) * k));
         }
         if (rc != SPS_OK) break;
@@ -1234,10 +1237,10 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
         rc = SPS_E_HIP;
     if (hipStreamSynchronize(p->stream) != hipSuccess && rc == SPS_OK) rc = SPS_E_HIP;
   } while (false);
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   for (void* v : saved)
-    if (v) hipFree(v);
+    if (v) (void)hipFree(v);
   if (rc != SPS_OK) {
     p->K = oldK;
     p->TY = oldTY;

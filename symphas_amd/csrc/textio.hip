@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void scatter_values(
 struct DevBufs {
   std::vector<void*> ptrs;
   ~DevBufs() {
-    for (void* p : ptrs) hipFree(p);
+    for (void* p : ptrs) (void)hipFree(p);
   }
   template <typename P>
   int alloc(P** p, size_t bytes) {
@@ -502,7 +502,7 @@ int write_text(const FieldView& v, const TextFormat& fm, char* text,
                                            (size_t)chunk_bytes) + 16;
       if (need > out_cap) {
         if (d_out) {
-          hipFree(d_out);
+          (void)hipFree(d_out);
           db.ptrs.erase(std::find(db.ptrs.begin(), db.ptrs.end(), (void*)d_out));
           d_out = nullptr;
         }

@@ -196,7 +196,10 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
           break;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          rc = SPS_E_HIP;
+          break;
+        }
         if (rep > 0) us = std::min(us, 1e3 * ms / (launches * spl));
       }
       if (rc != SPS_OK) break;
@@ -215,9 +218,9 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
       }
     }
   } while (false);
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  hipFree(saved);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(saved);
   if (rc != SPS_OK) {
     p->K = oldK;
     p->TY = oldTY;
