@@ -61,8 +61,8 @@ __device__ __forceinline__ void run_strip(const BatchInst& I, int parity,
   int col = cb % X;
   if (col < 0) col += X;
   const bool store_lane = lane >= HL && lane < 64 - HL && cb < X;
-  stream_tile<float, kBatchVec, K, R, HASA, 3, false>(in, out, X, Y, it.y0,
-                                                      it.ty, col, store_lane, c);
+  stream_tile<float, kBatchVec, K, R, HASA, 3>(in, out, X, Y, it.y0, it.ty, col,
+                                               store_lane, c);
 }
 
 // Persistent: a grid of resident waves walks the item list (strip x row

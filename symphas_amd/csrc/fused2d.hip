@@ -30,8 +30,8 @@ __global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
   const bool store_lane = lane >= HL && lane < 64 - HL && cb < X;
   const int y0 = chunk * TY;
   const int ty = min(TY, Y - y0);
-  stream_tile<T, VEC, K, R, HASA, POLY, false>(in, out, X, Y, y0, ty, col,
-                                               store_lane, c);
+  stream_tile<T, VEC, K, R, HASA, POLY>(in, out, X, Y, y0, ty, col, store_lane,
+                                        c);
 }
 
 template <typename T, int VEC, int K, int R, bool HASA, int POLY>

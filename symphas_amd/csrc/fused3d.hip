@@ -37,7 +37,7 @@ namespace {
 template <int S0, int S1, int S2, typename T, int NI>
 __device__ __forceinline__ T n6(const T (&a)[3][NI], int r) {
   const T ctr = a[S1][r + 1];
-  const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+  const T xl = from_prev_lane(ctr), xr = from_next_lane(ctr);
   // each lane move pairs with a register operand, so both fold into
   // v_add_f32_dpp
   return ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +

@@ -10,12 +10,8 @@
 //              psi' = psi + q(psi) + k2*(S[mu~] - 4*mu~)
 //   radius 1:  psi' = psi + q(psi) + k1*(S - 4*psi)
 // so stage s lags stage s-1 by R rows and loses R columns at each strip edge.
-//
-// Two strip layouts:
-//   halo  (ROT = false): HL lanes at each side only feed the neighbours; the
-//          strip's 64-2*HL inner lanes store.  Any X that is a multiple of VEC.
-//   whole (ROT = true):  X == 64*VEC, the wave holds entire rows and the x
-//          wrap is the DPP rotate; every lane stores.
+// HL lanes at each side of a strip only feed the neighbours; the strip's
+// 64-2*HL inner lanes store.  Any X that is a multiple of VEC.
 #pragma once
 
 #include "common.hpp"
@@ -23,7 +19,7 @@
 
 namespace sps {
 
-template <typename T, int VEC, int R, bool HASA, int POLY, bool ROT>
+template <typename T, int VEC, int R, bool HASA, int POLY>
 struct Stage {
   T P[3][VEC];  // input rows r-2, r-1, r (slots rotate with the phase)
   T M[3][VEC];  // mu rows r-3, r-2, r-1 (radius 2 only)
@@ -43,8 +39,8 @@ struct Stage {
   template <int C, int U, int D>
   __device__ __forceinline__ void nsum(const T (&A)[3][VEC],
                                        T (&s)[VEC]) const {
-    const T left0 = from_prev_lane<ROT>(A[C][VEC - 1]);
-    const T rightN = from_next_lane<ROT>(A[C][0]);
+    const T left0 = from_prev_lane(A[C][VEC - 1]);
+    const T rightN = from_next_lane(A[C][0]);
 #pragma unroll
     for (int j = 0; j < VEC; ++j) {
       const T l = j == 0 ? left0 : A[C][j - 1];
@@ -89,7 +85,7 @@ struct Stage {
 
 // Process one strip x chunk.  col: first column this lane loads (already
 // wrapped into [0, X)); y0/ty: first output row and row count of the chunk.
-template <typename T, int VEC, int K, int R, bool HASA, int POLY, bool ROT>
+template <typename T, int VEC, int K, int R, bool HASA, int POLY>
 __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
                                             T* __restrict__ out, int X, int Y,
                                             int y0, int ty, int col,
@@ -100,7 +96,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
   int rload = (y0 - LAG) % Y;
   if (rload < 0) rload += Y;
 
-  Stage<T, VEC, R, HASA, POLY, ROT> st[K];
+  Stage<T, VEC, R, HASA, POLY> st[K];
 #pragma unroll
   for (int s = 0; s < K; ++s) st[s].clear();
 

@@ -5,15 +5,11 @@
 
 namespace sps {
 
-// DPP controls of the GFX9 wave-wide shifts and rotates.
+// DPP controls of the GFX9 wave-wide shifts.
 //   lane i receives from lane i-1 (lane 0 receives 0):   wave_shr:1
 //   lane i receives from lane i+1 (lane 63 receives 0):  wave_shl:1
-//   lane i receives from lane i-1, lane 0 from lane 63:  wave_ror:1
-//   lane i receives from lane i+1, lane 63 from lane 0:  wave_rol:1
 constexpr int kDppWaveShl1 = 0x130;
-constexpr int kDppWaveRol1 = 0x134;
 constexpr int kDppWaveShr1 = 0x138;
-constexpr int kDppWaveRor1 = 0x13C;
 
 // bound_ctrl: a lane whose source lies outside the wave reads 0, so no "old"
 // value has to be set up before the move
@@ -35,14 +31,14 @@ __device__ __forceinline__ double dpp(double v) {
       double, ((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-// Value held by the previous / next lane; ROT wraps around the wave.
-template <bool ROT, typename T>
+// Value held by the previous / next lane (0 at the wave's ends).
+template <typename T>
 __device__ __forceinline__ T from_prev_lane(T v) {
-  return dpp<ROT ? kDppWaveRor1 : kDppWaveShr1>(v);
+  return dpp<kDppWaveShr1>(v);
 }
-template <bool ROT, typename T>
+template <typename T>
 __device__ __forceinline__ T from_next_lane(T v) {
-  return dpp<ROT ? kDppWaveRol1 : kDppWaveShl1>(v);
+  return dpp<kDppWaveShl1>(v);
 }
 
 __device__ __forceinline__ float fmad(float a, float b, float c) {

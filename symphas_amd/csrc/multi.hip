@@ -83,8 +83,8 @@ struct Stage2 {
   template <int C, int U, int D>
   __device__ __forceinline__ void diff5(const T (&A)[3][VEC],
                                         T (&d)[VEC]) const {
-    const T left0 = from_prev_lane<false>(A[C][VEC - 1]);
-    const T rightN = from_next_lane<false>(A[C][0]);
+    const T left0 = from_prev_lane(A[C][VEC - 1]);
+    const T rightN = from_next_lane(A[C][0]);
 #pragma unroll
     for (int j = 0; j < VEC; ++j) {
       const T l = j == 0 ? left0 : A[C][j - 1];
@@ -282,7 +282,7 @@ namespace {
 template <int S0, int S1, int S2, typename T, int NI>
 __device__ __forceinline__ T d7m(const T (&a)[3][NI], int r) {
   const T ctr = a[S1][r + 1];
-  const T xl = from_prev_lane<false>(ctr), xr = from_next_lane<false>(ctr);
+  const T xl = from_prev_lane(ctr), xr = from_next_lane(ctr);
   return fmad(T(-6), ctr,
               ((xl + a[S1][r]) + (xr + a[S1][r + 2])) +
                   (a[S0][r + 1] + a[S2][r + 1]));
