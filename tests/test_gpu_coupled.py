@@ -78,7 +78,7 @@ def test_fused_path(gpu_lib, orc, name, stencil, n, dt, c):
     base = gpu_run(d32, f32, 24, K=1, TY=8, vec=2)
     for vec in (2, 4):
         for K in (2, 3, 4):
-            for TY in (4, 16, 64):
+            for TY in (4, 16, 23, 64):
                 got = gpu_run(d32, f32, 24, K=K, TY=TY, vec=vec)
                 for g, b in zip(got, base):
                     assert np.array_equal(g, b), (K, TY, vec)
