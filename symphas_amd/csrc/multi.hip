@@ -66,7 +66,12 @@ __device__ __forceinline__ void eval_poly(int cls,
   }
 }
 
-template <typename T, int VEC, int R>
+// LIN (radius 2 only): 1 when field 1, 2 when field 0 has no bilaplacian and
+// B_f = b_f * psi_f (+ a constant).  That field's mu_f is b_f * psi_f, so its
+// first stage keeps the unscaled D(psi_f) and its update is
+// psi_f + Adt_f + k1_f * D(psi_f), k1_f = kd * b_f: the radius-one form, run
+// in step with the other field's two stages.
+template <typename T, int VEC, int R, int LIN>
 struct Stage2 {
   T P[2][3][VEC];  // input rows of both fields (slots rotate with the phase)
   T M[2][3][VEC];  // mu rows (radius 2)
@@ -113,26 +118,34 @@ struct Stage2 {
         v[1][j] = fmad(c.k1[1], d[1][j], P[1][S1][j] + a[1][j]);
       }
     } else {
-      eval_poly<T, VEC>(c.b_class[0], c.b[0], P[0][S1], P[1][S1], a[0]);
-      eval_poly<T, VEC>(c.b_class[1], c.b[1], P[0][S1], P[1][S1], a[1]);
+      constexpr bool lin0 = LIN == 2, lin1 = LIN == 1;
+      if constexpr (!lin0)
+        eval_poly<T, VEC>(c.b_class[0], c.b[0], P[0][S1], P[1][S1], a[0]);
+      if constexpr (!lin1)
+        eval_poly<T, VEC>(c.b_class[1], c.b[1], P[0][S1], P[1][S1], a[1]);
 #pragma unroll
-      for (int f = 0; f < 2; ++f)
+      for (int f = 0; f < 2; ++f) {
+        const bool lin = f == 0 ? lin0 : lin1;
 #pragma unroll
-        for (int j = 0; j < VEC; ++j) M[f][S2][j] = fmad(c.kmu[f], d[f][j], a[f][j]);
-      diff5<S1, S2, S0>(M[0], d[0]);
-      diff5<S1, S2, S0>(M[1], d[1]);
+        for (int j = 0; j < VEC; ++j)
+          M[f][S2][j] = lin ? d[f][j] : fmad(c.kmu[f], d[f][j], a[f][j]);
+      }
+      if constexpr (!lin0) diff5<S1, S2, S0>(M[0], d[0]);
+      if constexpr (!lin1) diff5<S1, S2, S0>(M[1], d[1]);
       eval_poly<T, VEC>(c.adt_class[0], c.adt[0], P[0][S0], P[1][S0], a[0]);
       eval_poly<T, VEC>(c.adt_class[1], c.adt[1], P[0][S0], P[1][S0], a[1]);
 #pragma unroll
       for (int j = 0; j < VEC; ++j) {
-        v[0][j] = fmad(c.kd, d[0][j], P[0][S0][j] + a[0][j]);
-        v[1][j] = fmad(c.kd, d[1][j], P[1][S0][j] + a[1][j]);
+        v[0][j] = lin0 ? fmad(c.k1[0], M[0][S1][j], P[0][S0][j] + a[0][j])
+                       : fmad(c.kd, d[0][j], P[0][S0][j] + a[0][j]);
+        v[1][j] = lin1 ? fmad(c.k1[1], M[1][S1][j], P[1][S0][j] + a[1][j])
+                       : fmad(c.kd, d[1][j], P[1][S0][j] + a[1][j]);
       }
     }
   }
 };
 
-template <typename T, int VEC, int K, int R>
+template <typename T, int VEC, int K, int R, int LIN>
 __global__ __launch_bounds__(256) void fused_multi(
     const T* __restrict__ in0, const T* __restrict__ in1, T* __restrict__ out0,
     T* __restrict__ out1, int X, int Y, int TY, int nstrips, int nitems,
@@ -156,7 +169,7 @@ __global__ __launch_bounds__(256) void fused_multi(
   int rload = (y0 - LAG) % Y;
   if (rload < 0) rload += Y;
 
-  Stage2<T, VEC, R> st[K];
+  Stage2<T, VEC, R, LIN> st[K];
 #pragma unroll
   for (int s = 0; s < K; ++s) st[s].clear();
 
@@ -208,7 +221,7 @@ __global__ __launch_bounds__(256) void fused_multi(
   }
 }
 
-template <typename T, int VEC, int K, int R>
+template <typename T, int VEC, int K, int R, int LIN>
 int launch_multi_v(const T* const* in, T* const* out, int X, int Y, int TY,
                    const MFusedCoef<T>& c, hipStream_t s) {
   constexpr int HL = (K * R + VEC - 1) / VEC;
@@ -216,18 +229,27 @@ int launch_multi_v(const T* const* in, T* const* out, int X, int Y, int TY,
   const int nstrips = (X + W - 1) / W;
   const int nchunks = (Y + TY - 1) / TY;
   const int nitems = nstrips * nchunks;
-  fused_multi<T, VEC, K, R><<<(nitems + 3) / 4, 256, 0, s>>>(
+  fused_multi<T, VEC, K, R, LIN><<<(nitems + 3) / 4, 256, 0, s>>>(
       in[0], in[1], out[0], out[1], X, Y, TY, nstrips, nitems, c);
   return SPS_OK;
 }
 
 // 16-byte lanes (VEC = 16 / sizeof(T)) or 8-byte lanes (VEC = 8 / sizeof(T))
-template <typename T, int K, int R>
+template <typename T, int K, int R, int LIN>
 int launch_multi_k(const T* const* in, T* const* out, int X, int Y, int TY,
                    int vec, const MFusedCoef<T>& c, hipStream_t s) {
   if (vec == 16 / (int)sizeof(T))
-    return launch_multi_v<T, 16 / sizeof(T), K, R>(in, out, X, Y, TY, c, s);
-  return launch_multi_v<T, 8 / sizeof(T), K, R>(in, out, X, Y, TY, c, s);
+    return launch_multi_v<T, 16 / sizeof(T), K, R, LIN>(in, out, X, Y, TY, c, s);
+  return launch_multi_v<T, 8 / sizeof(T), K, R, LIN>(in, out, X, Y, TY, c, s);
+}
+
+template <typename T, int K>
+int launch_multi_r(const T* const* in, T* const* out, int X, int Y, int TY,
+                   int vec, const MFusedCoef<T>& c, hipStream_t s) {
+  if (c.radius == 1) return launch_multi_k<T, K, 1, 0>(in, out, X, Y, TY, vec, c, s);
+  if (c.lin == 1) return launch_multi_k<T, K, 2, 1>(in, out, X, Y, TY, vec, c, s);
+  if (c.lin == 2) return launch_multi_k<T, K, 2, 2>(in, out, X, Y, TY, vec, c, s);
+  return launch_multi_k<T, K, 2, 0>(in, out, X, Y, TY, vec, c, s);
 }
 
 }  // namespace
@@ -241,17 +263,12 @@ int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
               "lane (8 or 16 bytes)", X, vec);
     return SPS_E_UNSUPPORTED;
   }
-  const bool r1 = c.radius == 1;
   int rc;
   switch (K) {
-    case 1: rc = r1 ? launch_multi_k<T, 1, 1>(in, out, X, Y, TY, vec, c, s)
-                    : launch_multi_k<T, 1, 2>(in, out, X, Y, TY, vec, c, s); break;
-    case 2: rc = r1 ? launch_multi_k<T, 2, 1>(in, out, X, Y, TY, vec, c, s)
-                    : launch_multi_k<T, 2, 2>(in, out, X, Y, TY, vec, c, s); break;
-    case 3: rc = r1 ? launch_multi_k<T, 3, 1>(in, out, X, Y, TY, vec, c, s)
-                    : launch_multi_k<T, 3, 2>(in, out, X, Y, TY, vec, c, s); break;
-    case 4: rc = r1 ? launch_multi_k<T, 4, 1>(in, out, X, Y, TY, vec, c, s)
-                    : launch_multi_k<T, 4, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 1: rc = launch_multi_r<T, 1>(in, out, X, Y, TY, vec, c, s); break;
+    case 2: rc = launch_multi_r<T, 2>(in, out, X, Y, TY, vec, c, s); break;
+    case 3: rc = launch_multi_r<T, 3>(in, out, X, Y, TY, vec, c, s); break;
+    case 4: rc = launch_multi_r<T, 4>(in, out, X, Y, TY, vec, c, s); break;
     default:
       set_error("fused coupled step: steps per launch %d not in 1..4", K);
       return SPS_E_INVALID;
@@ -687,6 +704,19 @@ void make_mfused_coef(const sps_fdm_desc& d, const MLowered& lw,
   }
   c.kd = T(d.dt * divh2);
   c.radius = lw.radius;
+  // a radius-two plan's field without a bilaplacian whose B is b*psi_f (plus
+  // a constant, whose laplacian is zero) takes the radius-one form
+  c.lin = 0;
+  if (lw.radius == 2)
+    for (int f = 0; f < 2; ++f) {
+      bool own = lw.kc[f] == 0.0 && (f == 0 ? lw.b[0][1][0] : lw.b[1][0][1]) != 0.0;
+      for (int i = 0; i <= kMDeg && own; ++i)
+        for (int j = 0; j <= kMDeg; ++j)
+          if (i + j >= 1 && !(f == 0 ? (i == 1 && j == 0) : (i == 0 && j == 1)) &&
+              lw.b[f][i][j] != 0.0)
+            own = false;
+      if (own) c.lin = f == 1 ? 1 : 2;
+    }
   for (int f = 0; f < 2; ++f) {
     c.adt_class[f] = poly_class(lw.a[f]);
     c.b_class[f] = poly_class(lw.b[f]);

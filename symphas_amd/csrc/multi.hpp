@@ -53,6 +53,7 @@ struct MFusedCoef {
   T kmu[2], k1[2], kd;
   int radius;
   int adt_class[2], b_class[2];
+  int lin;  // radius 2: 1 (field 1) or 2 (field 0) takes the radius-one form
 };
 
 struct MLowered {

@@ -40,6 +40,7 @@ def rel_l2(a, b):
 
 CASES = [
     ("model_c", Stencil2d2h(5, 6, 13), (40, 28), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
+    ("model_c_swapped", Stencil2d2h(5, 6, 13), (36, 30), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
     ("model_c", Stencil2d2h(9, 6, 17), (21, 19), 0.002, (1.0, 1.0, 0.5, 0.9, 0.3)),
     ("model_c", Stencil3d2h(7, 10, 25), (14, 11, 9), 0.002, (1.0, 1.0, 0.5, 0.9, 0.3)),
     ("gray_scott", Stencil2d2h(5, 6, 13), (48, 36), 0.5, (0.16, 0.08, 0.035, 0.065)),
@@ -49,6 +50,12 @@ CASES = [
 
 
 def build(name, stencil, n, dt, c, dtype, path):
+    if name == "model_c_swapped":
+        # Model C with the fields in the other order: the field without a
+        # bilaplacian (rho, B = rho) comes first
+        eqs = [[models.MTermSpec(t.op, t.coeff, tuple(t.pow[::-1]), t.axis, t.component)
+                for t in eq] for eq in reversed(models.model_c_terms(c))]
+        return models.make_mdesc(n, stencil, 1.0, dt, eqs, dtype, path)
     return getattr(models, name)(n, h=1.0, dt=dt, stencil=stencil, dtype=dtype, path=path, c=c)
 
 
