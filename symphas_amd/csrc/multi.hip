@@ -314,7 +314,7 @@ struct MRowsOf {
 // owns 64 columns (one per lane, R*K halo lanes per side) x TYW rows of a
 // z-chunk; K steps are a chain of radius-one stages (radius two: mu, then
 // update), every stage carrying both fields in rings of three planes.
-template <typename T, int TYW, int R, int K>
+template <typename T, int TYW, int R, int K, int LIN>
 __global__ __launch_bounds__(256) void fused3d_multi(
     const T* __restrict__ in0, const T* __restrict__ in1, T* __restrict__ out0,
     T* __restrict__ out1, int X, int Y, int Z, int TZ, int nstrips, int nrowblk,
@@ -391,14 +391,19 @@ __global__ __launch_bounds__(256) void fused3d_multi(
     T res[2][NO];
 #pragma unroll
     for (int r = 0; r < NO; ++r) {
-      const T d0 = d7m<S0, S1, S2>(a[0], r), d1 = d7m<S0, S1, S2>(a[1], r);
+      // a radius-one field (LIN) keeps its unscaled D from the first stage
+      // and its update reads that value instead of a second difference
+      constexpr bool lin0 = R == 2 && LIN == 2, lin1 = R == 2 && LIN == 1;
+      T d0 = T(0), d1 = T(0);
+      if constexpr (MU || !lin0) d0 = d7m<S0, S1, S2>(a[0], r);
+      if constexpr (MU || !lin1) d1 = d7m<S0, S1, S2>(a[1], r);
       if constexpr (MU) {
         const T xv = a[0][S1][r + 1], yv = a[1][S1][r + 1];
-        T p0, p1;
-        eval_poly<T, 1>(c.b_class[0], c.b[0], &xv, &yv, &p0);
-        eval_poly<T, 1>(c.b_class[1], c.b[1], &xv, &yv, &p1);
-        res[0][r] = fmad(c.kmu[0], d0, p0);
-        res[1][r] = fmad(c.kmu[1], d1, p1);
+        T p0 = T(0), p1 = T(0);
+        if constexpr (!lin0) eval_poly<T, 1>(c.b_class[0], c.b[0], &xv, &yv, &p0);
+        if constexpr (!lin1) eval_poly<T, 1>(c.b_class[1], c.b[1], &xv, &yv, &p1);
+        res[0][r] = lin0 ? d0 : fmad(c.kmu[0], d0, p0);
+        res[1][r] = lin1 ? d1 : fmad(c.kmu[1], d1, p1);
       } else {
         T xv, yv;
         if constexpr (R == 2) {
@@ -411,8 +416,10 @@ __global__ __launch_bounds__(256) void fused3d_multi(
         T p0, p1;
         eval_poly<T, 1>(c.adt_class[0], c.adt[0], &xv, &yv, &p0);
         eval_poly<T, 1>(c.adt_class[1], c.adt[1], &xv, &yv, &p1);
-        res[0][r] = fmad(R == 2 ? c.kd : c.k1[0], d0, xv + p0);
-        res[1][r] = fmad(R == 2 ? c.kd : c.k1[1], d1, yv + p1);
+        res[0][r] = lin0 ? fmad(c.k1[0], a[0][S1][r + 1], xv + p0)
+                         : fmad(R == 2 ? c.kd : c.k1[0], d0, xv + p0);
+        res[1][r] = lin1 ? fmad(c.k1[1], a[1][S1][r + 1], yv + p1)
+                         : fmad(R == 2 ? c.kd : c.k1[1], d1, yv + p1);
       }
     }
     if constexpr (LAST) {
@@ -462,7 +469,7 @@ __global__ __launch_bounds__(256) void fused3d_multi(
   }
 }
 
-template <typename T, int TYW, int R, int K>
+template <typename T, int TYW, int R, int K, int LIN>
 int launch3m(const T* const* in, T* const* out, int X, int Y, int Z, int TZ,
              const MFusedCoef<T>& c, hipStream_t s) {
   constexpr int W = 64 - 2 * R * K;
@@ -474,7 +481,7 @@ int launch3m(const T* const* in, T* const* out, int X, int Y, int Z, int TZ,
     set_error("fused coupled 3D step: too many work items");
     return SPS_E_INVALID;
   }
-  fused3d_multi<T, TYW, R, K><<<(unsigned)((nitems + 3) / 4), 256, 0, s>>>(
+  fused3d_multi<T, TYW, R, K, LIN><<<(unsigned)((nitems + 3) / 4), 256, 0, s>>>(
       in[0], in[1], out[0], out[1], X, Y, Z, TZ, nstrips, nrowblk, (int)nitems,
       c);
   return SPS_OK;
@@ -499,14 +506,20 @@ int launch_fused3d_multi(const T* const* in, T* const* out, int X, int Y,
   int rc;
   if (c.radius == 1) {
     if (K == 1)
-      rc = TY == 4 ? launch3m<T, 4, 1, 1>(in, out, X, Y, Z, TZ, c, s)
-                   : launch3m<T, 8, 1, 1>(in, out, X, Y, Z, TZ, c, s);
+      rc = TY == 4 ? launch3m<T, 4, 1, 1, 0>(in, out, X, Y, Z, TZ, c, s)
+                   : launch3m<T, 8, 1, 1, 0>(in, out, X, Y, Z, TZ, c, s);
     else
-      rc = TY == 4 ? launch3m<T, 4, 1, 2>(in, out, X, Y, Z, TZ, c, s)
-                   : launch3m<T, 8, 1, 2>(in, out, X, Y, Z, TZ, c, s);
+      rc = TY == 4 ? launch3m<T, 4, 1, 2, 0>(in, out, X, Y, Z, TZ, c, s)
+                   : launch3m<T, 8, 1, 2, 0>(in, out, X, Y, Z, TZ, c, s);
+  } else if (c.lin == 1) {
+    rc = TY == 4 ? launch3m<T, 4, 2, 1, 1>(in, out, X, Y, Z, TZ, c, s)
+                 : launch3m<T, 8, 2, 1, 1>(in, out, X, Y, Z, TZ, c, s);
+  } else if (c.lin == 2) {
+    rc = TY == 4 ? launch3m<T, 4, 2, 1, 2>(in, out, X, Y, Z, TZ, c, s)
+                 : launch3m<T, 8, 2, 1, 2>(in, out, X, Y, Z, TZ, c, s);
   } else {
-    rc = TY == 4 ? launch3m<T, 4, 2, 1>(in, out, X, Y, Z, TZ, c, s)
-                 : launch3m<T, 8, 2, 1>(in, out, X, Y, Z, TZ, c, s);
+    rc = TY == 4 ? launch3m<T, 4, 2, 1, 0>(in, out, X, Y, Z, TZ, c, s)
+                 : launch3m<T, 8, 2, 1, 0>(in, out, X, Y, Z, TZ, c, s);
   }
   if (rc != SPS_OK) return rc;
   hipError_t e = hipGetLastError();

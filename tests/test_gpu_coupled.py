@@ -217,6 +217,7 @@ def test_coupled_field_files(gpu_lib, orc):
 
 CASES3 = [
     ("model_c", (40, 18, 12), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
+    ("model_c_swapped", (34, 16, 10), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
     ("gray_scott", (36, 20, 14), 0.5, (0.16, 0.08, 0.035, 0.065)),
     ("turing", (30, 22, 9), 0.01, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
 ]
@@ -239,7 +240,7 @@ def test_fused_3d_coupled(gpu_lib, orc, name, n, dt, c):
     d32 = build(name, st, n, dt, c, F32, FUSED)
     f32 = [a.astype(np.float32) for a in fields]
     base = gpu_run(d32, f32, 12, K=1, TY=4)
-    radius_one = name != "model_c"
+    radius_one = not name.startswith("model_c")
     for K, TY in ((1, 8),) + (((2, 4), (2, 8)) if radius_one else ()):
         got = gpu_run(d32, f32, 12, K=K, TY=TY)
         for g, b in zip(got, base):
