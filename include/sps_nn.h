@@ -4,7 +4,7 @@
  *
  * No counterpart in the reference: SymPhas has no learned components.  The
  * surrogate is a convolutional neural operator that stands in for k-step
- * advances of the finite-difference solver (its fallback and judge); these
+ * advances of the finite-difference solver (its fallback and referee); these
  * entry points are its forward and backward building blocks, called by the
  * host-side training and inference code with device pointers.
  *

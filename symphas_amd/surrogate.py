@@ -1,5 +1,5 @@
 """Learned surrogate for k-step advances of the Model B solver, with the
-plain finite-difference solver as its data source, fallback and judge.
+plain finite-difference solver as its data source, fallback and referee.
 
 * Data: trajectories of the batched GPU solver (Model B 2D, periodic, h = 1,
   dt = 0.01), psi0 uniform in [-0.1, 0.1) plus a mean offset uniform in
@@ -176,11 +176,11 @@ class AdvanceReport:
 
 
 class SurrogateSolver:
-    """Surrogate k-step advances judged by the plain solver."""
+    """Surrogate k-step advances checked by the plain solver."""
 
     def __init__(self, net, batch, n, k=50, tol=1e-3, check_every=4):
         self.net, self.k, self.tol, self.check_every = net, k, tol, check_every
-        self.judge = PlainSolver(batch, n)
+        self.referee = PlainSolver(batch, n)
 
     def advance(self, psi, n_adv):
         rep = AdvanceReport()
@@ -199,7 +199,7 @@ class SurrogateSolver:
             cheap_bad = mass_bad | energy_bad
             need_ref = bool(on_plain.any()) or bool(cheap_bad.any()) or a % self.check_every == 0
             if need_ref:
-                ref = self.judge.advance(psi, self.k)
+                ref = self.referee.advance(psi, self.k)
                 rep.checks += 1
                 err = rel_l2(pred, ref)
                 rep.max_check_err = max(rep.max_check_err, float(err.max()))
@@ -216,7 +216,7 @@ class SurrogateSolver:
         return psi, rep
 
     def close(self):
-        self.judge.close()
+        self.referee.close()
 
 
 def time_plain(psi, steps, n):
