@@ -57,6 +57,24 @@ __device__ __forceinline__ void eval_poly(int cls,
 #pragma unroll
     for (int j = 0; j < VEC; ++j)
       p[j] = fmad(c[1][0], x[j], fmad(c[0][1], y[j], c[0][0]));
+  } else if (cls == kPolyXLin) {
+    // (c00 + c01 y + c02 y^2 + c03 y^3) + x (c10 + c11 y + c12 y^2)
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const T r1 = fmad(fmad(c[1][2], y[j], c[1][1]), y[j], c[1][0]);
+      const T r0 =
+          fmad(fmad(fmad(c[0][3], y[j], c[0][2]), y[j], c[0][1]), y[j], c[0][0]);
+      p[j] = fmad(r1, x[j], r0);
+    }
+  } else if (cls == kPolyYLin) {
+    // (c00 + c10 x + c20 x^2 + c30 x^3) + y (c01 + c11 x + c21 x^2)
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const T r1 = fmad(fmad(c[2][1], x[j], c[1][1]), x[j], c[0][1]);
+      const T r0 =
+          fmad(fmad(fmad(c[3][0], x[j], c[2][0]), x[j], c[1][0]), x[j], c[0][0]);
+      p[j] = fmad(r1, y[j], r0);
+    }
   } else if (cls == kPolyTri) {
 #pragma unroll
     for (int j = 0; j < VEC; ++j) p[j] = tripoly(c, x[j], y[j]);
@@ -696,8 +714,17 @@ int poly_class(const double (&c)[kMDeg + 1][kMDeg + 1]) {
       if (c[i][j] != 0.0) maxdeg = std::max(maxdeg, i + j);
   if (maxdeg < 0) return kPolyZero;
   if (maxdeg <= 1) return kPolyLin;
-  if (maxdeg <= 3) return kPolyTri;
-  return kPolyDense;
+  if (maxdeg > 3) return kPolyDense;
+  bool xlin = true, ylin = true;
+  for (int i = 0; i <= kMDeg; ++i)
+    for (int j = 0; j <= kMDeg; ++j)
+      if (c[i][j] != 0.0) {
+        if (i > 1) xlin = false;
+        if (j > 1) ylin = false;
+      }
+  if (xlin) return kPolyXLin;
+  if (ylin) return kPolyYLin;
+  return kPolyTri;
 }
 
 template <typename T>

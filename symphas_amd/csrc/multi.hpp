@@ -41,11 +41,20 @@ int launch_ref_multi(const MRefParams<T>& p, int dim, hipStream_t stream);
 // its nonzero coefficients, so the kernel evaluates only what is there:
 //   kPolyZero  identically 0                                   (0 FMAs)
 //   kPolyLin   c00 + c10 x + c01 y                             (2 FMAs)
+//   kPolyXLin  total degree <= 3, degree <= 1 in x             (6 FMAs)
+//   kPolyYLin  total degree <= 3, degree <= 1 in y             (6 FMAs)
 //   kPolyTri   total degree <= 3, triangular two-level Horner  (9 FMAs)
 //   kPolyDense every c[i][j], i, j <= 3                        (15 FMAs)
 // The class is a kernel argument, so the branch on it is uniform.
 constexpr int kMDeg = 3;
-enum { kPolyZero = 0, kPolyLin = 1, kPolyTri = 2, kPolyDense = 3 };
+enum {
+  kPolyZero = 0,
+  kPolyLin = 1,
+  kPolyTri = 2,
+  kPolyDense = 3,
+  kPolyXLin = 4,
+  kPolyYLin = 5
+};
 template <typename T>
 struct MFusedCoef {
   T adt[2][kMDeg + 1][kMDeg + 1];  // dt * A_f
