@@ -209,8 +209,10 @@ def _errors(got, want):
             "rel_l2_err": float(np.linalg.norm((g - w).ravel()) / np.linalg.norm(w.ravel()))}
 
 
-def extra_cfg3(torch, stream, steps=200):
-    """Model B 3D 512^3, h=1, dt=0.005, <7,10,25>, FP32 fused kernel."""
+def extra_cfg3(torch, stream, steps=200, cpu_seconds=6.0):
+    """Model B 3D 512^3, h=1, dt=0.005, <7,10,25>, FP32 fused kernel; the FP64
+    oracle timed on a few steps of the same field beside it (cpu_seconds = 0
+    skips that)."""
     import numpy as np
     from symphas_amd import abi, inputs, models
     from symphas_amd.solver import FDPlan
@@ -248,6 +250,18 @@ def extra_cfg3(torch, stream, steps=200):
         want = p.download()
     err = _errors(got, want)
     del got, want
+    cpu = None
+    if cpu_seconds > 0:
+        dcpu = models.model_b((n, n, n), h=1.0, dt=0.005, stencil=models.Stencil3d2h(7, 10, 25),
+                              dtype=abi.F64, path=abi.PATH_REFERENCE)
+        c = cpu_rate(dcpu, psi.astype(np.float64), (cpu_seconds, cpu_seconds / 2), "512^3")
+        cpu = {"value": round(c["omp"][0], 6), "unit": "Gsite-updates/s", "cores": c["omp"][1],
+               "kind": "port",
+               "sample": (f"FP64 C restatement of SolverFT (oracle/sps_oracle.c), OpenMP, "
+                          f"{c['omp'][2]} steps of this 512^3 field on the padded layout; "
+                          f"1 thread: {c['single'][0]:.6f} Gsite-updates/s over "
+                          f"{c['single'][2]} steps"),
+               "single_thread_value": round(c["single"][0], 6)}
     g = n ** 3 * steps / (ms / 1e3) / 1e9
     return {"workload": "Model B 3D 512^3 FP32, Stencil3d2h<7,10,25>, dt=0.005",
             "steps": steps, "Gsite_updates_per_s": g,
@@ -255,6 +269,7 @@ def extra_cfg3(torch, stream, steps=200):
                              "planes_per_chunk": shape[2]},
             "roofline": {"achieved_GBs": 8 * g, "peak_GBs": HBM_PEAK_GBS, "frac": 8 * g / HBM_PEAK_GBS},
             "mass_drift_rel": abs(m1 - m0) / float(np.abs(psi.astype(np.float64)).sum()),
+            "cpu_baseline": cpu,
             "accuracy_vs_fp64": {"reference": "FP64 GPU run of the same steps from the same "
                                               "FP32-rounded field", "steps": steps, **err}}
 
@@ -382,23 +397,23 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(args):
-    """Time the FP64 oracle (OpenMP, then 1 thread) on a bounded number of
-    steps of the same 4096^2 Model B workload."""
+def cpu_rate(desc, psi, budgets, what):
+    """Time the FP64 oracle's SolverFT step (equation, Euler, refresh on the
+    padded layout) on a bounded number of steps: OpenMP on the host's cores,
+    then 1 thread.  budgets = seconds for each; returns label -> (Gsite/s,
+    threads, steps)."""
+    import ctypes
     import numpy as np
 
     from oracle import pyoracle
-    from symphas_amd import abi, inputs, models
 
-    desc = models.model_b((N, N), dtype=abi.F64, path=abi.PATH_REFERENCE)
-    psi = inputs.uniform((N, N), seed=SEED)
     g = pyoracle.pad(desc, psi)
     lib = pyoracle.load()
     dframe = np.zeros_like(g)
-    import ctypes
     dptr = ctypes.cast(ctypes.byref(desc), ctypes.c_void_p)
     gp, fp = g.ctypes.data_as(ctypes.c_void_p), dframe.ctypes.data_as(ctypes.c_void_p)
     lib.orc_refresh(dptr, gp)
+    sites = int(np.prod([desc.n[a] for a in range(desc.dim)]))
 
     def one_step(nt):
         lib.orc_equation(dptr, gp, fp, nt)
@@ -406,20 +421,32 @@ def cpu_baseline(args):
         lib.orc_refresh(dptr, gp)
 
     out = {}
-    for label, nt in (("omp", host_cores()), ("single", 1)):
+    for (label, nt), budget in zip((("omp", host_cores()), ("single", 1)), budgets):
         t0 = time.perf_counter()
         one_step(nt)
         t1 = time.perf_counter() - t0
-        budget = args.cpu_seconds if label == "omp" else args.cpu_seconds / 2
-        nsteps = max(1, int(budget / max(t1, 1e-6)))
-        t0 = time.perf_counter()
-        for _ in range(nsteps):
-            one_step(nt)
-        dt = time.perf_counter() - t0
-        out[label] = (N * N * nsteps / dt / 1e9, nt, nsteps)
-        print(f"[cpu] {label}: {nt} threads, {nsteps} steps, {out[label][0]:.4f} Gsite/s",
+        if t1 >= budget:  # the first step already fills the budget: it is the sample
+            nsteps, dt = 1, t1
+        else:
+            nsteps = max(1, int(budget / max(t1, 1e-6)))
+            t0 = time.perf_counter()
+            for _ in range(nsteps):
+                one_step(nt)
+            dt = time.perf_counter() - t0
+        out[label] = (sites * nsteps / dt / 1e9, nt, nsteps)
+        print(f"[cpu] {what} {label}: {nt} threads, {nsteps} steps, {out[label][0]:.4f} Gsite/s",
               file=sys.stderr, flush=True)
     return out
+
+
+def cpu_baseline(args):
+    """The FP64 oracle on a bounded number of steps of the same 4096^2 Model B
+    workload."""
+    from symphas_amd import abi, inputs, models
+
+    desc = models.model_b((N, N), dtype=abi.F64, path=abi.PATH_REFERENCE)
+    psi = inputs.uniform((N, N), seed=SEED)
+    return cpu_rate(desc, psi, (args.cpu_seconds, args.cpu_seconds / 2), "4096^2")
 
 
 def extra_checkpoint(torch, stream):
@@ -587,7 +614,9 @@ def extra_coupled(torch, stream, steps=1000):
 
 def extra_configs(torch, stream, args):
     return (("cfg1_allen_cahn_64_fp64", lambda: extra_cfg1(torch, stream)),
-            ("cfg3_model_b_512cubed_fp32", lambda: extra_cfg3(torch, stream)),
+            ("cfg3_model_b_512cubed_fp32",
+             lambda: extra_cfg3(torch, stream,
+                                cpu_seconds=0.0 if args.no_cpu else args.cpu_seconds / 2)),
             ("cfg4_batched_256", lambda: extra_cfg4(torch, stream)),
             ("cfg5_surrogate", lambda: extra_cfg5(torch, stream, args.surrogate_iters)),
             ("checkpoint_gnu_text", lambda: extra_checkpoint(torch, stream)),
