@@ -75,6 +75,14 @@ __device__ __forceinline__ void eval_poly(int cls,
           fmad(fmad(fmad(c[3][0], x[j], c[2][0]), x[j], c[1][0]), x[j], c[0][0]);
       p[j] = fmad(r1, y[j], r0);
     }
+  } else if (cls == kPolySep) {
+    // (c10 x + c20 x^2 + c30 x^3) + (c00 + c01 y + c02 y^2 + c03 y^3)
+#pragma unroll
+    for (int j = 0; j < VEC; ++j) {
+      const T g =
+          fmad(fmad(fmad(c[0][3], y[j], c[0][2]), y[j], c[0][1]), y[j], c[0][0]);
+      p[j] = fmad(fmad(fmad(c[3][0], x[j], c[2][0]), x[j], c[1][0]), x[j], g);
+    }
   } else if (cls == kPolyTri) {
 #pragma unroll
     for (int j = 0; j < VEC; ++j) p[j] = tripoly(c, x[j], y[j]);
@@ -715,15 +723,17 @@ int poly_class(const double (&c)[kMDeg + 1][kMDeg + 1]) {
   if (maxdeg < 0) return kPolyZero;
   if (maxdeg <= 1) return kPolyLin;
   if (maxdeg > 3) return kPolyDense;
-  bool xlin = true, ylin = true;
+  bool xlin = true, ylin = true, sep = true;
   for (int i = 0; i <= kMDeg; ++i)
     for (int j = 0; j <= kMDeg; ++j)
       if (c[i][j] != 0.0) {
         if (i > 1) xlin = false;
         if (j > 1) ylin = false;
+        if (i > 0 && j > 0) sep = false;
       }
   if (xlin) return kPolyXLin;
   if (ylin) return kPolyYLin;
+  if (sep) return kPolySep;
   return kPolyTri;
 }
 

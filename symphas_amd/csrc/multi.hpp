@@ -43,6 +43,7 @@ int launch_ref_multi(const MRefParams<T>& p, int dim, hipStream_t stream);
 //   kPolyLin   c00 + c10 x + c01 y                             (2 FMAs)
 //   kPolyXLin  total degree <= 3, degree <= 1 in x             (6 FMAs)
 //   kPolyYLin  total degree <= 3, degree <= 1 in y             (6 FMAs)
+//   kPolySep   f(x) + g(y), each of degree <= 3                (6 FMAs)
 //   kPolyTri   total degree <= 3, triangular two-level Horner  (9 FMAs)
 //   kPolyDense every c[i][j], i, j <= 3                        (15 FMAs)
 // The class is a kernel argument, so the branch on it is uniform.
@@ -53,7 +54,8 @@ enum {
   kPolyTri = 2,
   kPolyDense = 3,
   kPolyXLin = 4,
-  kPolyYLin = 5
+  kPolyYLin = 5,
+  kPolySep = 6
 };
 template <typename T>
 struct MFusedCoef {
