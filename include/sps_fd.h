@@ -189,12 +189,44 @@ int sps_fd_equation(sps_fd_plan* plan, void* host_dframe, int layout);
 int sps_fd_apply(sps_fd_plan* plan, int op, int axis, int component,
                  void* host_out, int layout);
 
+/* Free-energy functional of the reductions: the Landau density of landau_fe
+ * (sol/inc/modelspecialized.h:31-45),
+ *     f(psi) = -c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2,
+ * whose gradient coefficient is 1/2 in the reference (kappa = 1); kappa lets a
+ * model with mu = -c1 psi + c2 psi^3 - kappa lap psi report the energy its
+ * dynamics decrease.  grad psi is taken per axis as
+ *   SPS_GRAD_FORWARD: (psi[i+1] - psi[i]) / h.  With it the gradient sum obeys
+ *                     discrete summation by parts on the periodic grid,
+ *                     sum |grad psi|^2 = -sum psi * lap psi for the 5-point
+ *                     (2D) / 7-point (3D) laplacian, so Model B with those
+ *                     stencils decreases exactly this energy;
+ *   SPS_GRAD_CENTRAL: (psi[i+1] - psi[i-1]) / (2h), the reference's generated
+ *                     first-derivative stencil at its default second-order
+ *                     accuracy (make_operator_derivative<1>,
+ *                     sol/inc/stencil.h:610-660, DEFAULT_STENCIL_ACCURACY 2
+ *                     at :45).
+ * mean = 0 reports the integral h^D * sum f; mean = 1 reports the mean of f
+ * over the sites, which is what the reference's Energy data process prints
+ * (energy * (1 / len), dataprocesses/dataenergy.h:11-40). */
+#define SPS_GRAD_FORWARD 0
+#define SPS_GRAD_CENTRAL 1
+typedef struct {
+  double c1, c2, kappa;
+  int32_t gradient; /* SPS_GRAD_FORWARD or SPS_GRAD_CENTRAL */
+  int32_t mean;     /* 0: h^D * sum; 1: sum / sites */
+} sps_fd_energy;
+
 /* FP64 global sums of the current field: mass = sum psi over the interior,
- * energy = h^D * sum [ -1/2 psi^2 + 1/4 psi^4 + 1/2 |grad+ psi|^2 ] with
- * forward differences for grad+.  Either pointer may be NULL.
- * Replaces: DensityData (dataprocesses/datadensity.h:11-41) and the Landau
- * free energy landau_fe (sol/inc/modelspecialized.h:31-45). */
+ * energy = the functional set by sps_fd_set_energy (by default c1 = c2 =
+ * kappa = 1, forward differences, the integral h^D * sum f).  Either pointer
+ * may be NULL.
+ * Replaces: DensityData (dataprocesses/datadensity.h:11-41) and the Energy
+ * data process over landau_fe (dataprocesses/dataenergy.h:11-40,
+ * sol/inc/modelspecialized.h:31-45). */
 int sps_fd_reduce(sps_fd_plan* plan, double* mass, double* energy);
+/* Set the functional sps_fd_reduce evaluates (NULL restores the default).
+ * Coefficients must be finite. */
+int sps_fd_set_energy(sps_fd_plan* plan, const sps_fd_energy* energy);
 
 /* Wait for all work enqueued on the plan. */
 int sps_fd_sync(sps_fd_plan* plan);
@@ -322,8 +354,13 @@ int sps_fd_batch_upload(sps_fd_batch* batch, int index, const void* host,
 int sps_fd_batch_download(sps_fd_batch* batch, int index, void* host,
                           int layout);
 int sps_fd_batch_step(sps_fd_batch* batch, int nsteps);
-/* mass and energy are arrays of count doubles (either may be NULL). */
+/* mass and energy are arrays of count doubles (either may be NULL); each
+ * instance's energy is the functional set for it by sps_fd_batch_set_energy
+ * (default as sps_fd_reduce). */
 int sps_fd_batch_reduce(sps_fd_batch* batch, double* mass, double* energy);
+/* Per-instance free-energy functionals: energy points to count entries (NULL
+ * restores the default for every instance). */
+int sps_fd_batch_set_energy(sps_fd_batch* batch, const sps_fd_energy* energy);
 int sps_fd_batch_sync(sps_fd_batch* batch);
 /* Launch shape of the batched kernel (0 keeps the current value):
  * steps_per_launch in 1..4, rows_per_item = rows of one work item, 4..512. */

@@ -51,6 +51,7 @@ def load():
         ("orc_run", c_int, [P, P, c_int, c_int]),
         ("orc_mass", c_double, [P, P]),
         ("orc_energy", c_double, [P, P]),
+        ("orc_energy_ex", c_double, [P, P, c_double, c_double, c_double, c_int, c_int]),
         ("orc_fill_uniform", None, [P, c_size_t, c_uint64, c_double, c_double]),
         ("orc_checkpoint_text", c_int64, [P, P, c_int, P, P, c_size_t]),
         ("orcm_desc_size", c_size_t, []),
@@ -144,8 +145,12 @@ def mass(desc, padded):
     return load().orc_mass(_d(desc), _p(padded))
 
 
-def energy(desc, padded):
-    return load().orc_energy(_d(desc), _p(padded))
+def energy(desc, padded, c1=1.0, c2=1.0, kappa=1.0, gradient=0, mean=False):
+    """Landau free energy of a padded field with fresh ghosts (gradient 0:
+    forward differences, 1: central; mean: divide by the site count instead
+    of multiplying by h^D)."""
+    return load().orc_energy_ex(_d(desc), _p(padded), float(c1), float(c2),
+                                float(kappa), int(gradient), int(bool(mean)))
 
 
 def fill_uniform(count, seed, lo, hi):

@@ -726,11 +726,20 @@ double orc_mass(const orc_desc* d, const double* psi) {
   return m;
 }
 
-double orc_energy(const orc_desc* d, const double* psi) {
+/* Landau free energy (sol/inc/modelspecialized.h:31-45):
+ *   f = -c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2,
+ * with grad psi per axis the forward difference (psi[i+1] - psi[i]) / h
+ * (gradient 0) or the central one (psi[i+1] - psi[i-1]) / (2h) (gradient 1,
+ * the reference's generated first-order stencil at accuracy 2,
+ * sol/inc/stencil.h:45,610-660).  Sequential sum over the interior, then
+ * times h^D (mean 0) or divided by the site count (mean 1, the Energy data
+ * process, dataprocesses/dataenergy.h:11-40).  psi must have fresh ghosts. */
+double orc_energy_ex(const orc_desc* d, const double* psi, double c1,
+                     double c2, double kappa, int gradient, int mean) {
   const int64_t X = pdim(d, 0), Y = pdim(d, 1);
   const int64_t nz = d->dim == 3 ? d->n[2] : 1;
   const int64_t z0 = d->dim == 3 ? ORC_DEPTH : 0;
-  const double divh = 1.0 / d->h;
+  const double divh = gradient == 1 ? 1.0 / (2.0 * d->h) : 1.0 / d->h;
   const int64_t st[3] = {1, X, X * Y};
   double e = 0.0;
   for (int64_t z = 0; z < nz; ++z)
@@ -738,17 +747,23 @@ double orc_energy(const orc_desc* d, const double* psi) {
       for (int64_t x = 0; x < d->n[0]; ++x) {
         int64_t i = (x + ORC_DEPTH) + X * ((y + ORC_DEPTH) + Y * (z + z0));
         double p = psi[i];
-        double p2 = p * p;
         double g2 = 0.0;
         for (int a = 0; a < d->dim; ++a) {
-          double gd = (psi[i + st[a]] - p) * divh;
+          double back = gradient == 1 ? psi[i - st[a]] : p;
+          double gd = (psi[i + st[a]] - back) * divh;
           g2 += gd * gd;
         }
-        e += -0.5 * p2 + 0.25 * (p2 * p2) + 0.5 * g2;
+        e += -c1 * 0.5 * (p * p) + c2 * 0.25 * (p * p * p * p) +
+             kappa * 0.5 * g2;
       }
+  if (mean) return e / ((double)d->n[0] * d->n[1] * nz);
   double hd = d->h * d->h;
   if (d->dim == 3) hd *= d->h;
   return e * hd;
+}
+
+double orc_energy(const orc_desc* d, const double* psi) {
+  return orc_energy_ex(d, psi, 1.0, 1.0, 1.0, 0, 0);
 }
 
 /* ------------------------------------------------------------- inputs */

@@ -125,8 +125,12 @@ int orc_run(const orc_desc* d, double* psi, int nsteps, int nthreads);
 
 /* Sequential FP64 sum of the interior (dataprocesses/datadensity.h:11-41). */
 double orc_mass(const orc_desc* d, const double* psi);
-/* h^D * sum[-1/2 psi^2 + 1/4 psi^4 + 1/2 |grad+ psi|^2], forward differences;
- * psi must have fresh ghosts. */
+/* Landau free energy h^D * sum[-c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2]
+ * (mean: the sum divided by the site count instead), grad by forward
+ * (gradient 0) or central (gradient 1) differences; psi must have fresh
+ * ghosts.  orc_energy is c1 = c2 = kappa = 1, forward, integral. */
+double orc_energy_ex(const orc_desc* d, const double* psi, double c1,
+                     double c2, double kappa, int gradient, int mean);
 double orc_energy(const orc_desc* d, const double* psi);
 
 /* The reference's GNU checkpoint file of a field (io/src/writegp.cpp:

@@ -24,6 +24,7 @@ AXIS_X, AXIS_Y, AXIS_Z = 0, 1, 2
 LAYOUT_INTERIOR, LAYOUT_PADDED = 0, 1
 PATH_AUTO, PATH_REFERENCE, PATH_FUSED = 0, 1, 2
 PLOT_GNU, PLOT_CSV = 1, 2
+GRAD_FORWARD, GRAD_CENTRAL = 0, 1
 
 STATUS_NAMES = {OK: "SPS_OK", E_INVALID: "SPS_E_INVALID", E_UNSUPPORTED: "SPS_E_UNSUPPORTED",
                 E_HIP: "SPS_E_HIP", E_NOMEM: "SPS_E_NOMEM", E_STATE: "SPS_E_STATE"}
@@ -57,6 +58,12 @@ class MDesc(ctypes.Structure):
                 ("bilap_points", c_int32), ("nfields", c_int32), ("path", c_int32),
                 ("h", c_double), ("dt", c_double), ("nterms", c_int32 * MAX_FIELDS),
                 ("terms", (MTerm * MAX_MTERMS) * MAX_FIELDS)]
+
+
+class Energy(ctypes.Structure):
+    # -c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2 (sps_fd_energy)
+    _fields_ = [("c1", c_double), ("c2", c_double), ("kappa", c_double),
+                ("gradient", c_int32), ("mean", c_int32)]
 
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libsps_fd.so"
@@ -94,6 +101,8 @@ def load() -> ctypes.CDLL:
         "sps_fd_equation": (c_int, [P, P, c_int]),
         "sps_fd_apply": (c_int, [P, c_int, c_int, c_int, P, c_int]),
         "sps_fd_reduce": (c_int, [P, POINTER(c_double), POINTER(c_double)]),
+        "sps_fd_set_energy": (c_int, [P, P]),
+        "sps_fd_batch_set_energy": (c_int, [P, P]),
         "sps_fd_sync": (c_int, [P]),
         "sps_fd_device_field": (c_int, [P, POINTER(P)]),
         "sps_fd_set_stream": (c_int, [P, P]),

@@ -60,12 +60,30 @@ def _compile(src: str, extra: list[str]) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
+def _stale(src: str) -> bool:
+    """An object is stale when older than its source or than any header."""
+    obj = OBJ / (Path(src).stem + ".o")
+    if not obj.exists():
+        return True
+    deps = [CSRC / src, *CSRC.glob("*.hpp"), *(PKG.parent / "include").glob("*.h")]
+    return obj.stat().st_mtime < max(d.stat().st_mtime for d in deps)
+
+
+def build(verbose: bool = True, stale_only: bool = False) -> Path:
+    """Compile every source (stale_only: only those whose object is older
+    than the source or a header) and link the library."""
     OBJ.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     jobs = min(8, len(SOURCES))
+
+    def one(kv):
+        src, extra = kv
+        if stale_only and not _stale(src):
+            return OBJ / (Path(src).stem + ".o")
+        return _compile(src, extra)
+
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), SOURCES.items()))
+        objs = list(ex.map(one, SOURCES.items()))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
            *map(str, objs)]

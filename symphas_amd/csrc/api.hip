@@ -515,6 +515,7 @@ int sps_fd_create(const sps_fd_desc* desc, sps_fd_plan** out) {
   p->nsites = sites_of(*desc);
   p->esize = desc->dtype == SPS_F32 ? 4 : 8;
   p->choose_default_shape();
+  (void)make_energy_params(nullptr, p->d.h, p->energy);  // the default is valid
   int rc = p->allocate();
   if (rc != SPS_OK) {
     sps_fd_destroy(p);
@@ -683,11 +684,11 @@ int sps_fd_reduce(sps_fd_plan* p, double* mass, double* energy) {
   }
   int rc = p->d.dtype == SPS_F32
                ? launch_reduce<float>((const float*)p->buf[p->cur], p->d.dim,
-                                      p->d.n, p->d.h, p->partials, p->sums,
-                                      p->stream)
+                                      p->d.n, p->d.h, p->energy, p->partials,
+                                      p->sums, p->stream)
                : launch_reduce<double>((const double*)p->buf[p->cur],
-                                       p->d.dim, p->d.n, p->d.h, p->partials,
-                                       p->sums, p->stream);
+                                       p->d.dim, p->d.n, p->d.h, p->energy,
+                                       p->partials, p->sums, p->stream);
   if (rc != SPS_OK) return rc;
   double h[2];
   SPS_HIP_TRY(hipMemcpyAsync(h, p->sums, sizeof(h), hipMemcpyDeviceToHost,
@@ -695,6 +696,17 @@ int sps_fd_reduce(sps_fd_plan* p, double* mass, double* energy) {
   SPS_HIP_TRY(hipStreamSynchronize(p->stream));
   if (mass) *mass = h[0];
   if (energy) *energy = h[1];
+  return SPS_OK;
+}
+
+int sps_fd_set_energy(sps_fd_plan* p, const sps_fd_energy* energy) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  EnergyParams ep;
+  SPS_TRY(make_energy_params(energy, p->d.h, ep));
+  p->energy = ep;
   return SPS_OK;
 }
 

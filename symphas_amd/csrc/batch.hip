@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "energy.hpp"
 #include "fused_tile.hpp"
 #include "plan.hpp"
 
@@ -89,24 +90,29 @@ __global__ __launch_bounds__(256) void batch_kernel(
   }
 }
 
-// FP64 sums per instance: one workgroup per instance.
+// FP64 sums per instance: one workgroup per instance, each with the free-
+// energy functional set for it (energy.hpp).
 __global__ __launch_bounds__(256) void batch_reduce(
-    const BatchInst* __restrict__ insts, int parity, double* __restrict__ out) {
+    const BatchInst* __restrict__ insts, const EnergyParams* __restrict__ eps,
+    int parity, double* __restrict__ out) {
   const BatchInst& I = insts[blockIdx.x];
+  const EnergyParams ep = eps[blockIdx.x];
   const float* f = I.buf[parity];
   const int X = I.X, Y = I.Y;
-  const double divh = I.divh;
+  const bool central = ep.central != 0;
   double m = 0.0, e = 0.0;
   for (int y = 0; y < Y; ++y) {
     const float* row = f + (int64_t)y * X;
-    const float* rowy = f + (int64_t)(y + 1 == Y ? 0 : y + 1) * X;
+    const float* rowyn = f + (int64_t)(y + 1 == Y ? 0 : y + 1) * X;
+    const float* rowyp =
+        central ? f + (int64_t)(y == 0 ? Y - 1 : y - 1) * X : row;
     for (int x = threadIdx.x; x < X; x += 256) {
       const double p = row[x];
-      const double p2 = p * p;
-      const double gx = ((double)row[x + 1 == X ? 0 : x + 1] - p) * divh;
-      const double gy = ((double)rowy[x] - p) * divh;
+      const int xp = central ? (x == 0 ? X - 1 : x - 1) : x;
+      const double gx = (double)row[x + 1 == X ? 0 : x + 1] - (double)row[xp];
+      const double gy = (double)rowyn[x] - (double)rowyp[x];
       m += p;
-      e += -0.5 * p2 + 0.25 * (p2 * p2) + 0.5 * (gx * gx + gy * gy);
+      e += landau_density(ep, p, gx * gx + gy * gy);
     }
   }
 #pragma unroll
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void batch_reduce(
   if (threadIdx.x == 0) {
     out[2 * blockIdx.x] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
     out[2 * blockIdx.x + 1] = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) *
-                              (I.h * I.h);
+                              (ep.mean ? 1.0 / ((double)X * Y) : I.h * I.h);
   }
 }
 
@@ -149,6 +155,7 @@ struct sps_fd_batch {
   BatchInst* d_insts = nullptr;
   BatchItem* d_items[4] = {};  // item table per steps-per-launch (strip widths differ)
   double* d_red = nullptr;
+  EnergyParams* d_energy = nullptr;  // per-instance free-energy functional
   int nitems[4] = {};
   int parity = 0;
   // 4 steps x 64-row chunks: best on 256 mixed instances (1.61 T site-updates/s;
@@ -361,10 +368,12 @@ int sps_fd_batch_create(const sps_fd_desc* descs, int count,
   if (hipMalloc(&b->d_insts, sizeof(BatchInst) * count) != hipSuccess ||
       hipMemcpy(b->d_insts, b->h_insts.data(), sizeof(BatchInst) * count,
                 hipMemcpyHostToDevice) != hipSuccess ||
-      hipMalloc(&b->d_red, 2 * sizeof(double) * count) != hipSuccess) {
+      hipMalloc(&b->d_red, 2 * sizeof(double) * count) != hipSuccess ||
+      hipMalloc(&b->d_energy, sizeof(EnergyParams) * count) != hipSuccess) {
     set_error("batch device set-up failed");
     return fail(SPS_E_HIP);
   }
+  if (sps_fd_batch_set_energy(b, nullptr) != SPS_OK) return fail(SPS_E_HIP);
   int rc = b->build_items();
   if (rc != SPS_OK) return fail(rc);
   // persistent grid: as many workgroups as are resident on the device at once
@@ -397,6 +406,7 @@ int sps_fd_batch_destroy(sps_fd_batch* b) {
   for (BatchItem* p : b->d_items)
     if (p) (void)hipFree(p);
   if (b->d_red) (void)hipFree(b->d_red);
+  if (b->d_energy) (void)hipFree(b->d_energy);
   if (b->own_stream) (void)hipStreamDestroy(b->own_stream);
   delete b;
   return SPS_OK;
@@ -475,8 +485,8 @@ int sps_fd_batch_reduce(sps_fd_batch* b, double* mass, double* energy) {
     set_error("batch is NULL");
     return SPS_E_INVALID;
   }
-  batch_reduce<<<b->count, 256, 0, b->stream>>>(b->d_insts, b->parity,
-                                                 b->d_red);
+  batch_reduce<<<b->count, 256, 0, b->stream>>>(b->d_insts, b->d_energy,
+                                                 b->parity, b->d_red);
   SPS_HIP_TRY(hipGetLastError());
   std::vector<double> h(2 * (size_t)b->count);
   SPS_HIP_TRY(hipMemcpyAsync(h.data(), b->d_red, h.size() * sizeof(double),
@@ -486,6 +496,29 @@ int sps_fd_batch_reduce(sps_fd_batch* b, double* mass, double* energy) {
     if (mass) mass[i] = h[2 * i];
     if (energy) energy[i] = h[2 * i + 1];
   }
+  return SPS_OK;
+}
+
+int sps_fd_batch_set_energy(sps_fd_batch* b, const sps_fd_energy* energy) {
+  if (!b) {
+    set_error("batch is NULL");
+    return SPS_E_INVALID;
+  }
+  std::vector<EnergyParams> eps((size_t)b->count);
+  for (int i = 0; i < b->count; ++i) {
+    const int rc = make_energy_params(energy ? &energy[i] : nullptr,
+                                      b->descs[i].h, eps[i]);
+    if (rc != SPS_OK) {
+      const std::string why = last_error();  // set_error overwrites it
+      set_error("batch instance %d: %s", i, why.c_str());
+      return rc;
+    }
+  }
+  // ordered after the batch's earlier reductions on its stream
+  SPS_HIP_TRY(hipMemcpyAsync(b->d_energy, eps.data(),
+                             sizeof(EnergyParams) * eps.size(),
+                             hipMemcpyHostToDevice, b->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(b->stream));
   return SPS_OK;
 }
 

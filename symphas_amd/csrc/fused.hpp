@@ -19,11 +19,13 @@ template <typename T>
 int launch_fused3d(const T* in, T* out, int X, int Y, int Z, int K, int rows,
                    int TZ, const FusedCoef<T>& c, hipStream_t s);
 
-// FP64 mass and free energy of an unpadded periodic field.  partials holds
-// 2*kReduceBlocks doubles; sums receives {mass, energy}.
+// FP64 mass and free energy (functional ep) of an unpadded periodic field.
+// partials holds 2*kReduceBlocks doubles; sums receives {mass, energy}.
 constexpr int kReduceBlocks = 1024;
+struct EnergyParams;
 template <typename T>
 int launch_reduce(const T* f, int dim, const int* n, double h,
-                  double* partials, double* sums, hipStream_t s);
+                  const EnergyParams& ep, double* partials, double* sums,
+                  hipStream_t s);
 
 }  // namespace sps

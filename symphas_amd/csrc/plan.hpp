@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "common.hpp"
+#include "energy.hpp"
 #include "fused.hpp"
 
 namespace sps {
@@ -37,6 +38,7 @@ struct sps_fd_plan {
   void* work = nullptr;               // dframe / derivative output
   double* partials = nullptr;
   double* sums = nullptr;
+  sps::EnergyParams energy{};  // functional of sps_fd_reduce
   bool has_data = false;
   int path = SPS_PATH_REFERENCE;
   sps::Lowered low;

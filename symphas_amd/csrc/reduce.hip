@@ -1,11 +1,13 @@
 // reduce.hip - FP64 mass and free-energy sums of a periodic field.
 //
-// Pass 1: a fixed grid of kReduceBlocks workgroups walks the rows (y, z) in a
-// block-strided order; every site contributes psi (mass) and the Landau
-// density -1/2 psi^2 + 1/4 psi^4 + 1/2 |grad+ psi|^2 (energy), accumulated in
-// FP64 even for FP32 fields.  Waves reduce with xor shuffles, the four waves
-// of a block through LDS.  Pass 2: one workgroup adds the partials in a fixed
-// order.  The result is deterministic for a given field and size.
+// Pass 1: a fixed grid of at most kReduceBlocks workgroups walks the rows
+// (y, z) in a block-strided order; every site contributes psi (mass) and the
+// Landau density of energy.hpp (energy), accumulated in FP64 even for FP32
+// fields.  Waves reduce with xor shuffles, the four waves of a block through
+// LDS.  Pass 2: one workgroup adds the partials in a fixed order and applies
+// the functional's scale (h^D, or 1/sites for the mean).  The result is
+// deterministic for a given field and size.
+#include "energy.hpp"
 #include "fused.hpp"
 
 namespace sps {
@@ -20,31 +22,37 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <typename T>
 __global__ __launch_bounds__(256) void reduce_pass1(const T* __restrict__ f,
                                                     int dim, int X, int Y,
-                                                    int Z, double divh,
+                                                    int Z, EnergyParams ep,
                                                     double* partials) {
   double m = 0.0, e = 0.0;
   const int rows = Y * Z;
   const int64_t XY = (int64_t)X * Y;
+  const bool central = ep.central != 0;
   for (int r = blockIdx.x; r < rows; r += gridDim.x) {
     const int y = r % Y, z = r / Y;
     const int yn = y + 1 == Y ? 0 : y + 1;
     const int zn = z + 1 == Z ? 0 : z + 1;
+    // the backward neighbour is the site itself for the forward difference
+    const int yp = central ? (y == 0 ? Y - 1 : y - 1) : y;
+    const int zp = central ? (z == 0 ? Z - 1 : z - 1) : z;
     const T* row = f + (int64_t)z * XY + (int64_t)y * X;
-    const T* rowy = f + (int64_t)z * XY + (int64_t)yn * X;
-    const T* rowz = f + (int64_t)zn * XY + (int64_t)y * X;
+    const T* rowyn = f + (int64_t)z * XY + (int64_t)yn * X;
+    const T* rowyp = f + (int64_t)z * XY + (int64_t)yp * X;
+    const T* rowzn = f + (int64_t)zn * XY + (int64_t)y * X;
+    const T* rowzp = f + (int64_t)zp * XY + (int64_t)y * X;
     for (int x = threadIdx.x; x < X; x += blockDim.x) {
       const int xn = x + 1 == X ? 0 : x + 1;
+      const int xp = central ? (x == 0 ? X - 1 : x - 1) : x;
       const double p = (double)row[x];
-      const double p2 = p * p;
-      const double gx = ((double)row[xn] - p) * divh;
-      const double gy = ((double)rowy[x] - p) * divh;
+      const double gx = (double)row[xn] - (double)row[xp];
+      const double gy = (double)rowyn[x] - (double)rowyp[x];
       double g2 = gx * gx + gy * gy;
       if (dim == 3) {
-        const double gz = ((double)rowz[x] - p) * divh;
+        const double gz = (double)rowzn[x] - (double)rowzp[x];
         g2 += gz * gz;
       }
       m += p;
-      e += -0.5 * p2 + 0.25 * (p2 * p2) + 0.5 * g2;
+      e += landau_density(ep, p, g2);
     }
   }
   __shared__ double sh[2][4];
@@ -64,7 +72,7 @@ __global__ __launch_bounds__(256) void reduce_pass1(const T* __restrict__ f,
 }
 
 __global__ __launch_bounds__(256) void reduce_pass2(const double* partials,
-                                                    int nb, double hd,
+                                                    int nb, double escale,
                                                     double* sums) {
   double m = 0.0, e = 0.0;
   for (int i = threadIdx.x; i < nb; i += 256) {
@@ -82,7 +90,7 @@ __global__ __launch_bounds__(256) void reduce_pass2(const double* partials,
   __syncthreads();
   if (threadIdx.x == 0) {
     sums[0] = (sh[0][0] + sh[0][1]) + (sh[0][2] + sh[0][3]);
-    sums[1] = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) * hd;
+    sums[1] = ((sh[1][0] + sh[1][1]) + (sh[1][2] + sh[1][3])) * escale;
   }
 }
 
@@ -90,14 +98,15 @@ __global__ __launch_bounds__(256) void reduce_pass2(const double* partials,
 
 template <typename T>
 int launch_reduce(const T* f, int dim, const int* n, double h,
-                  double* partials, double* sums, hipStream_t s) {
+                  const EnergyParams& ep, double* partials, double* sums,
+                  hipStream_t s) {
   const int X = n[0], Y = n[1], Z = dim == 3 ? n[2] : 1;
   const int rows = Y * Z;
   const int nb = rows < kReduceBlocks ? rows : kReduceBlocks;
-  reduce_pass1<T><<<nb, 256, 0, s>>>(f, dim, X, Y, Z, 1.0 / h, partials);
-  double hd = h * h;
-  if (dim == 3) hd *= h;
-  reduce_pass2<<<1, 256, 0, s>>>(partials, nb, hd, sums);
+  reduce_pass1<T><<<nb, 256, 0, s>>>(f, dim, X, Y, Z, ep, partials);
+  const double sites = (double)X * Y * Z;
+  reduce_pass2<<<1, 256, 0, s>>>(partials, nb, energy_scale(ep, dim, h, sites),
+                                 sums);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("reduction launch failed: %s", hipGetErrorString(e));
@@ -107,8 +116,10 @@ int launch_reduce(const T* f, int dim, const int* n, double h,
 }
 
 template int launch_reduce<float>(const float*, int, const int*, double,
-                                  double*, double*, hipStream_t);
+                                  const EnergyParams&, double*, double*,
+                                  hipStream_t);
 template int launch_reduce<double>(const double*, int, const int*, double,
-                                   double*, double*, hipStream_t);
+                                   const EnergyParams&, double*, double*,
+                                   hipStream_t);
 
 }  // namespace sps

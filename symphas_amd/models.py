@@ -127,13 +127,28 @@ def model_b_terms(c1: float = 1.0, c2: float = 1.0, mobility: float = 1.0,
 
 def allen_cahn(shape, h=0.5, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
                path=abi.PATH_AUTO, mobility=1.0, kappa=1.0) -> abi.Desc:
-    return make_desc(shape, stencil, h, dt, allen_cahn_terms(mobility, kappa), dtype, path)
+    d = make_desc(shape, stencil, h, dt, allen_cahn_terms(mobility, kappa), dtype, path)
+    d.landau = (1.0, 1.0, float(kappa))
+    return d
 
 
 def model_b(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
             path=abi.PATH_AUTO, mobility=1.0, kappa=1.0, c1=1.0, c2=1.0) -> abi.Desc:
-    return make_desc(shape, stencil, h, dt, model_b_terms(c1, c2, mobility, kappa),
-                     dtype, path)
+    d = make_desc(shape, stencil, h, dt, model_b_terms(c1, c2, mobility, kappa),
+                  dtype, path)
+    d.landau = (float(c1), float(c2), float(kappa))
+    return d
+
+
+def landau_energy(desc=None, gradient: int = abi.GRAD_FORWARD, mean: bool = False,
+                  c1=None, c2=None, kappa=None) -> "abi.Energy":
+    """The free-energy functional -c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2
+    whose gradient flow a Model B / Allen-Cahn descriptor made here follows
+    (mu = -c1 psi + c2 psi^3 - kappa lap psi); explicit values override the
+    descriptor's, and a descriptor from elsewhere defaults to 1, 1, 1."""
+    base = getattr(desc, "landau", (1.0, 1.0, 1.0)) if desc is not None else (1.0, 1.0, 1.0)
+    return abi.Energy(base[0] if c1 is None else c1, base[1] if c2 is None else c2,
+                      base[2] if kappa is None else kappa, int(gradient), int(bool(mean)))
 
 
 # ------------------------------------------------- provisional variables

@@ -90,6 +90,12 @@ class FDPlan:
         abi.check(self.lib.sps_fd_reduce(self._h, byref(m), byref(e)))
         return m.value, e.value
 
+    def set_energy(self, energy: "abi.Energy | None" = None):
+        """Functional of reduce() (None: c1 = c2 = kappa = 1, forward
+        differences, integral)."""
+        arg = None if energy is None else ctypes.cast(byref(energy), c_void_p)
+        abi.check(self.lib.sps_fd_set_energy(self._h, arg))
+
     def sync(self):
         abi.check(self.lib.sps_fd_sync(self._h))
 
@@ -267,6 +273,15 @@ class FDBatch:
         e = (c_double * n)()
         abi.check(self.lib.sps_fd_batch_reduce(self._h, m, e))
         return np.array(m[:]), np.array(e[:])
+
+    def set_energy(self, energies=None):
+        """Per-instance functionals of reduce(): a sequence of abi.Energy, one
+        per instance (None: the default for every instance)."""
+        if energies is None:
+            abi.check(self.lib.sps_fd_batch_set_energy(self._h, None))
+            return
+        arr = (abi.Energy * len(self.descs))(*energies)
+        abi.check(self.lib.sps_fd_batch_set_energy(self._h, ctypes.cast(arr, c_void_p)))
 
     def sync(self):
         abi.check(self.lib.sps_fd_batch_sync(self._h))

@@ -66,6 +66,36 @@ def test_batch_graph_replay_and_reduce(gpu_lib):
             assert abs(m1[i] - m0[i]) <= 1e-6 * np.abs(fields[i].astype(np.float64)).sum()
 
 
+def test_batch_energy_per_instance(gpu_lib, orc):
+    """Each instance reports the functional its own dynamics descend,
+    -1/2 psi^2 + 1/4 psi^4 + kappa_i/2 |grad+ psi|^2, equal to the oracle's,
+    and that energy decreases under the instance's steps."""
+    descs, fields = _mixed(count=8, seed=11)
+    energies = [models.landau_energy(d) for d in descs]
+    assert len({round(e.kappa, 12) for e in energies}) == len(descs)
+    print("[case] batch energy per instance", flush=True)
+    with FDBatch(descs) as b:
+        b.set_energy(energies)
+        for i, f in enumerate(fields):
+            b.upload(i, f)
+        _, e0 = b.reduce()
+        traj = [e0]
+        for _ in range(3):
+            b.step(40)
+            traj.append(b.reduce()[1])
+        last = [b.download(i) for i in range(len(descs))]
+        b.set_energy([models.landau_energy(d, gradient=abi.GRAD_CENTRAL, mean=True) for d in descs])
+        _, ec = b.reduce()
+    for i, d in enumerate(descs):
+        g = orc.refresh(d, orc.pad(d, last[i].astype(np.float64)))
+        want = orc.energy(d, g, kappa=energies[i].kappa)
+        assert abs(traj[-1][i] - want) <= 1e-12 * abs(want)
+        wantc = orc.energy(d, g, kappa=energies[i].kappa, gradient=1, mean=True)
+        assert abs(ec[i] - wantc) <= 1e-12 * abs(wantc)
+        seq = [t[i] for t in traj]
+        assert all(b_ < a_ for a_, b_ in zip(seq, seq[1:])), (i, seq)
+
+
 def test_batch_against_oracle(gpu_lib, orc):
     descs, fields = _mixed(count=4, seed=9)
     print("[case] batch vs oracle", flush=True)

@@ -339,19 +339,40 @@ def test_fused_unsupported_width_raises(gpu_lib):
 
 # --------------------------------------------------------------- sums
 @pytest.mark.parametrize("dim", [2, 3])
-def test_reductions_match_oracle(gpu_lib, orc, dim):
+@pytest.mark.parametrize("dtype", [F64, F32])
+@pytest.mark.parametrize("energy", [None, (0.7, 1.9, 2.5, 0, 0), (0.7, 1.9, 2.5, 1, 0),
+                                    (1.3, 0.4, 0.6, 1, 1), (1.0, 1.0, 1.0, 0, 1)])
+def test_reductions_match_oracle(gpu_lib, orc, dim, dtype, energy):
+    """Mass and the free-energy functional (c1, c2, kappa, gradient, mean)
+    against the oracle's sequential FP64 sums (whose functional the CPU tests
+    pin by closed forms and summation by parts)."""
     if dim == 2:
-        desc = models.model_b((96, 80), h=0.8, dtype=F64)
+        desc = models.model_b((96, 80), h=0.8, dtype=dtype)
     else:
-        desc = models.model_b((20, 18, 16), h=0.8, stencil=Stencil3d2h(7, 10, 25), dtype=F64)
+        desc = models.model_b((20, 18, 16), h=0.8, stencil=Stencil3d2h(7, 10, 25), dtype=dtype)
     psi = field(desc, seed=16, lo=-0.7, hi=0.9)
-    say("reductions", dim)
+    if dtype == F32:
+        psi = psi.astype(np.float32)
+    say("reductions", dim, dtype, energy)
     with FDPlan(desc) as p:
         p.upload(psi)
+        if energy is not None:
+            p.set_energy(abi.Energy(*energy))
         m, e = p.reduce()
-    g = orc.refresh(desc, orc.pad(desc, psi))
-    assert abs(m - orc.mass(desc, g)) <= 1e-12 * np.abs(psi).sum()
-    assert abs(e - orc.energy(desc, g)) <= 1e-12 * abs(orc.energy(desc, g))
+    g = orc.refresh(desc, orc.pad(desc, psi.astype(np.float64)))
+    args = {} if energy is None else dict(zip(("c1", "c2", "kappa", "gradient", "mean"), energy))
+    want = orc.energy(desc, g, **args)
+    assert abs(m - orc.mass(desc, g)) <= 1e-12 * np.abs(psi.astype(np.float64)).sum()
+    assert abs(e - want) <= 1e-12 * abs(want)
+
+
+def test_energy_rejects_bad_functional(gpu_lib):
+    desc = models.model_b((16, 16), dtype=F64)
+    with FDPlan(desc) as p:
+        for bad in [(1.0, 1.0, float("nan"), 0, 0), (1.0, 1.0, 1.0, 2, 0), (1.0, 1.0, 1.0, 0, 3)]:
+            with pytest.raises(abi.SpsError) as ei:
+                p.set_energy(abi.Energy(*bad))
+            assert ei.value.status == abi.E_INVALID
 
 
 def test_fp32_model_b_mass_drift_and_energy(gpu_lib):

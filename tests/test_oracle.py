@@ -11,6 +11,11 @@ CMake-generated headers, so the oracle is pinned here by:
   * exact recursions: a single Fourier mode of dpsi/dt = lap psi, a constant
     Allen-Cahn field, a constant Model B field;
   * invariants: Model B mass conservation and free-energy decrease;
+  * the free-energy functional: a constant field's closed form (pins the
+    1/2 and 1/4 and h^D), a single Fourier mode's closed form for the
+    forward and the central gradient (pins kappa/2 and the difference
+    scheme), discrete summation by parts against the pinned lap5/lap7, and
+    linearity in (c1, c2, kappa);
   * the 2D ghost strips against modular indexing;
   * the reference's own quirks: lap19's "vz + vz", the 2D 21-point
     bilaplacian's 1/12 normalisation (a quarter of the bilaplacian) and the
@@ -355,6 +360,89 @@ def test_model_b_energy_decreases(orc):
         e = orc.energy(desc, g)
         assert e < e_prev
         e_prev = e
+
+
+# ----------------------------------------------------------- free energy
+def _energy_desc(dim, h):
+    if dim == 2:
+        return models.model_b((24, 20), h=h)
+    return models.model_b((12, 10, 8), h=h, stencil=Stencil3d2h(7, 10, 25))
+
+
+@pytest.mark.parametrize("dim,h", [(2, 0.5), (2, 1.3), (3, 0.5), (3, 1.3)])
+@pytest.mark.parametrize("gradient", [0, 1])
+def test_energy_constant_field_closed_form(orc, dim, h, gradient):
+    """f(c) = -c1/2 c^2 + c2/4 c^4 (no gradient) times N h^D, or its mean."""
+    desc = _energy_desc(dim, h)
+    c, c1, c2 = -0.83, 0.7, 1.9
+    shape = tuple(int(desc.n[a]) for a in reversed(range(dim)))
+    g = orc.refresh(desc, orc.pad(desc, np.full(shape, c)))
+    n = int(np.prod(shape))
+    dens = -c1 * c * c / 2 + c2 * c ** 4 / 4
+    e = orc.energy(desc, g, c1=c1, c2=c2, kappa=3.0, gradient=gradient)
+    assert e == pytest.approx(n * h ** dim * dens, rel=1e-14)
+    em = orc.energy(desc, g, c1=c1, c2=c2, kappa=3.0, gradient=gradient, mean=True)
+    assert em == pytest.approx(dens, rel=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("gradient", [0, 1])
+def test_energy_fourier_mode_closed_form(orc, dim, gradient):
+    """psi = A cos(k.x): sum over sites of |grad psi|^2 is N A^2/2 times
+    sum_a (2 - 2cos(k_a h))/h^2 (forward) or sin^2(k_a h)/h^2 (central)."""
+    h = 0.7
+    desc = _energy_desc(dim, h)
+    n = [int(desc.n[a]) for a in range(dim)]
+    modes = [3, 2, 1][:dim]
+    k = [2 * math.pi * m / (n[a] * h) for a, m in enumerate(modes)]
+    axes = np.meshgrid(*[np.arange(n[a]) * h for a in reversed(range(dim))], indexing="ij")
+    phase = sum(k[a] * axes[dim - 1 - a] for a in range(dim))
+    amp = 0.37
+    psi = amp * np.cos(phase + 0.4)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    if gradient == 0:
+        sym = sum((2 - 2 * math.cos(k[a] * h)) / h ** 2 for a in range(dim))
+    else:
+        sym = sum(math.sin(k[a] * h) ** 2 / h ** 2 for a in range(dim))
+    N = int(np.prod(n))
+    kappa = 0.6
+    want = h ** dim * kappa / 2 * N * amp ** 2 / 2 * sym
+    got = orc.energy(desc, g, c1=0.0, c2=0.0, kappa=kappa, gradient=gradient)
+    assert got == pytest.approx(want, rel=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_energy_summation_by_parts(orc, dim):
+    """Forward differences: sum |grad+ psi|^2 = -sum psi * lap psi on the
+    periodic grid, with lap the (pinned) 5-point / 7-point stencil."""
+    from symphas_amd import inputs
+    h = 0.8
+    desc = _energy_desc(dim, h)
+    shape = tuple(int(desc.n[a]) for a in reversed(range(dim)))
+    psi = inputs.uniform(shape, seed=41, lo=-1.0, hi=1.0)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    lap = orc.unpad(desc, orc.apply(desc, LAP, 0, 0, g))
+    rhs = -np.sum(psi * lap)
+    got = orc.energy(desc, g, c1=0.0, c2=0.0, kappa=2.0) / h ** dim
+    assert got == pytest.approx(rhs, rel=1e-12)
+    # the central gradient is not the one summation by parts pairs with lap5
+    cen = orc.energy(desc, g, c1=0.0, c2=0.0, kappa=2.0, gradient=1) / h ** dim
+    assert abs(cen - rhs) > 1e-3 * abs(rhs)
+
+
+@pytest.mark.parametrize("gradient", [0, 1])
+def test_energy_linear_in_coefficients(orc, gradient):
+    from symphas_amd import inputs
+    desc = _energy_desc(2, 1.1)
+    psi = inputs.uniform((20, 24), seed=42, lo=-1.2, hi=1.2)
+    g = orc.refresh(desc, orc.pad(desc, psi))
+    e = lambda c1, c2, k: orc.energy(desc, g, c1=c1, c2=c2, kappa=k, gradient=gradient)
+    parts = e(1, 0, 0), e(0, 1, 0), e(0, 0, 1)
+    assert e(0.3, 1.7, 2.5) == pytest.approx(0.3 * parts[0] + 1.7 * parts[1] + 2.5 * parts[2],
+                                             rel=1e-13)
+    # the default functional is c1 = c2 = kappa = 1, forward, integral
+    if gradient == 0:
+        assert orc.energy(desc, g) == e(1, 1, 1)
 
 
 def test_mass_is_sequential_sum(orc):
