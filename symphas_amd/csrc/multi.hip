@@ -278,7 +278,48 @@ int launch_multi_r(const T* const* in, T* const* out, int X, int Y, int TY,
   return launch_multi_k<T, K, 2, 0>(in, out, X, Y, TY, vec, c, s);
 }
 
+// Private-segment (scratch) bytes per lane of one kernel instance, as the
+// compiler's resource report records it: nonzero when the instance spills
+// its registers to scratch memory.
+template <typename T, int VEC, int K, int R, int LIN>
+size_t multi_scratch_v() {
+  hipFuncAttributes a{};
+  if (hipFuncGetAttributes(&a, (const void*)fused_multi<T, VEC, K, R, LIN>) !=
+      hipSuccess)
+    return 0;
+  return a.localSizeBytes;
+}
+
+template <typename T, int K, int R, int LIN>
+size_t multi_scratch_k(int vec) {
+  if (vec == 16 / (int)sizeof(T)) return multi_scratch_v<T, 16 / sizeof(T), K, R, LIN>();
+  return multi_scratch_v<T, 8 / sizeof(T), K, R, LIN>();
+}
+
+template <typename T, int K>
+size_t multi_scratch_r(int vec, const MFusedCoef<T>& c) {
+  if (c.radius == 1) return multi_scratch_k<T, K, 1, 0>(vec);
+  if (c.lin == 1) return multi_scratch_k<T, K, 2, 1>(vec);
+  if (c.lin == 2) return multi_scratch_k<T, K, 2, 2>(vec);
+  return multi_scratch_k<T, K, 2, 0>(vec);
+}
+
 }  // namespace
+
+// Scratch bytes per lane of the 2D coupled kernel a launch of this shape
+// would run (0 for a shape it does not build).
+template <typename T>
+size_t fused_multi_scratch(int K, int vec, const MFusedCoef<T>& c) {
+  switch (K) {
+    case 1: return multi_scratch_r<T, 1>(vec, c);
+    case 2: return multi_scratch_r<T, 2>(vec, c);
+    case 3: return multi_scratch_r<T, 3>(vec, c);
+    case 4: return multi_scratch_r<T, 4>(vec, c);
+    default: return 0;
+  }
+}
+template size_t fused_multi_scratch<float>(int, int, const MFusedCoef<float>&);
+template size_t fused_multi_scratch<double>(int, int, const MFusedCoef<double>&);
 
 template <typename T>
 int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
@@ -1278,6 +1319,13 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
             cand_rows.push_back((int)ty);
         }
       }
+      // an instance the compiler had to spill to scratch memory is left
+      // out instead of timed (it runs several times slower than its
+      // neighbours)
+      if (p->d.dim == 2 &&
+          (p->d.dtype == SPS_F32 ? fused_multi_scratch<float>(k, vec, p->fc32)
+                                 : fused_multi_scratch<double>(k, vec, p->fc64)) > 0)
+        continue;
       for (int rows : cand_rows) {
         p->K = k;
         p->TY = rows;

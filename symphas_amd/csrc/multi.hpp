@@ -83,6 +83,10 @@ int launch_fused_multi(const T* const* in, T* const* out, int X, int Y, int K,
 // 3D: radius one K in {1, 2}, radius two K = 1; TY rows per wave in {4, 8},
 // TZ planes per streamed chunk
 bool fused3d_multi_shape_ok(int radius, int K, int rows);
+// Scratch (register spill) bytes per lane of the 2D coupled kernel instance a
+// launch of K steps at vec values per lane would run; 0 when it does not spill.
+template <typename T>
+size_t fused_multi_scratch(int K, int vec, const MFusedCoef<T>& c);
 template <typename T>
 int launch_fused3d_multi(const T* const* in, T* const* out, int X, int Y,
                          int Z, int K, int TY, int TZ, const MFusedCoef<T>& c,

@@ -99,12 +99,16 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
   const std::string key = plan_key(p);
   int ck, cr, cz, cv;
   if (use_cache && cache_lookup(key, &ck, &cr, &cz, &cv)) {
-    p->K = ck;
-    p->TY = cr;
-    if (p->d.dim == 3) p->TZ = cz;
-    if (p->d.dim == 2) p->VEC = cv;
-    p->drop_graphs();
-    return report();
+    // a cached shape is applied only if sps_fd_tune accepts it as an explicit
+    // shape (no zero meaning "keep"); a stale or corrupt entry is ignored and
+    // the candidates are timed again
+    const bool explicit_shape =
+        ck >= 1 && cr >= 1 && (p->d.dim == 2 ? (cv >= 1 && cz == 0) : cz >= 1);
+    if (explicit_shape &&
+        sps_fd_tune(p, ck, cr, p->d.dim == 3 ? cz : 0,
+                    p->d.dim == 2 ? cv : 0) == SPS_OK)
+      return report();
+    set_error("");
   }
   struct Cand {
     int k, rows, planes, vec;

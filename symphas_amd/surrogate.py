@@ -9,11 +9,12 @@ plain finite-difference solver as its data source, fallback and referee.
   copied device to device into torch tensors.
 * Model: nn.UNet (BF16 MFMA kernels), trained with Adam on the per-sample
   relative increment loss mean_i ||pred_i - y_i||^2 / ||y_i - x_i||^2.
-* Inference with fallback (SurrogateSolver.advance): every advance is checked
-  for mass drift and free-energy increase; every `check_every`-th advance (and
-  any advance failing a cheap check) is compared with the plain solver.  A
-  sample whose check exceeds `tol` relative takes the plain solver's state and
-  the plain solver advances it from then on.
+* Inference with fallback (SurrogateSolver.advance): the network advances the
+  samples still on the surrogate, each checked for mass drift and free-energy
+  increase; every `check_every`-th advance (and any advance failing a cheap
+  check) is compared with the plain solver.  A sample whose check exceeds
+  `tol` relative takes the plain solver's state and the plain solver advances
+  it from then on; the network no longer sees it.
 """
 from __future__ import annotations
 
@@ -176,7 +177,19 @@ class AdvanceReport:
 
 
 class SurrogateSolver:
-    """Surrogate k-step advances checked by the plain solver."""
+    """Surrogate k-step advances checked by the plain solver.
+
+    Per advance: the network predicts only the samples still on the surrogate
+    ("live"); a cheap check (mass drift, free-energy increase) runs on those.
+    The plain solver (one batched launch over every sample) runs when some
+    sample is already on the plain path, when a cheap check failed, or every
+    `check_every`-th advance; that one call is both the referee of the live
+    samples and the advance of the fallen-back ones.  A live sample whose
+    relative L2 error against the referee (or whose cheap check) exceeds `tol`
+    takes the referee's state and stays on the plain path.  Once every sample
+    is on the plain path the network is not called again and the remaining
+    advances run as one plain call, so an all-fallback batch costs the plain
+    solver plus the network calls made before the fallbacks."""
 
     def __init__(self, net, batch, n, k=50, tol=1e-3, check_every=4):
         self.net, self.k, self.tol, self.check_every = net, k, tol, check_every
@@ -185,31 +198,49 @@ class SurrogateSolver:
     def advance(self, psi, n_adv):
         rep = AdvanceReport()
         psi = psi.clone()
-        on_plain = torch.zeros(psi.shape[0], dtype=torch.bool, device=psi.device)
+        batch = psi.shape[0]
+        on_plain = np.zeros(batch, dtype=bool)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for a in range(n_adv):
+            live = np.nonzero(~on_plain)[0]
+            if len(live) == 0:
+                # every sample is on the plain path: the rest in one call
+                psi = self.referee.advance(psi, (n_adv - a) * self.k)
+                rep.advances += n_adv - a
+                break
+            all_live = len(live) == batch
+            idx = None if all_live else torch.from_numpy(live).to(psi.device)
+            x = psi if all_live else psi.index_select(0, idx)
             with torch.no_grad():
-                pred = self.net(psi)
+                pred = self.net(x)
             rep.surrogate_calls += 1
-            m0, m1 = psi.double().sum((1, 2)), pred.double().sum((1, 2))
-            mass_bad = (m1 - m0).abs() > self.tol * psi.double().abs().sum((1, 2))
-            f0, f1 = free_energy(psi), free_energy(pred)
+            m0, m1 = x.double().sum((1, 2)), pred.double().sum((1, 2))
+            mass_bad = (m1 - m0).abs() > self.tol * x.double().abs().sum((1, 2))
+            f0, f1 = free_energy(x), free_energy(pred)
             energy_bad = (f1 - f0) > self.tol * f0.abs()
-            cheap_bad = mass_bad | energy_bad
-            need_ref = bool(on_plain.any()) or bool(cheap_bad.any()) or a % self.check_every == 0
+            cheap_bad = (mass_bad | energy_bad).cpu().numpy()
+            need_ref = (not all_live) or bool(cheap_bad.any()) or a % self.check_every == 0
             if need_ref:
+                # one batched plain advance: referee of the live samples and
+                # the advance of those already on the plain path
                 ref = self.referee.advance(psi, self.k)
                 rep.checks += 1
-                err = rel_l2(pred, ref)
+                ref_live = ref if all_live else ref.index_select(0, idx)
+                err = rel_l2(pred, ref_live)
                 rep.max_check_err = max(rep.max_check_err, float(err.max()))
-                bad = (err > self.tol) | cheap_bad
-                newly = bad & ~on_plain
-                rep.fallbacks += int(newly.sum())
-                on_plain |= bad
-                psi = torch.where(on_plain[:, None, None], ref, pred)
+                bad = (err > self.tol).cpu().numpy() | cheap_bad
+                rep.fallbacks += int(bad.sum())
+                on_plain[live[bad]] = True
+                keep = torch.from_numpy(~bad).to(psi.device)[:, None, None]
+                merged = torch.where(keep, pred, ref_live)
+                if all_live:
+                    psi = merged
+                else:
+                    psi = ref
+                    psi.index_copy_(0, idx, merged)
             else:
-                psi = pred
+                psi = pred  # every sample is live here
             rep.advances += 1
         torch.cuda.synchronize()
         rep.seconds = time.perf_counter() - t0
