@@ -77,11 +77,26 @@ class SpsError(RuntimeError):
         self.status = status
 
 
+def _one_hip_runtime():
+    """torch's ROCm wheel ships its own libamdhip64 (and HSA runtime) under the
+    same SONAME as /opt/rocm's.  A library loaded first binds that SONAME for
+    every later library that names it, but torch's libraries name their copy
+    by file, so a torch imported after this library loads a second HIP
+    runtime, and that one finds no device.  Importing torch first, when it is
+    installed, leaves one HIP runtime in the process (torch's), which this
+    library then shares; without torch it uses /opt/rocm's."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load() -> ctypes.CDLL:
     """Load the in-tree HIP library; fails loudly when it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    _one_hip_runtime()
     if not LIB_PATH.exists():
         raise RuntimeError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"

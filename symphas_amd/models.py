@@ -302,6 +302,33 @@ def turing_terms(c=(1.0, 1.0, 1.0, 1.0, 1.0, 1.0)):
     return [psi, rho]
 
 
+def mixed_terms(k=(0.6, 0.8)):
+    """A synthetic two-field model whose pointwise polynomials carry mixed
+    monomials of both fields, and where both fields have a bilaplacian:
+        dpsi = lap(psi^3 - psi + 0.3 psi rho + 0.2 psi^2 rho + 0.15 psi rho^2)
+               - k0 bilap psi + 0.1 psi - 0.05 psi rho^3
+        drho = lap(rho^3 - rho + 0.2 psi^2 + 0.1 psi rho + 0.1 rho^2)
+               - k1 bilap rho - 0.1 rho + 0.05 psi^2 rho^2
+    B_psi and B_rho have total degree 3 and are linear in neither field nor
+    separable; A_psi and A_rho have total degree 4 (it exercises every
+    polynomial class of the fused coupled kernel's general radius-two form)."""
+    k0, k1 = k
+    psi = [MTermSpec(abi.OP_LAP, 1.0, (3, 0)), MTermSpec(abi.OP_LAP, -1.0, (1, 0)),
+           MTermSpec(abi.OP_LAP, 0.3, (1, 1)), MTermSpec(abi.OP_LAP, 0.2, (2, 1)),
+           MTermSpec(abi.OP_LAP, 0.15, (1, 2)), MTermSpec(abi.OP_BILAP, -k0, (1, 0)),
+           MTermSpec(abi.OP_ID, 0.1, (1, 0)), MTermSpec(abi.OP_ID, -0.05, (1, 3))]
+    rho = [MTermSpec(abi.OP_LAP, 1.0, (0, 3)), MTermSpec(abi.OP_LAP, -1.0, (0, 1)),
+           MTermSpec(abi.OP_LAP, 0.2, (2, 0)), MTermSpec(abi.OP_LAP, 0.1, (1, 1)),
+           MTermSpec(abi.OP_LAP, 0.1, (0, 2)), MTermSpec(abi.OP_BILAP, -k1, (0, 1)),
+           MTermSpec(abi.OP_ID, -0.1, (0, 1)), MTermSpec(abi.OP_ID, 0.05, (2, 2))]
+    return [psi, rho]
+
+
+def mixed(shape, h=1.0, dt=0.002, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
+          path=abi.PATH_AUTO, c=(0.6, 0.8)) -> abi.MDesc:
+    return make_mdesc(shape, stencil, h, dt, mixed_terms(c), dtype, path)
+
+
 def model_c(shape, h=1.0, dt=0.01, stencil=Stencil2d2h(5, 6, 13), dtype=abi.F64,
             path=abi.PATH_AUTO, c=(1.0, 1.0, 1.0, 1.0, 1.0)) -> abi.MDesc:
     return make_mdesc(shape, stencil, h, dt, model_c_terms(c), dtype, path)

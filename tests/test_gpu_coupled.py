@@ -46,6 +46,10 @@ CASES = [
     ("gray_scott", Stencil2d2h(5, 6, 13), (48, 36), 0.5, (0.16, 0.08, 0.035, 0.065)),
     ("turing", Stencil2d2h(5, 6, 13), (32, 30), 0.01, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
     ("turing", Stencil3d2h(15, 10, 41), (9, 8, 7), 0.005, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+    # both fields with a bilaplacian and mixed monomials: the general
+    # radius-two kernel with the triangular and dense polynomial classes
+    ("mixed", Stencil2d2h(5, 6, 13), (44, 30), 0.002, (0.6, 0.8)),
+    ("mixed", Stencil3d2h(7, 10, 25), (12, 10, 9), 0.002, (0.6, 0.8)),
 ]
 
 
@@ -220,6 +224,7 @@ CASES3 = [
     ("model_c_swapped", (34, 16, 10), 0.002, (1.1, 0.8, 0.6, 1.3, 0.4)),
     ("gray_scott", (36, 20, 14), 0.5, (0.16, 0.08, 0.035, 0.065)),
     ("turing", (30, 22, 9), 0.01, (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+    ("mixed", (38, 20, 12), 0.002, (0.6, 0.8)),
 ]
 
 
@@ -240,7 +245,7 @@ def test_fused_3d_coupled(gpu_lib, orc, name, n, dt, c):
     d32 = build(name, st, n, dt, c, F32, FUSED)
     f32 = [a.astype(np.float32) for a in fields]
     base = gpu_run(d32, f32, 12, K=1, TY=4)
-    radius_one = not name.startswith("model_c")
+    radius_one = name not in ("model_c", "model_c_swapped", "mixed")
     for K, TY in ((1, 8),) + (((2, 4), (2, 8)) if radius_one else ()):
         got = gpu_run(d32, f32, 12, K=K, TY=TY)
         for g, b in zip(got, base):

@@ -625,6 +625,13 @@ def _rhs_roll(name, psi, rho, h, c, lapf):
         prr = psi * rho * rho
         return (c1 * lapf(psi, h) - prr + c3 * (1 - psi),
                 c2 * lapf(rho, h) + prr - (c3 + c4) * rho)
+    if name == "mixed":
+        k0, k1 = c
+        return (lapf(psi ** 3 - psi + 0.3 * psi * rho + 0.2 * psi * psi * rho
+                     + 0.15 * psi * rho * rho, h) - k0 * lapf(lapf(psi, h), h)
+                + 0.1 * psi - 0.05 * psi * rho ** 3,
+                lapf(rho ** 3 - rho + 0.2 * psi * psi + 0.1 * psi * rho + 0.1 * rho * rho, h)
+                - k1 * lapf(lapf(rho, h), h) - 0.1 * rho + 0.05 * psi * psi * rho * rho)
     c1, c2, c3, c4, c5, c6 = c
     return (c1 * lapf(psi, h) + c2 * (psi + c3 * rho - psi * rho * rho - c4 * psi * rho),
             lapf(rho, h) + c2 * (c5 * rho + c6 * psi + psi * rho * rho + c4 * psi * rho))
@@ -632,7 +639,8 @@ def _rhs_roll(name, psi, rho, h, c, lapf):
 
 @pytest.mark.parametrize("name,c", [("model_c", (1.1, 0.8, 0.6, 1.3, 0.4)),
                                     ("gray_scott", (0.16, 0.08, 0.035, 0.065)),
-                                    ("turing", (1.0, 0.7, 1.2, 0.5, -0.4, 0.9))])
+                                    ("turing", (1.0, 0.7, 1.2, 0.5, -0.4, 0.9)),
+                                    ("mixed", (0.6, 0.8))])
 def test_coupled_rhs_matches_numpy(orc, name, c):
     from symphas_amd import inputs
     n, h = (18, 13), 0.8
@@ -653,6 +661,18 @@ def test_model_c_3d_rhs_matches_numpy(orc):
     d = models.model_c(n, h=h, stencil=Stencil3d2h(7, 10, 25), c=c)
     got = orc.mequation(d, [psi, rho])
     want = _rhs_roll("model_c", psi, rho, h, c, _lap7_roll)
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(g, w, rtol=0, atol=1e-12 * np.abs(w).max())
+
+
+def test_mixed_3d_rhs_matches_numpy(orc):
+    from symphas_amd import inputs
+    n, h, c = (6, 7, 5), 0.9, (0.6, 0.8)
+    psi = inputs.uniform((5, 7, 6), seed=66, lo=-0.8, hi=0.8)
+    rho = inputs.uniform((5, 7, 6), seed=67, lo=-0.8, hi=0.8)
+    d = models.mixed(n, h=h, stencil=Stencil3d2h(7, 10, 25), c=c)
+    got = orc.mequation(d, [psi, rho])
+    want = _rhs_roll("mixed", psi, rho, h, c, _lap7_roll)
     for g, w in zip(got, want):
         np.testing.assert_allclose(g, w, rtol=0, atol=1e-12 * np.abs(w).max())
 
