@@ -303,6 +303,41 @@ int sps_fd_checkpoint_xdr(sps_fd_plan* plan, int index, const double* origin,
 int sps_fd_checkpoint_read_xdr(sps_fd_plan* plan, const void* data,
                                size_t length, int* index);
 
+/* ------------------------------------------------ checkpoint directory */
+
+/* Checkpoint the field as field `id` of a run whose output directory is dir,
+ * the way the reference's IO driver does at a checkpoint index
+ * (save_if_checkpoint → checkpoint_system → save_grid and record_index,
+ * modules-io.h:306-327, modules-io.cpp:29-60):
+ *  - single_file = 1 (the reference's default, params::single_output_file):
+ *    "<dir>/checkpoint/data<id>" holds every checkpoint of the field; the
+ *    first one this process writes truncates the file and carries the full
+ *    header, later ones are appended under the header "<index> @"
+ *    (io/src/writedefines.cpp:160-185, io/src/writegp.cpp:427-465);
+ *  - single_file = 0: "<dir>/checkpoint/data<id>_<index>", full header
+ *    (OUTPUT_CHECKPOINT_FMT / _INDEX_FMT, io/inc/io.h:185-188);
+ *  - "#$<index>" is appended to "<dir>/checkpoint/progress.txt", followed by
+ *    "#$DONE" when done is nonzero (the run reached its stop index).
+ * Directories are created as needed.  The text is sps_fd_checkpoint_text's. */
+int sps_fd_checkpoint_save(sps_fd_plan* plan, const char* dir, int id,
+                           int index, int single_file, int done,
+                           const double* origin);
+/* Resume: read field `id`'s checkpoint for `index` into the plan.  With
+ * single_file the blocks of "<dir>/checkpoint/data<id>" are read in order and
+ * the last one read is kept, continuing while the next block's index does
+ * not pass `index` (read_grid_standardized, io/inc/readdefines.h:340-420);
+ * otherwise "<dir>/checkpoint/data<id>_<index>" is read.  *index_read (if not
+ * NULL) receives the index of the block loaded. */
+int sps_fd_checkpoint_load(sps_fd_plan* plan, const char* dir, int id,
+                           int index, int single_file, int* index_read);
+/* The progress record "<dir>/checkpoint/progress.txt": the last recorded
+ * index (-1 when none), how many indices it lists and whether the run was
+ * recorded as done; a missing record is no error (count 0).  The reference
+ * collects the same list when it loads a backup configuration
+ * (get_completed_indices, conf/src/conf.cpp:117-155).  Host only. */
+int sps_fd_checkpoint_progress(const char* dir, int* last_index, int* count,
+                               int* done);
+
 /* Path the plan runs (sps_path, never AUTO) and the steps fused per launch. */
 int sps_fd_query(const sps_fd_plan* plan, int* path, int* steps_per_launch);
 
@@ -462,6 +497,14 @@ int sps_fdm_checkpoint_xdr(sps_fdm_plan* plan, int field, int index,
                            size_t* length);
 int sps_fdm_checkpoint_read_xdr(sps_fdm_plan* plan, int field,
                                 const void* data, size_t length, int* index);
+/* Every field of a coupled plan checkpointed / resumed as
+ * sps_fd_checkpoint_save / _load do for one, field f being id f; one progress
+ * line per call.  Loading requires every field's block to carry the same
+ * index. */
+int sps_fdm_checkpoint_save(sps_fdm_plan* plan, const char* dir, int index,
+                            int single_file, int done, const double* origin);
+int sps_fdm_checkpoint_load(sps_fdm_plan* plan, const char* dir, int index,
+                            int single_file, int* index_read);
 /* Caller-owned hipStream_t for later work (NULL: the HIP null stream). */
 int sps_fdm_set_stream(sps_fdm_plan* plan, void* hip_stream);
 

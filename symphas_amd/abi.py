@@ -144,6 +144,13 @@ def load() -> ctypes.CDLL:
         "sps_fdm_checkpoint_xdr": (c_int, [P, c_int, c_int, P, c_void_p, c_size_t, POINTER(c_size_t)]),
         "sps_fdm_checkpoint_read_xdr": (c_int, [P, c_int, c_void_p, c_size_t, POINTER(c_int)]),
         "sps_fdm_set_stream": (c_int, [P, P]),
+        "sps_fd_checkpoint_save": (c_int, [P, ctypes.c_char_p, c_int, c_int, c_int, c_int, P]),
+        "sps_fd_checkpoint_load": (c_int, [P, ctypes.c_char_p, c_int, c_int, c_int,
+                                           POINTER(c_int)]),
+        "sps_fd_checkpoint_progress": (c_int, [ctypes.c_char_p, POINTER(c_int), POINTER(c_int),
+                                               POINTER(c_int)]),
+        "sps_fdm_checkpoint_save": (c_int, [P, ctypes.c_char_p, c_int, c_int, c_int, P]),
+        "sps_fdm_checkpoint_load": (c_int, [P, ctypes.c_char_p, c_int, c_int, POINTER(c_int)]),
         "sps_fd_tune": (c_int, [P, c_int, c_int, c_int, c_int]),
         "sps_fd_autotune": (c_int, [P, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int),
                                     POINTER(c_int)]),

@@ -31,6 +31,7 @@ SOURCES = {
     "textio.hip": [],
     "multi_ref.hip": ["-ffp-contract=off"],
     "multi.hip": [],
+    "ckpt.hip": [],
 }
 
 COMMON = [

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "lanes.hpp"
+#include "ckpt.hpp"
 #include "multi.hpp"
 #include "plan.hpp"
 #include "textio.hpp"
@@ -1244,6 +1245,50 @@ int sps_fdm_checkpoint_read_xdr(sps_fdm_plan* p, int field, const void* data,
   SPS_TRY(field_view(p, field, false, &v));
   SPS_TRY(file_xdr_read(v, data, length, index));
   p->has_data = true;
+  return SPS_OK;
+}
+
+int sps_fdm_checkpoint_save(sps_fdm_plan* p, const char* dir, int index,
+                            int single_file, int done, const double* origin) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  // one file per field, the field's index being its id (io/inc/io.h:185-188)
+  for (int f = 0; f < p->d.nfields; ++f) {
+    SPS_TRY(ckpt_write_field(
+        dir, f, index, single_file, [&](int idx, std::string& text) {
+          size_t len = 0;
+          SPS_TRY(sps_fdm_checkpoint_text(p, f, idx, origin, nullptr, 0, &len));
+          text.assign(len, '\0');
+          return sps_fdm_checkpoint_text(p, f, idx, origin, &text[0], len, &len);
+        }));
+  }
+  return ckpt_record(dir, index, done);
+}
+
+int sps_fdm_checkpoint_load(sps_fdm_plan* p, const char* dir, int index,
+                            int single_file, int* index_read) {
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  int first = -1;
+  for (int f = 0; f < p->d.nfields; ++f) {
+    int got = -1;
+    SPS_TRY(ckpt_read_field(dir, f, index, single_file, &got,
+                            [&](const char* text, size_t len, int* idx) {
+                              return sps_fdm_checkpoint_read(p, f, text, len, idx);
+                            }));
+    if (f == 0) {
+      first = got;
+    } else if (got != first) {
+      set_error("checkpoint of field %d is at index %d, field 0 at %d", f, got,
+                first);
+      return SPS_E_INVALID;
+    }
+  }
+  if (index_read) *index_read = first;
   return SPS_OK;
 }
 

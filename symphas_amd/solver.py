@@ -186,6 +186,24 @@ class FDPlan:
                                                       arr.size, byref(idx)))
         return idx.value
 
+    def checkpoint_save(self, directory: str, index: int, field_id: int = 0,
+                        single_file: bool = True, done: bool = False, origin=None):
+        """Checkpoint the field as field `field_id` of the run in `directory`
+        (<directory>/checkpoint/data<id>[_<index>] plus the progress record)."""
+        keep, optr = _origin_arg(origin)
+        abi.check(self.lib.sps_fd_checkpoint_save(self._h, str(directory).encode(), int(field_id),
+                                                  int(index), int(bool(single_file)),
+                                                  int(bool(done)), optr))
+
+    def checkpoint_load(self, directory: str, index: int, field_id: int = 0,
+                        single_file: bool = True) -> int:
+        """Resume from the checkpoint of `index` (single file: the last block
+        whose index does not pass it); returns the index loaded."""
+        got = c_int(-1)
+        abi.check(self.lib.sps_fd_checkpoint_load(self._h, str(directory).encode(), int(field_id),
+                                                  int(index), int(bool(single_file)), byref(got)))
+        return got.value
+
     def tune(self, steps_per_launch: int = 0, rows_per_tile: int = 0, planes_per_chunk: int = 0,
              values_per_lane: int = 0):
         abi.check(self.lib.sps_fd_tune(self._h, steps_per_launch, rows_per_tile, planes_per_chunk,
@@ -214,6 +232,16 @@ def _text_cap(desc, per_value: int, per_row: int, per_slice_header: int = 0) -> 
     rows = sites // n[0]
     slices = n[2] if desc.dim == 3 else 1
     return per_value * sites + per_row * rows + slices * per_slice_header + 4096
+
+
+def checkpoint_progress(directory: str):
+    """(last index or -1, number of indices recorded, done) of the run's
+    progress record <directory>/checkpoint/progress.txt."""
+    lib = abi.load()
+    last, count, done = c_int(), c_int(), c_int()
+    abi.check(lib.sps_fd_checkpoint_progress(str(directory).encode(), byref(last), byref(count),
+                                             byref(done)))
+    return last.value, count.value, bool(done.value)
 
 
 def device_count() -> int:
@@ -391,6 +419,20 @@ class MultiPlan:
                                                        arr.ctypes.data_as(c_void_p), arr.size,
                                                        byref(idx)))
         return idx.value
+
+    def checkpoint_save(self, directory: str, index: int, single_file: bool = True,
+                        done: bool = False, origin=None):
+        """Every field's checkpoint (field f is id f) and one progress line."""
+        keep, optr = _origin_arg(origin)
+        abi.check(self.lib.sps_fdm_checkpoint_save(self._h, str(directory).encode(), int(index),
+                                                   int(bool(single_file)), int(bool(done)), optr))
+
+    def checkpoint_load(self, directory: str, index: int, single_file: bool = True) -> int:
+        got = c_int(-1)
+        abi.check(self.lib.sps_fdm_checkpoint_load(self._h, str(directory).encode(), int(index),
+                                                   int(bool(single_file)), byref(got)))
+        return got.value
+
     def step(self, nsteps: int = 1):
         abi.check(self.lib.sps_fdm_step(self._h, int(nsteps)))
 
