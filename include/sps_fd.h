@@ -252,6 +252,74 @@ int sps_fd_device_field(sps_fd_plan* plan, void** dptr);
  * states without a host round trip. */
 int sps_fd_copy_device(sps_fd_plan* plan, void* dev, int direction);
 
+/* ---------------------------------------------------------- regional grid
+ * SolverFT on a regional grid: the reference's SolverSystemFDwSD
+ * (sol/inc/solversystem.h:62-77), the system SolverFT associates second
+ * (examples/solvers/solverft.h:220), whose field lives on a RegionalGrid
+ * (datatypes/inc/grid.h:2038-2157).  The field is stored on a box of the
+ * periodic domain with a boundary layer of SPS_BOUNDARY_DEPTH cells per
+ * side; every site outside the box reads as `empty`.  A step evaluates the
+ * right-hand side and forward Euler over the box interior only, in the
+ * reference order (FP64 results equal the CPU SolverFT's on the box), so
+ * its cost follows the box, not the domain.
+ *
+ * Update before each right-hand side (SolverSystemFDwSD::update,
+ * update_regional_system, sol/inc/boundarysystem.h:437-479):
+ *   - the first update fixes cutoff = min(psi) + cutoff_eps and
+ *     empty = min(psi), then places a box of resize_factor times the active
+ *     box's dims (at most the domain), centred on it
+ *     (resize_adjust_region, datatypes/inc/gridfunctions.h:2194-2238);
+ *   - later, once time / resize_time reaches the next resize count, the box
+ *     is re-centred on the active box with its dims kept (fixed_resize = 1,
+ *     adjust_region, :2166-2176), or resized as above unless every axis
+ *     changes by at most dims_relative_eps relatively (fixed_resize = 0);
+ *   - along an axis the box spans, its layer is the periodic image of its
+ *     interior (regional_update_boundary,
+ *     datatypes/inc/boundaryupdatedata.h:1398-1433); along other axes the
+ *     layer keeps the values it was given when the box last moved.
+ * The active box along an axis is the shortest periodic interval covering
+ * every coordinate of an interior site whose value exceeds the cutoff.
+ * When the box moves, a site of the new box takes the old interior's value
+ * at the same domain site, or empty.  Box interiors have at least 3 sites
+ * per axis (or the whole axis).
+ * Departures from the reference: its periodic search grows a box from the
+ * first active site until an empty slab (get_view_resized_periodic,
+ * gridfunctions.h:2011-2163), so a second cluster beyond an empty slab falls
+ * outside the box and is set to empty; the covering interval here keeps
+ * every active site.  Provisional variables are not supported. */
+typedef struct {
+  double cutoff_eps;        /* params::regional_resize_cutoff_eps (1e-6) */
+  double resize_factor;     /* params::regional_resize_factor (2.0), >= 1 */
+  double resize_time;       /* params::regional_resize_time (3.0), > 0 */
+  int32_t fixed_resize;     /* params::regional_resize_is_fixed (1) */
+  int32_t reserved;
+  double dims_relative_eps; /* REGIONAL_GRID_RELATIVE_DIMS_EPS (0.05) */
+} sps_fd_regional;         /* defaults: sol/src/boundarysystem.cpp:24-27,
+                              sol/inc/boundarysystem.h:137 */
+
+typedef struct {
+  int32_t origin[3]; /* domain coordinate of the box's first interior site */
+  int32_t dims[3];   /* box dims with the layer; all 0 when no site is active */
+  double empty, cutoff, time;
+  int32_t index, reserved;
+} sps_fdr_region_info;
+
+typedef struct sps_fdr_plan sps_fdr_plan;
+
+/* params NULL: the reference's defaults.  Replaces constructing a
+ * SolverSystemFDwSD (solversystem.h:67-71). */
+int sps_fdr_create(const sps_fd_desc* desc, const sps_fd_regional* params,
+                   sps_fdr_plan** out);
+int sps_fdr_destroy(sps_fdr_plan* plan);
+/* The whole domain's field; the box becomes the whole domain and the update
+ * state (cutoff, empty, resize count, time) restarts. */
+int sps_fdr_upload(sps_fdr_plan* plan, const void* host, int layout);
+/* nsteps x (update, right-hand side, Euler) (modules-models.h:436-467). */
+int sps_fdr_step(sps_fdr_plan* plan, int nsteps);
+/* The whole domain's field: box interior values inside, empty outside. */
+int sps_fdr_download(sps_fdr_plan* plan, void* host, int layout);
+int sps_fdr_region(const sps_fdr_plan* plan, sps_fdr_region_info* out);
+
 /* ------------------------------------------------------------- checkpoints */
 
 /* The reference's GNU checkpoint file of the current field, formatted on the

@@ -62,6 +62,8 @@ def load():
         ("orc_plot_text", c_int64, [P, P, c_int, P, P, c_size_t]),
         ("orc_xdr_write", c_int64, [P, P, c_int, P, P, c_size_t]),
         ("orc_xdr_read", c_int, [P, ctypes.c_char_p, c_size_t, P, POINTER(c_int)]),
+        ("orc_cover_interval", c_int, [P, c_int, POINTER(c_int), POINTER(c_int)]),
+        ("orc_regional_run", c_int, [P, P, P, c_int, P, c_int]),
     ]:
         fn = getattr(lib, name)
         fn.restype = res
@@ -243,3 +245,41 @@ def mcheck(mdesc):
 
 def check(desc):
     return load().orc_check(_d(desc))
+
+
+class Regional(ctypes.Structure):
+    """orc_regional (same layout as the product's sps_fd_regional)."""
+    _fields_ = [("cutoff_eps", c_double), ("resize_factor", c_double),
+                ("resize_time", c_double), ("fixed_resize", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dims_relative_eps", c_double)]
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_int32 * 3), ("dims", ctypes.c_int32 * 3),
+                ("empty", c_double), ("cutoff", c_double), ("time", c_double)]
+
+
+def regional_defaults():
+    """The reference's defaults (sol/src/boundarysystem.cpp:24-27,
+    sol/inc/boundarysystem.h:137)."""
+    return Regional(1e-6, 2.0, 3.0, 1, 0, 0.05)
+
+
+def cover_interval(occ):
+    """(start, length) of the shortest periodic interval covering the set
+    coordinates of occ."""
+    a = np.ascontiguousarray(np.asarray(occ, dtype=np.uint8))
+    s, n = c_int(0), c_int(0)
+    load().orc_cover_interval(_p(a), int(a.size), ctypes.byref(s), ctypes.byref(n))
+    return s.value, n.value
+
+
+def regional_run(desc, params, interior, nsteps, nthreads=1):
+    """(whole-domain interior field, Region) after nsteps on a regional grid."""
+    f = np.ascontiguousarray(interior, dtype=np.float64).copy()
+    st = Region()
+    rc = load().orc_regional_run(_d(desc), ctypes.cast(ctypes.byref(params), c_void_p), _p(f),
+                                 int(nsteps), ctypes.cast(ctypes.byref(st), c_void_p), nthreads)
+    if rc:
+        raise RuntimeError(f"orc_regional_run failed ({rc})")
+    return f, st

@@ -212,6 +212,45 @@ int orcm_equation(const orcm_desc* d, const double* fields, double* dframe,
  * refresh of each field). */
 int orcm_run(const orcm_desc* d, double* fields, int nsteps, int nthreads);
 
+/* ------------------------------------------------------------ regional grid
+ * SolverFT on a regional grid (SolverSystemFDwSD, sol/inc/solversystem.h:62-77):
+ * the field is stored only on a box of the periodic domain plus a boundary
+ * layer of ORC_DEPTH cells, every other site reads as `empty`, and the box
+ * follows the sites above a cutoff.  sps_oracle_regional.c states the rules.
+ * Same layout as the product's sps_fd_regional. */
+typedef struct {
+  double cutoff_eps;        /* params::regional_resize_cutoff_eps (1e-6) */
+  double resize_factor;     /* params::regional_resize_factor (2.0), >= 1 */
+  double resize_time;       /* params::regional_resize_time (3.0), > 0 */
+  int32_t fixed_resize;     /* params::regional_resize_is_fixed (1) */
+  int32_t reserved;
+  double dims_relative_eps; /* REGIONAL_GRID_RELATIVE_DIMS_EPS (0.05) */
+} orc_regional;
+
+/* The box after a run: origin = domain coordinate of the first interior site
+ * per axis, dims = interior + 2*ORC_DEPTH (all 0 when no site was above the
+ * cutoff), and the value outside, the cutoff and the time reached. */
+typedef struct {
+  int32_t origin[3];
+  int32_t dims[3];
+  double empty, cutoff, time;
+} orc_region;
+
+/* Shortest periodic interval of [0, n) covering every coordinate with
+ * occ[i] != 0: the complement of the largest run of zeros on the circle (the
+ * lowest-starting one among equals).  Returns 0 (start 0, length 0) when no
+ * coordinate is set, else 1. */
+int orc_cover_interval(const unsigned char* occ, int n, int* start,
+                       int* length);
+
+/* nsteps of (update, equation, Euler) from a whole-domain field (interior
+ * layout, n values, replaced by the result: box interior values inside the
+ * box, `empty` outside).  The box starts as the whole domain; the first
+ * update fixes cutoff = min + eps and empty = min and resizes.  Provisional
+ * variables are not supported. */
+int orc_regional_run(const orc_desc* d, const orc_regional* rp, double* field,
+                     int nsteps, orc_region* st, int nthreads);
+
 /* SplitMix64 stream -> 53-bit uniform doubles in [lo, hi). */
 void orc_fill_uniform(double* out, size_t count, uint64_t seed, double lo,
                       double hi);

@@ -60,6 +60,23 @@ class MDesc(ctypes.Structure):
                 ("terms", (MTerm * MAX_MTERMS) * MAX_FIELDS)]
 
 
+class Regional(ctypes.Structure):
+    # sps_fd_regional; defaults are the reference's parameters
+    _fields_ = [("cutoff_eps", c_double), ("resize_factor", c_double),
+                ("resize_time", c_double), ("fixed_resize", c_int32),
+                ("reserved", c_int32), ("dims_relative_eps", c_double)]
+
+
+def regional_defaults() -> "Regional":
+    return Regional(1e-6, 2.0, 3.0, 1, 0, 0.05)
+
+
+class RegionInfo(ctypes.Structure):
+    _fields_ = [("origin", c_int32 * 3), ("dims", c_int32 * 3), ("empty", c_double),
+                ("cutoff", c_double), ("time", c_double), ("index", c_int32),
+                ("reserved", c_int32)]
+
+
 class Energy(ctypes.Structure):
     # -c1/2 psi^2 + c2/4 psi^4 + kappa/2 |grad psi|^2 (sps_fd_energy)
     _fields_ = [("c1", c_double), ("c2", c_double), ("kappa", c_double),
@@ -164,6 +181,12 @@ def load() -> ctypes.CDLL:
         "sps_fd_batch_sync": (c_int, [P]),
         "sps_fd_batch_tune": (c_int, [P, c_int, c_int]),
         "sps_fd_batch_set_stream": (c_int, [P, P]),
+        "sps_fdr_create": (c_int, [POINTER(Desc), P, POINTER(P)]),
+        "sps_fdr_destroy": (c_int, [P]),
+        "sps_fdr_upload": (c_int, [P, P, c_int]),
+        "sps_fdr_step": (c_int, [P, c_int]),
+        "sps_fdr_download": (c_int, [P, P, c_int]),
+        "sps_fdr_region": (c_int, [P, POINTER(RegionInfo)]),
     }
     I, F = c_int, ctypes.c_float
     sig.update({

@@ -32,6 +32,7 @@ SOURCES = {
     "multi_ref.hip": ["-ffp-contract=off"],
     "multi.hip": [],
     "ckpt.hip": [],
+    "regional.hip": [],
 }
 
 COMMON = [

@@ -256,7 +256,7 @@ void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c) {
 // Reference-order parameters for one term list: group 0 is the right-hand
 // side, group k >= 1 the definition of var(k).
 template <typename T>
-void make_ref_params(const sps_fd_desc& d, RefParams<T>& p, int group = 0) {
+void make_ref_params(const sps_fd_desc& d, RefParams<T>& p, int group) {
   std::memset(&p, 0, sizeof(p));
   for (int a = 0; a < 3; ++a) p.n[a] = a < d.dim ? d.n[a] : 1;
   p.L = d.lap_points;
@@ -291,6 +291,10 @@ void make_ref_params(const sps_fd_desc& d, RefParams<T>& p, int group = 0) {
   }
 }
 
+template void make_ref_params<float>(const sps_fd_desc&, RefParams<float>&,
+                                     int);
+template void make_ref_params<double>(const sps_fd_desc&, RefParams<double>&,
+                                      int);
 template void make_fused_coef<float>(const sps_fd_desc&, const Lowered&,
                                      FusedCoef<float>&);
 template void make_fused_coef<double>(const sps_fd_desc&, const Lowered&,

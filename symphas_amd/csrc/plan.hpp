@@ -14,6 +14,10 @@ int validate_desc(const sps_fd_desc* d);
 Lowered lower(const sps_fd_desc& d);
 template <typename T>
 void make_fused_coef(const sps_fd_desc& d, const Lowered& lw, FusedCoef<T>& c);
+// Reference-order parameters of term group `group` (0: the right-hand side,
+// k >= 1: the definition of var(k)).
+template <typename T>
+void make_ref_params(const sps_fd_desc& d, RefParams<T>& p, int group = 0);
 void interior_to_padded(const sps_fd_desc& d, const char* src, char* dst,
                         size_t es, bool fill_ghosts);
 void padded_to_interior(const sps_fd_desc& d, const char* src, char* dst,

@@ -146,7 +146,11 @@ __device__ __forceinline__ int wrap(int c, int n) {
 // The value one site takes: mode 0 psi + dt*rhs, 1 rhs, 2 one derivative of
 // psi.  `in` may point to global memory or to LDS (flat addressing); the
 // arithmetic is the same either way.
-template <typename T, int DIM>
+// BOX: p.n are the dims of a box whose boundary layer (BOUNDARY_DEPTH cells
+// per side) holds the neighbours of its interior sites, so offsets are taken
+// as they are; otherwise the array is the unpadded periodic field and
+// coordinates wrap.
+template <typename T, int DIM, bool BOX = false>
 __device__ __forceinline__ T site_value(const RefParams<T>& p, const T* in,
                                         int x, int y, int z) {
   const int64_t nx = p.n[0], nxy = (int64_t)p.n[0] * p.n[1];
@@ -154,9 +158,15 @@ __device__ __forceinline__ T site_value(const RefParams<T>& p, const T* in,
   w.f = in;
 #pragma unroll
   for (int d = -2; d <= 2; ++d) {
-    w.xo[d + 2] = wrap(x + d, p.n[0]);
-    w.yo[d + 2] = (int64_t)wrap(y + d, p.n[1]) * nx;
-    w.zo[d + 2] = DIM == 3 ? (int64_t)wrap(z + d, p.n[2]) * nxy : 0;
+    if (BOX) {
+      w.xo[d + 2] = x + d;
+      w.yo[d + 2] = (int64_t)(y + d) * nx;
+      w.zo[d + 2] = DIM == 3 ? (int64_t)(z + d) * nxy : 0;
+    } else {
+      w.xo[d + 2] = wrap(x + d, p.n[0]);
+      w.yo[d + 2] = (int64_t)wrap(y + d, p.n[1]) * nx;
+      w.zo[d + 2] = DIM == 3 ? (int64_t)wrap(z + d, p.n[2]) * nxy : 0;
+    }
   }
   const int64_t idx = x + nx * y + nxy * z;
   if (p.mode == 2)
@@ -198,7 +208,47 @@ __global__ __launch_bounds__(256) void ref_kernel(const T* __restrict__ in,
   out[idx] = site_value<T, DIM>(p, in, x, y, z);
 }
 
+// One thread per site of a box (a regional grid's stored region): interior
+// sites take psi + dt*rhs, boundary-layer sites keep their value.
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void ref_box_kernel(const T* __restrict__ in,
+                                                      T* __restrict__ out,
+                                                      RefParams<T> p) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= p.n[0] || y >= p.n[1]) return;
+  const int64_t idx = x + (int64_t)p.n[0] * (y + (int64_t)p.n[1] * z);
+  constexpr int b = SPS_BOUNDARY_DEPTH;
+  const bool interior = x >= b && x < p.n[0] - b && y >= b && y < p.n[1] - b &&
+                        (DIM == 2 || (z >= b && z < p.n[2] - b));
+  out[idx] = interior ? site_value<T, DIM, true>(p, in, x, y, z) : in[idx];
+}
+
 }  // namespace
+
+template <typename T>
+int launch_ref_box(const T* in, T* out, const RefParams<T>& p, int dim,
+                   hipStream_t stream) {
+  dim3 block(64, 4, 1);
+  dim3 grid((p.n[0] + 63) / 64, (p.n[1] + 3) / 4, dim == 3 ? p.n[2] : 1);
+  if (dim == 2)
+    ref_box_kernel<T, 2><<<grid, block, 0, stream>>>(in, out, p);
+  else
+    ref_box_kernel<T, 3><<<grid, block, 0, stream>>>(in, out, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("regional step launch failed: %s", hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+template int launch_ref_box<float>(const float*, float*,
+                                   const RefParams<float>&, int, hipStream_t);
+template int launch_ref_box<double>(const double*, double*,
+                                    const RefParams<double>&, int,
+                                    hipStream_t);
 
 template <typename T>
 int launch_ref(const T* in, T* out, const RefParams<T>& p, int dim,

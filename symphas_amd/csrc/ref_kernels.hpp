@@ -10,4 +10,11 @@ template <typename T>
 int launch_ref(const T* in, T* out, const RefParams<T>& p, int dim,
                hipStream_t stream);
 
+// Mode 0 over a box of dims p.n whose boundary layer (SPS_BOUNDARY_DEPTH
+// cells per side) supplies the neighbours: interior sites get psi + dt*rhs,
+// layer sites are copied.
+template <typename T>
+int launch_ref_box(const T* in, T* out, const RefParams<T>& p, int dim,
+                   hipStream_t stream);
+
 }  // namespace sps

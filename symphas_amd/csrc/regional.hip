@@ -1,0 +1,695 @@
+// regional.hip - SolverFT on a regional grid (SolverSystemFDwSD), one GPU.
+//
+// The field lives on a box of the periodic domain with a boundary layer of
+// SPS_BOUNDARY_DEPTH cells per side (RegionalGrid, datatypes/inc/grid.h:
+// 2038-2157); every site outside the box reads as the grid's `empty` value.
+// A step evaluates the right-hand side and forward Euler over the box
+// interior with the reference-order kernel in its box form, so its FP64
+// arithmetic is that of the CPU SolverFT.  The box follows the sites above a
+// cutoff (SolverSystemFDwSD::update, sol/inc/solversystem.h:62-77;
+// update_regional_system, sol/inc/boundarysystem.h:437-479):
+//   - first update: cutoff = min + eps, empty = min, resize by the padding
+//     factor (resize_adjust_region, datatypes/inc/gridfunctions.h:2194-2238);
+//   - then, whenever time / resize_time reaches the next resize count:
+//     re-centre a box of the same dims (adjust_region, :2166-2176) or, when
+//     the resize is not fixed, resize with the relative dims tolerance.
+// The active box along an axis is the shortest periodic interval covering
+// every coordinate of an interior site above the cutoff.  The device marks
+// the occupied coordinates of each axis; the host finds the interval.
+// Along an axis the box spans, its layer is the periodic image of its
+// interior, refreshed before every right-hand side
+// (regional_update_boundary, datatypes/inc/boundaryupdatedata.h:1398-1433);
+// along the others the layer keeps what it was given when the box moved.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "plan.hpp"
+#include "ref_kernels.hpp"
+
+namespace sps {
+namespace {
+
+constexpr int kB = SPS_BOUNDARY_DEPTH;
+constexpr int kMinBlocks = 256;
+
+__device__ __forceinline__ int modn(int v, int n) {
+  int r = v % n;
+  return r < 0 ? r + n : r;
+}
+
+struct BoxGeom {
+  int n[3];  // domain interior counts (1 beyond dim)
+  int o[3];  // domain coordinate of the first interior site of the box
+  int b[3];  // box dims including the layer (1 beyond dim)
+};
+
+// Per-block minimum of the box interior, FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void box_min_pass1(const T* __restrict__ box,
+                                                     BoxGeom g, int dim,
+                                                     double* partial) {
+  const int mx = g.b[0] - 2 * kB, my = g.b[1] - 2 * kB;
+  const int mz = dim == 3 ? g.b[2] - 2 * kB : 1;
+  const int zo = dim == 3 ? kB : 0;
+  const int rows = my * mz;
+  double m = INFINITY;
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int y = r % my, z = r / my;
+    const T* row = box + ((int64_t)(z + zo) * g.b[1] + (y + kB)) * g.b[0] + kB;
+    for (int x = threadIdx.x; x < mx; x += blockDim.x) m = fmin(m, (double)row[x]);
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+}
+
+__global__ __launch_bounds__(256) void box_min_pass2(const double* partial,
+                                                     int nb, double* out) {
+  double m = INFINITY;
+  for (int i = threadIdx.x; i < nb; i += 256) m = fmin(m, partial[i]);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) m = fmin(m, __shfl_xor(m, o, 64));
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+}
+
+__global__ __launch_bounds__(256) void fill_int(int* p, int count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// occ[a][coordinate] = 1 for every axis-a coordinate of an interior site
+// whose value exceeds the cutoff (occ holds n0 + n1 + n2 flags).  Every
+// writer of a flag writes the same 1.
+template <typename T>
+__global__ __launch_bounds__(256) void box_occupancy(const T* __restrict__ box,
+                                                     BoxGeom g, int dim,
+                                                     double cutoff, int* occ) {
+  const int mx = g.b[0] - 2 * kB, my = g.b[1] - 2 * kB;
+  const int mz = dim == 3 ? g.b[2] - 2 * kB : 1;
+  const int zo = dim == 3 ? kB : 0;
+  const int64_t total = (int64_t)mx * my * mz;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int x = (int)(i % mx);
+    const int64_t r = i / mx;
+    const int y = (int)(r % my), z = (int)(r / my);
+    const T v = box[((int64_t)(z + zo) * g.b[1] + (y + kB)) * g.b[0] + x + kB];
+    if ((double)v > cutoff) {
+      occ[modn(g.o[0] + x, g.n[0])] = 1;
+      occ[g.n[0] + modn(g.o[1] + y, g.n[1])] = 1;
+      if (dim == 3) occ[g.n[0] + g.n[1] + modn(g.o[2] + z, g.n[2])] = 1;
+    }
+  }
+}
+
+// New box from the old: a new site takes the old interior value at the same
+// domain site, else `empty`.
+template <typename T>
+__global__ __launch_bounds__(256) void box_move(const T* __restrict__ old,
+                                                BoxGeom go, T* __restrict__ nw,
+                                                BoxGeom gn, int dim, T empty) {
+  const int64_t total = (int64_t)gn.b[0] * gn.b[1] * gn.b[2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    int l[3];
+    l[0] = (int)(i % gn.b[0]);
+    const int64_t r = i / gn.b[0];
+    l[1] = (int)(r % gn.b[1]);
+    l[2] = (int)(r / gn.b[1]);
+    int64_t src = 0, stride = 1;
+    bool inside = true;
+    for (int a = 0; a < dim; ++a) {
+      const int gc = modn(gn.o[a] + l[a] - kB, gn.n[a]);
+      const int k = modn(gc - go.o[a], gn.n[a]);
+      if (k >= go.b[a] - 2 * kB) inside = false;
+      src += (int64_t)(k + kB) * stride;
+      stride *= go.b[a];
+    }
+    nw[i] = inside ? old[src] : empty;
+  }
+}
+
+// Periodic layer along axis `ax` of a box that spans it.
+template <typename T>
+__global__ __launch_bounds__(256) void box_refresh_axis(T* __restrict__ box,
+                                                        BoxGeom g, int ax) {
+  const int64_t total = (int64_t)g.b[0] * g.b[1] * g.b[2];
+  int64_t sa = 1;
+  for (int j = 0; j < ax; ++j) sa *= g.b[j];
+  const int64_t shift = (int64_t)g.n[ax] * sa;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int la = (int)((i / sa) % g.b[ax]);
+    if (la < kB)
+      box[i] = box[i + shift];
+    else if (la >= g.b[ax] - kB)
+      box[i] = box[i - shift];
+  }
+}
+
+// Whole-domain field (interior layout) <-> box.
+template <typename T>
+__global__ __launch_bounds__(256) void box_gather(const T* __restrict__ box,
+                                                  BoxGeom g, int dim,
+                                                  int has_box, T empty,
+                                                  T* __restrict__ out) {
+  const int64_t total = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    int c[3];
+    c[0] = (int)(i % g.n[0]);
+    const int64_t r = i / g.n[0];
+    c[1] = (int)(r % g.n[1]);
+    c[2] = (int)(r / g.n[1]);
+    int64_t src = 0, stride = 1;
+    bool inside = has_box != 0;
+    for (int a = 0; a < dim; ++a) {
+      const int k = modn(c[a] - g.o[a], g.n[a]);
+      if (k >= g.b[a] - 2 * kB) inside = false;
+      src += (int64_t)(k + kB) * stride;
+      stride *= g.b[a];
+    }
+    out[i] = inside ? box[src] : empty;
+  }
+}
+
+// The padded layout of the whole domain (the box spanning every axis at
+// origin 0) from an interior-layout field.
+template <typename T>
+__global__ __launch_bounds__(256) void box_scatter_full(
+    const T* __restrict__ in, BoxGeom g, int dim, T* __restrict__ box) {
+  const int64_t total = (int64_t)g.b[0] * g.b[1] * g.b[2];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    int l[3];
+    l[0] = (int)(i % g.b[0]);
+    const int64_t r = i / g.b[0];
+    l[1] = (int)(r % g.b[1]);
+    l[2] = (int)(r / g.b[1]);
+    int64_t src = 0, stride = 1;
+    for (int a = 0; a < dim; ++a) {
+      src += (int64_t)modn(l[a] - kB, g.n[a]) * stride;
+      stride *= g.n[a];
+    }
+    box[i] = in[src];
+  }
+}
+
+int grid_for(int64_t work) {
+  const int64_t blocks = (work + 255) / 256;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 8192));
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s launch failed: %s", what, hipGetErrorString(e));
+    return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+// Shortest periodic interval of [0, n) covering the set flags: the
+// complement of the largest run of clear flags on the circle, the
+// lowest-starting among equals.  false when no flag is set.
+bool cover_interval(const int* occ, int n, int& start, int& length) {
+  int first = -1;
+  for (int i = 0; i < n; ++i)
+    if (occ[i]) {
+      first = i;
+      break;
+    }
+  if (first < 0) {
+    start = 0;
+    length = 0;
+    return false;
+  }
+  int best = 0, best_start = 0, run = 0, run_start = 0;
+  for (int k = 1; k <= n; ++k) {
+    const int i = (first + k) % n;
+    if (!occ[i]) {
+      if (run == 0) run_start = i;
+      ++run;
+      continue;
+    }
+    if (run > best || (run > 0 && run == best && run_start < best_start)) {
+      best = run;
+      best_start = run_start;
+    }
+    run = 0;
+  }
+  if (best == 0) {
+    start = 0;
+    length = n;
+  } else {
+    start = (best_start + best) % n;
+    length = n - best;
+  }
+  return true;
+}
+
+int mod_host(int64_t v, int n) {
+  const int64_t r = v % n;
+  return (int)(r < 0 ? r + n : r);
+}
+
+}  // namespace
+}  // namespace sps
+
+struct sps_fdr_plan {
+  sps_fd_desc d{};
+  sps_fd_regional rp{};
+  hipStream_t stream = nullptr;
+  size_t esize = 0, nsites = 0, cap = 0;  // cap: sites of the whole padded box
+  void* box[2] = {nullptr, nullptr};
+  int cur = 0;
+  void* staging = nullptr;  // whole-domain field, interior layout
+  int* occ = nullptr;       // n0 + n1 + n2 flags
+  double* partial = nullptr;
+  double* minval = nullptr;
+  sps::RefParams<float> rp32;
+  sps::RefParams<double> rp64;
+  int o[3] = {0, 0, 0}, b[3] = {1, 1, 1};
+  bool has_data = false, has_box = true;
+  int next_resize = 0, index = 0;
+  double empty = 0.0, cutoff = 0.0, time = 0.0;
+
+  sps::BoxGeom geom() const {
+    sps::BoxGeom g;
+    for (int a = 0; a < 3; ++a) {
+      g.n[a] = a < d.dim ? d.n[a] : 1;
+      g.o[a] = a < d.dim ? o[a] : 0;
+      g.b[a] = a < d.dim ? b[a] : 1;
+    }
+    return g;
+  }
+
+  int64_t box_sites() const {
+    return (int64_t)b[0] * b[1] * (d.dim == 3 ? b[2] : 1);
+  }
+
+  int find_min(double& out) {
+    const sps::BoxGeom g = geom();
+    const int rows = (b[1] - 2 * sps::kB) * (d.dim == 3 ? b[2] - 2 * sps::kB : 1);
+    const int nb = std::max(1, std::min(rows, sps::kMinBlocks));
+    if (d.dtype == SPS_F32)
+      sps::box_min_pass1<float><<<nb, 256, 0, stream>>>((const float*)box[cur], g,
+                                                         d.dim, partial);
+    else
+      sps::box_min_pass1<double><<<nb, 256, 0, stream>>>((const double*)box[cur],
+                                                          g, d.dim, partial);
+    sps::box_min_pass2<<<1, 256, 0, stream>>>(partial, nb, minval);
+    SPS_TRY(sps::check_launch("minimum"));
+    SPS_HIP_TRY(hipMemcpyAsync(&out, minval, sizeof(double),
+                               hipMemcpyDeviceToHost, stream));
+    SPS_HIP_TRY(hipStreamSynchronize(stream));
+    return SPS_OK;
+  }
+
+  // Active box (start s, length L) per axis; found = false when no interior
+  // site exceeds the cutoff.
+  int active_box(int* s, int* L, bool& found) {
+    const sps::BoxGeom g = geom();
+    const int nflags = d.n[0] + d.n[1] + (d.dim == 3 ? d.n[2] : 0);
+    sps::fill_int<<<(nflags + 255) / 256, 256, 0, stream>>>(occ, nflags);
+    const int64_t work = (int64_t)(b[0] - 2 * sps::kB) * (b[1] - 2 * sps::kB) *
+                         (d.dim == 3 ? b[2] - 2 * sps::kB : 1);
+    if (d.dtype == SPS_F32)
+      sps::box_occupancy<float><<<sps::grid_for(work), 256, 0, stream>>>(
+          (const float*)box[cur], g, d.dim, cutoff, occ);
+    else
+      sps::box_occupancy<double><<<sps::grid_for(work), 256, 0, stream>>>(
+          (const double*)box[cur], g, d.dim, cutoff, occ);
+    SPS_TRY(sps::check_launch("occupancy"));
+    std::vector<int> h((size_t)nflags);
+    SPS_HIP_TRY(hipMemcpyAsync(h.data(), occ, sizeof(int) * nflags,
+                               hipMemcpyDeviceToHost, stream));
+    SPS_HIP_TRY(hipStreamSynchronize(stream));
+    found = true;
+    int off = 0;
+    for (int a = 0; a < d.dim; ++a) {
+      if (!sps::cover_interval(h.data() + off, d.n[a], s[a], L[a])) found = false;
+      off += d.n[a];
+    }
+    return SPS_OK;
+  }
+
+  // New box from the active box: resize (resize_adjust_region) or re-centre
+  // with the same dims (adjust_region).  Integer division truncates toward
+  // zero, as in the reference.
+  void place(const int* s, const int* L, bool resize, double rel_eps, int* no,
+             int* nb) const {
+    constexpr int B2 = 2 * sps::kB;
+    bool same = true;
+    for (int a = 0; a < d.dim; ++a) {
+      const int dims = L[a] + B2;
+      if (!resize) {
+        no[a] = sps::mod_host((int64_t)s[a] - (b[a] - dims) / 2, d.n[a]);
+        nb[a] = b[a];
+        continue;
+      }
+      const int grid = d.n[a] + B2;
+      int dim0 = std::min(grid, (int)(dims * rp.resize_factor));
+      if (dim0 < 3 + B2) dim0 = std::min(grid, 3 + B2);
+      no[a] = sps::mod_host((int64_t)s[a] - (dim0 - dims) / 2, d.n[a]);
+      nb[a] = dim0;
+      if (std::fabs((double)(dim0 - b[a])) / (double)b[a] > rel_eps) same = false;
+    }
+    if (resize && same) {
+      for (int a = 0; a < d.dim; ++a) {
+        no[a] = sps::mod_host((int64_t)s[a] - (b[a] - (L[a] + B2)) / 2, d.n[a]);
+        nb[a] = b[a];
+      }
+    }
+    for (int a = d.dim; a < 3; ++a) {
+      no[a] = 0;
+      nb[a] = 1;
+    }
+  }
+
+  int move_to(const int* no, const int* nb) {
+    const sps::BoxGeom go = geom();
+    for (int a = 0; a < 3; ++a) {
+      o[a] = no[a];
+      b[a] = nb[a];
+    }
+    const sps::BoxGeom gn = geom();
+    const int nxt = cur ^ 1;
+    if (d.dtype == SPS_F32)
+      sps::box_move<float><<<sps::grid_for(box_sites()), 256, 0, stream>>>(
+          (const float*)box[cur], go, (float*)box[nxt], gn, d.dim, (float)empty);
+    else
+      sps::box_move<double><<<sps::grid_for(box_sites()), 256, 0, stream>>>(
+          (const double*)box[cur], go, (double*)box[nxt], gn, d.dim, empty);
+    SPS_TRY(sps::check_launch("box move"));
+    cur = nxt;
+    return SPS_OK;
+  }
+
+  // SolverSystemFDwSD::update(index, time): resize when due, then the
+  // periodic layer of every axis the box spans.
+  int update() {
+    if (next_resize == 0) {
+      double mn = 0.0;
+      SPS_TRY(find_min(mn));
+      cutoff = mn + rp.cutoff_eps;
+      empty = mn;
+      int s[3], L[3];
+      bool found = false;
+      SPS_TRY(active_box(s, L, found));
+      if (!found) {
+        has_box = false;
+      } else {
+        int no[3], nb[3];
+        place(s, L, true, 0.0, no, nb);
+        SPS_TRY(move_to(no, nb));
+      }
+      next_resize = 1;
+    } else if (has_box) {
+      const int current = (int)(time / rp.resize_time);
+      if (current >= next_resize) {
+        int s[3], L[3];
+        bool found = false;
+        SPS_TRY(active_box(s, L, found));
+        if (found) {
+          int no[3], nb[3];
+          place(s, L, rp.fixed_resize == 0, rp.dims_relative_eps, no, nb);
+          SPS_TRY(move_to(no, nb));
+        }
+        next_resize = current + 1;
+      }
+    }
+    if (!has_box) return SPS_OK;
+    const sps::BoxGeom g = geom();
+    for (int a = 0; a < d.dim; ++a) {
+      if (b[a] - 2 * sps::kB != d.n[a]) continue;
+      if (d.dtype == SPS_F32)
+        sps::box_refresh_axis<float><<<sps::grid_for(box_sites()), 256, 0, stream>>>(
+            (float*)box[cur], g, a);
+      else
+        sps::box_refresh_axis<double><<<sps::grid_for(box_sites()), 256, 0, stream>>>(
+            (double*)box[cur], g, a);
+      SPS_TRY(sps::check_launch("layer refresh"));
+    }
+    return SPS_OK;
+  }
+
+  int step_once() {
+    SPS_TRY(update());
+    if (has_box) {
+      const int nxt = cur ^ 1;
+      int rc;
+      if (d.dtype == SPS_F32) {
+        for (int a = 0; a < 3; ++a) rp32.n[a] = a < d.dim ? b[a] : 1;
+        rc = sps::launch_ref_box<float>((const float*)box[cur], (float*)box[nxt],
+                                        rp32, d.dim, stream);
+      } else {
+        for (int a = 0; a < 3; ++a) rp64.n[a] = a < d.dim ? b[a] : 1;
+        rc = sps::launch_ref_box<double>((const double*)box[cur],
+                                         (double*)box[nxt], rp64, d.dim, stream);
+      }
+      if (rc != SPS_OK) return rc;
+      cur = nxt;
+    }
+    ++index;
+    time += d.dt;  // run_model's time += dt (modules-models.h:489)
+    return SPS_OK;
+  }
+
+  void release() {
+    for (int i = 0; i < 2; ++i)
+      if (box[i]) (void)hipFree(box[i]);
+    if (staging) (void)hipFree(staging);
+    if (occ) (void)hipFree(occ);
+    if (partial) (void)hipFree(partial);
+    if (minval) (void)hipFree(minval);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int validate_regional(const sps_fd_regional& r) {
+  if (!std::isfinite(r.cutoff_eps) || !(r.resize_factor >= 1.0) ||
+      !std::isfinite(r.resize_factor) || !(r.resize_time > 0) ||
+      !std::isfinite(r.resize_time) || !(r.dims_relative_eps >= 0) ||
+      !std::isfinite(r.dims_relative_eps) ||
+      (r.fixed_resize != 0 && r.fixed_resize != 1)) {
+    sps::set_error("regional parameters: cutoff_eps finite, resize_factor >= 1, "
+                   "resize_time > 0, dims_relative_eps >= 0, fixed_resize 0 or 1");
+    return SPS_E_INVALID;
+  }
+  return SPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sps_fdr_create(const sps_fd_desc* desc, const sps_fd_regional* params,
+                   sps_fdr_plan** out) {
+  using namespace sps;
+  if (!out) {
+    set_error("out is NULL");
+    return SPS_E_INVALID;
+  }
+  *out = nullptr;
+  SPS_TRY(validate_desc(desc));
+  if (desc->nprov != 0) {
+    set_error("regional plans take no provisional variables");
+    return SPS_E_UNSUPPORTED;
+  }
+  sps_fd_regional rp{1e-6, 2.0, 3.0, 1, 0, 0.05};
+  if (params) rp = *params;
+  SPS_TRY(validate_regional(rp));
+  if (sps_fd_device_count() < 1) {
+    set_error("no HIP device: this library runs the step on an MI355X only");
+    return SPS_E_HIP;
+  }
+  sps_fdr_plan* p = new (std::nothrow) sps_fdr_plan();
+  if (!p) {
+    set_error("host allocation failed");
+    return SPS_E_NOMEM;
+  }
+  p->d = *desc;
+  p->rp = rp;
+  p->esize = desc->dtype == SPS_F32 ? 4 : 8;
+  p->nsites = 1;
+  p->cap = 1;
+  for (int a = 0; a < desc->dim; ++a) {
+    p->nsites *= (size_t)desc->n[a];
+    p->cap *= (size_t)desc->n[a] + 2 * SPS_BOUNDARY_DEPTH;
+  }
+  make_ref_params(*desc, p->rp32);
+  make_ref_params(*desc, p->rp64);
+  int rc = SPS_OK;
+  auto alloc = [&](void** ptr, size_t bytes) {
+    if (rc != SPS_OK) return;
+    const hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+      *ptr = nullptr;
+      set_error("hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+      rc = SPS_E_NOMEM;
+    }
+  };
+  const hipError_t se = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+  if (se != hipSuccess) {
+    p->stream = nullptr;
+    set_error("hipStreamCreate failed: %s", hipGetErrorString(se));
+    rc = SPS_E_HIP;
+  }
+  alloc(&p->box[0], p->cap * p->esize);
+  alloc(&p->box[1], p->cap * p->esize);
+  alloc(&p->staging, p->nsites * p->esize);
+  alloc((void**)&p->occ,
+        sizeof(int) * (desc->n[0] + desc->n[1] + (desc->dim == 3 ? desc->n[2] : 0)));
+  alloc((void**)&p->partial, sizeof(double) * kMinBlocks);
+  alloc((void**)&p->minval, sizeof(double));
+  if (rc != SPS_OK) {
+    p->release();
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return SPS_OK;
+}
+
+int sps_fdr_destroy(sps_fdr_plan* p) {
+  if (!p) return SPS_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  p->release();
+  delete p;
+  return SPS_OK;
+}
+
+int sps_fdr_upload(sps_fdr_plan* p, const void* host, int layout) {
+  using namespace sps;
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (layout != SPS_LAYOUT_INTERIOR && layout != SPS_LAYOUT_PADDED) {
+    set_error("layout must be SPS_LAYOUT_INTERIOR or SPS_LAYOUT_PADDED");
+    return SPS_E_INVALID;
+  }
+  std::vector<char> tmp;
+  const void* src = host;
+  if (layout == SPS_LAYOUT_PADDED) {
+    tmp.resize(p->nsites * p->esize);
+    padded_to_interior(p->d, (const char*)host, tmp.data(), p->esize);
+    src = tmp.data();
+  }
+  SPS_HIP_TRY(hipMemcpyAsync(p->staging, src, p->nsites * p->esize,
+                             hipMemcpyHostToDevice, p->stream));
+  // the box starts as the whole domain at origin 0 (the padded layout)
+  for (int a = 0; a < 3; ++a) {
+    p->o[a] = 0;
+    p->b[a] = a < p->d.dim ? p->d.n[a] + 2 * SPS_BOUNDARY_DEPTH : 1;
+  }
+  p->cur = 0;
+  const BoxGeom g = p->geom();
+  if (p->d.dtype == SPS_F32)
+    box_scatter_full<float><<<grid_for(p->box_sites()), 256, 0, p->stream>>>(
+        (const float*)p->staging, g, p->d.dim, (float*)p->box[0]);
+  else
+    box_scatter_full<double><<<grid_for(p->box_sites()), 256, 0, p->stream>>>(
+        (const double*)p->staging, g, p->d.dim, (double*)p->box[0]);
+  SPS_TRY(check_launch("upload scatter"));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  p->has_data = true;
+  p->has_box = true;
+  p->next_resize = 0;
+  p->index = 0;
+  p->time = 0.0;
+  p->empty = 0.0;
+  p->cutoff = 0.0;
+  return SPS_OK;
+}
+
+int sps_fdr_step(sps_fdr_plan* p, int nsteps) {
+  using namespace sps;
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  if (nsteps < 0) {
+    set_error("nsteps must be >= 0");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  for (int i = 0; i < nsteps; ++i) SPS_TRY(p->step_once());
+  return SPS_OK;
+}
+
+int sps_fdr_download(sps_fdr_plan* p, void* host, int layout) {
+  using namespace sps;
+  if (!p || !host) {
+    set_error("plan or host pointer is NULL");
+    return SPS_E_INVALID;
+  }
+  if (layout != SPS_LAYOUT_INTERIOR && layout != SPS_LAYOUT_PADDED) {
+    set_error("layout must be SPS_LAYOUT_INTERIOR or SPS_LAYOUT_PADDED");
+    return SPS_E_INVALID;
+  }
+  if (!p->has_data) {
+    set_error("no field uploaded");
+    return SPS_E_STATE;
+  }
+  const BoxGeom g = p->geom();
+  // before the first update every site is still stored
+  const int has_box = p->has_box ? 1 : 0;
+  if (p->d.dtype == SPS_F32)
+    box_gather<float><<<grid_for((int64_t)p->nsites), 256, 0, p->stream>>>(
+        (const float*)p->box[p->cur], g, p->d.dim, has_box, (float)p->empty,
+        (float*)p->staging);
+  else
+    box_gather<double><<<grid_for((int64_t)p->nsites), 256, 0, p->stream>>>(
+        (const double*)p->box[p->cur], g, p->d.dim, has_box, p->empty,
+        (double*)p->staging);
+  SPS_TRY(check_launch("download gather"));
+  if (layout == SPS_LAYOUT_INTERIOR) {
+    SPS_HIP_TRY(hipMemcpyAsync(host, p->staging, p->nsites * p->esize,
+                               hipMemcpyDeviceToHost, p->stream));
+    SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+    return SPS_OK;
+  }
+  std::vector<char> tmp(p->nsites * p->esize);
+  SPS_HIP_TRY(hipMemcpyAsync(tmp.data(), p->staging, tmp.size(),
+                             hipMemcpyDeviceToHost, p->stream));
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  interior_to_padded(p->d, tmp.data(), (char*)host, p->esize, true);
+  return SPS_OK;
+}
+
+int sps_fdr_region(const sps_fdr_plan* p, sps_fdr_region_info* out) {
+  using namespace sps;
+  if (!p || !out) {
+    set_error("plan or out is NULL");
+    return SPS_E_INVALID;
+  }
+  for (int a = 0; a < 3; ++a) {
+    out->origin[a] = p->o[a];
+    out->dims[a] = p->has_box ? p->b[a] : 0;
+  }
+  out->empty = p->empty;
+  out->cutoff = p->cutoff;
+  out->time = p->time;
+  out->index = p->index;
+  out->reserved = 0;
+  return SPS_OK;
+}
+
+}  // extern "C"

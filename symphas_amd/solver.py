@@ -458,3 +458,57 @@ class MultiPlan:
 
     def set_stream(self, stream_handle: int):
         abi.check(self.lib.sps_fdm_set_stream(self._h, c_void_p(stream_handle)))
+
+
+class RegionalPlan:
+    """A field on a regional grid (SolverSystemFDwSD): stored and stepped only
+    on a box that follows the sites above a cutoff (sps_fdr_* in
+    include/sps_fd.h).  Upload and download take the whole domain."""
+
+    def __init__(self, desc: abi.Desc, params: "abi.Regional | None" = None):
+        self.lib = abi.load()
+        self.desc = desc
+        self.dtype = _np_dtype(desc)
+        h = c_void_p()
+        pp = byref(params) if params is not None else None
+        abi.check(self.lib.sps_fdr_create(byref(desc), pp, byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.sps_fdr_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, layout):
+        return interior_shape(self.desc) if layout == abi.LAYOUT_INTERIOR else padded_shape(self.desc)
+
+    def upload(self, field: np.ndarray, layout: int = abi.LAYOUT_INTERIOR):
+        arr = np.ascontiguousarray(field, dtype=self.dtype)
+        if arr.shape != self._shape(layout):
+            raise ValueError(f"field shape {arr.shape} != {self._shape(layout)}")
+        abi.check(self.lib.sps_fdr_upload(self._h, arr.ctypes.data_as(c_void_p), layout))
+
+    def download(self, layout: int = abi.LAYOUT_INTERIOR) -> np.ndarray:
+        out = np.empty(self._shape(layout), dtype=self.dtype)
+        abi.check(self.lib.sps_fdr_download(self._h, out.ctypes.data_as(c_void_p), layout))
+        return out
+
+    def step(self, nsteps: int = 1):
+        abi.check(self.lib.sps_fdr_step(self._h, int(nsteps)))
+
+    def region(self) -> "abi.RegionInfo":
+        info = abi.RegionInfo()
+        abi.check(self.lib.sps_fdr_region(self._h, byref(info)))
+        return info

@@ -1,0 +1,140 @@
+"""Regional-grid plans (SolverSystemFDwSD) on the GPU against the oracle.
+
+The FP64 plan runs the reference-order kernel on the box, so its field and
+its box (origin, dims, empty, cutoff, time) must equal the oracle's bit for
+bit, through resizes in both modes.  A box that spans the domain must also
+equal the plain plan.  Argument checks run before any HIP call and are
+tested without a GPU.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from symphas_amd import abi, inputs, models
+from symphas_amd.solver import FDPlan, RegionalPlan, interior_shape
+
+
+def _blob(shape_np, lo, size, value=0.5, noise_seed=None):
+    psi = np.zeros(shape_np)
+    sl = tuple(slice(l, l + s) for l, s in zip(reversed(lo), reversed(size)))
+    psi[sl] = value
+    if noise_seed is not None:
+        psi[sl] += 0.05 * inputs.uniform(psi[sl].shape, seed=noise_seed, lo=0.0, hi=1.0)
+    return psi
+
+
+def _params(fixed=1, factor=2.0, every=3.0, eps=1e-6, rel=0.05):
+    return abi.Regional(eps, factor, every, fixed, 0, rel)
+
+
+def _oparams(p):
+    return po.Regional(p.cutoff_eps, p.resize_factor, p.resize_time, p.fixed_resize, 0,
+                       p.dims_relative_eps)
+
+
+def _run_gpu(desc, params, psi, steps):
+    with RegionalPlan(desc, params) as rp:
+        rp.upload(psi)
+        rp.step(steps)
+        return rp.download(), rp.region()
+
+
+def _same_region(info, st, dim):
+    assert list(info.origin)[:dim] == list(st.origin)[:dim]
+    assert list(info.dims)[:dim] == list(st.dims)[:dim]
+    assert info.empty == st.empty and info.cutoff == st.cutoff and info.time == st.time
+
+
+def test_regional_rejects_bad_arguments_before_any_hip_call():
+    lib = abi.load()
+    h = ctypes.c_void_p()
+    d = models.model_bb((16, 16), dtype=abi.F64)
+    assert lib.sps_fdr_create(ctypes.byref(d), None, ctypes.byref(h)) == abi.E_UNSUPPORTED
+    d = models.model_b((16, 16), dtype=abi.F64)
+    for bad in (_params(factor=0.5), _params(every=0.0), _params(fixed=2),
+                _params(rel=-1.0), _params(eps=float("nan"))):
+        assert lib.sps_fdr_create(ctypes.byref(d), ctypes.byref(bad), ctypes.byref(h)) == abi.E_INVALID
+        assert b"regional parameters" in lib.sps_fd_last_error()
+
+
+CASES = [
+    # name, desc factory, field factory, params, steps
+    ("mb2_fixed", lambda: models.model_b((64, 48), dt=0.01, dtype=abi.F64),
+     lambda d: _blob(interior_shape(d), (20, 14), (12, 8), noise_seed=1), _params(every=0.04), 60),
+    ("mb2_resize", lambda: models.model_b((64, 48), dt=0.01, dtype=abi.F64),
+     lambda d: _blob(interior_shape(d), (50, 30), (10, 12), noise_seed=2),
+     _params(fixed=0, every=0.05), 80),
+    ("ac2_wrapping", lambda: models.allen_cahn((40, 36), h=0.5, dt=0.01, dtype=abi.F64),
+     lambda d: _blob(interior_shape(d), (34, 30), (9, 9), value=0.8, noise_seed=3) - 0.8,
+     _params(fixed=0, every=0.02, factor=1.5), 50),
+    ("mb3_resize", lambda: models.model_b((28, 24, 20), dt=0.005, dtype=abi.F64,
+                                          stencil=models.Stencil3d2h(7, 10, 25)),
+     lambda d: _blob(interior_shape(d), (6, 8, 5), (7, 6, 5), noise_seed=4),
+     _params(fixed=0, every=0.02), 24),
+    ("mb2_lap9_bilap17", lambda: models.model_b((40, 40), dt=0.005, dtype=abi.F64,
+                                                stencil=models.Stencil2d2h(9, 6, 17)),
+     lambda d: _blob(interior_shape(d), (10, 12), (8, 6), noise_seed=5), _params(every=0.03), 40),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,mk,field,params,steps", CASES, ids=[c[0] for c in CASES])
+def test_regional_fp64_bitwise_vs_oracle(gpu_lib, name, mk, field, params, steps):
+    d = mk()
+    psi = field(d)
+    print(f"regional {name}: n {[d.n[a] for a in range(d.dim)]} steps {steps}", flush=True)
+    got, info = _run_gpu(d, params, psi, steps)
+    want, st = po.regional_run(d, _oparams(params), psi, steps)
+    _same_region(info, st, d.dim)
+    assert info.index == steps
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.gpu
+def test_regional_spanning_box_equals_plain_plan(gpu_lib):
+    d = models.model_b((48, 40), dt=0.01, dtype=abi.F64, path=abi.PATH_REFERENCE)
+    psi = inputs.uniform(interior_shape(d), seed=21)
+    got, info = _run_gpu(d, _params(factor=40.0, every=0.05), psi, 100)
+    assert [info.dims[a] for a in range(2)] == [54, 46]
+    with FDPlan(d) as p:
+        p.upload(psi)
+        p.step(100)
+        want = p.download()
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_regional_fp32_against_fp64_oracle(gpu_lib):
+    d64 = models.model_b((64, 48), dt=0.01, dtype=abi.F64)
+    d32 = models.model_b((64, 48), dt=0.01, dtype=abi.F32)
+    psi = _blob(interior_shape(d64), (20, 14), (12, 8), noise_seed=1).astype(np.float32)
+    params = _params(every=0.1)
+    got, info = _run_gpu(d32, params, psi, 50)
+    want, st = po.regional_run(d64, _oparams(params), psi.astype(np.float64), 50)
+    assert list(info.dims)[:2] == list(st.dims)[:2]
+    assert list(info.origin)[:2] == list(st.origin)[:2]
+    err = np.linalg.norm(got.astype(np.float64) - want) / np.linalg.norm(want)
+    assert err < 1e-6, err
+
+
+@pytest.mark.gpu
+def test_regional_padded_download_and_no_active_site(gpu_lib):
+    d = models.model_b((32, 24), dt=0.01, dtype=abi.F64)
+    psi = _blob(interior_shape(d), (4, 6), (8, 5), noise_seed=7)
+    with RegionalPlan(d) as rp:
+        rp.upload(psi)
+        assert np.array_equal(rp.download(), psi)  # before the first update
+        rp.step(10)
+        inner = rp.download()
+        padded = rp.download(abi.LAYOUT_PADDED)
+    assert np.array_equal(padded, np.pad(inner, 3, mode="wrap"))
+    flat = np.full(interior_shape(d), 0.25)
+    with RegionalPlan(d) as rp:
+        rp.upload(flat)
+        rp.step(5)
+        info = rp.region()
+        out = rp.download()
+    assert list(info.dims) == [0, 0, 0]
+    assert np.all(out == 0.25)
