@@ -228,6 +228,36 @@ int sps_fd_reduce(sps_fd_plan* plan, double* mass, double* energy);
  * Coefficients must be finite. */
 int sps_fd_set_energy(sps_fd_plan* plan, const sps_fd_energy* energy);
 
+/* Allocation-free variant of sps_fd_reduce for a caller-owned device field.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It enqueues the same two kernels as
+ * sps_fd_reduce, in the same summation order, so its sums are bit-identical
+ * to sps_fd_reduce's for the same field and functional; it calls no
+ * hipMalloc, hipFree or other allocating routine.
+ *   dim, dtype, n, h: the field's shape, element type and grid spacing;
+ *   energy:     the functional (NULL: c1 = c2 = kappa = 1, forward
+ *               differences, the integral h^D * sum f);
+ *   field:      n[0]*n[1](*n[2]) values of dtype on the current device,
+ *               interior layout, x fastest;
+ *   sums:       2 doubles of device memory on the current device, which
+ *               receive {mass, energy};
+ *   hip_stream: the stream the kernels are enqueued on (NULL: the null
+ *               stream).  The call returns without waiting for them.
+ * Limits: 1 <= n[a] <= 2^20 and at most SPS_REDUCE_DEVICE_MAX_SITES sites;
+ * anything beyond is rejected with SPS_E_INVALID before any HIP call.
+ * Scratch: the pass-1 partial sums live in a fixed 16 KiB array of the
+ * library's device code (2 x 1024 doubles; pass 1 never uses more than 1024
+ * workgroups, whatever the field size), one copy per GPU, held until the
+ * process ends and shared by every call on that GPU.  Calls that overlap on
+ * one GPU (on different streams or host threads) race on it.  The first call
+ * on a device looks up that device's copy (hipGetSymbolAddress); make that
+ * first call outside stream capture. */
+#define SPS_REDUCE_DEVICE_MAX_SITES 2147483647
+int sps_fd_reduce_device(int dim, int dtype, const int32_t* n, double h,
+                         const sps_fd_energy* energy, const void* field,
+                         double* sums, void* hip_stream);
+
 /* Wait for all work enqueued on the plan. */
 int sps_fd_sync(sps_fd_plan* plan);
 

@@ -134,6 +134,7 @@ def load() -> ctypes.CDLL:
         "sps_fd_apply": (c_int, [P, c_int, c_int, c_int, P, c_int]),
         "sps_fd_reduce": (c_int, [P, POINTER(c_double), POINTER(c_double)]),
         "sps_fd_set_energy": (c_int, [P, P]),
+        "sps_fd_reduce_device": (c_int, [c_int, c_int, POINTER(c_int32), c_double, P, P, P, P]),
         "sps_fd_batch_set_energy": (c_int, [P, P]),
         "sps_fd_sync": (c_int, [P]),
         "sps_fd_device_field": (c_int, [P, POINTER(P)]),

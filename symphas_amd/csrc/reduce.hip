@@ -7,6 +7,8 @@
 // LDS.  Pass 2: one workgroup adds the partials in a fixed order and applies
 // the functional's scale (h^D, or 1/sites for the mean).  The result is
 // deterministic for a given field and size.
+#include <mutex>
+
 #include "energy.hpp"
 #include "fused.hpp"
 
@@ -123,3 +125,95 @@ template int launch_reduce<double>(const double*, int, const int*, double,
                                    hipStream_t);
 
 }  // namespace sps
+
+// ---------------------------------------------------- allocation-free sums
+// sps_fd_reduce_device runs the two passes above with their partials in a
+// file-scope device array instead of plan memory, so it calls no allocator.
+// Pass 1 never uses more than kReduceBlocks workgroups, so this fixed 16 KiB
+// holds the partials of a field of any size.  Each GPU holds its own copy of
+// the array; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+namespace sps {
+namespace {
+
+__device__ __align__(256) double scratch[2 * kReduceBlocks];
+
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+double* scratch_dev[kScratchDevices] = {};
+
+// Address of the current device's scratch array, looked up once per device.
+int scratch_address(double** out) {
+  int dev = -1;
+  SPS_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) {
+    set_error("device %d beyond the %d devices the scratch table covers", dev,
+              kScratchDevices);
+    return SPS_E_HIP;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                hipGetErrorString(e));
+      return SPS_E_HIP;
+    }
+    scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = scratch_dev[dev];
+  return SPS_OK;
+}
+
+}  // namespace
+}  // namespace sps
+
+extern "C" int sps_fd_reduce_device(int dim, int dtype, const int32_t* n,
+                                    double h, const sps_fd_energy* energy,
+                                    const void* field, double* sums,
+                                    void* hip_stream) {
+  using namespace sps;
+  // every argument is checked before the first HIP call
+  if (dim != 2 && dim != 3) {
+    set_error("dim must be 2 or 3 (got %d)", dim);
+    return SPS_E_INVALID;
+  }
+  if (dtype != SPS_F32 && dtype != SPS_F64) {
+    set_error("dtype must be SPS_F32 or SPS_F64 (got %d)", dtype);
+    return SPS_E_INVALID;
+  }
+  if (!n || !field || !sums) {
+    set_error("n, field and sums must not be NULL");
+    return SPS_E_INVALID;
+  }
+  double sites = 1.0;
+  for (int a = 0; a < dim; ++a) {
+    if (n[a] < 1 || n[a] > (1 << 20)) {
+      set_error("n[%d] = %d out of range [1, 2^20]", a, n[a]);
+      return SPS_E_INVALID;
+    }
+    sites *= n[a];
+  }
+  if (sites > (double)SPS_REDUCE_DEVICE_MAX_SITES) {
+    set_error("field of %.0f sites exceeds the maximum of %d sites", sites,
+              SPS_REDUCE_DEVICE_MAX_SITES);
+    return SPS_E_INVALID;
+  }
+  if (!(h > 0) || !std::isfinite(h)) {
+    set_error("h must be positive and finite");
+    return SPS_E_INVALID;
+  }
+  EnergyParams ep;
+  SPS_TRY(make_energy_params(energy, h, ep));
+  double* partials = nullptr;
+  SPS_TRY(scratch_address(&partials));
+  const int nn[3] = {n[0], n[1], dim == 3 ? n[2] : 1};
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  return dtype == SPS_F32
+             ? launch_reduce<float>(static_cast<const float*>(field), dim, nn,
+                                    h, ep, partials, sums, s)
+             : launch_reduce<double>(static_cast<const double*>(field), dim,
+                                     nn, h, ep, partials, sums, s);
+}
