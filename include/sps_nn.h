@@ -44,11 +44,25 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
                    int epilogue, const void* skip, void* y, void* pre,
                    void* stream);
 
+/* Weight gradients are deterministic: each workgroup reduces its share of
+ * the pixels in a fixed order into its own partial gradient in a caller-owned
+ * FP32 workspace, and one pass then adds the partials to dweight in
+ * workgroup order.  The same inputs give the same bits on every call.
+ * sps_nn_wgrad_workspace gives the workspace size in floats for a layer
+ * (kind SPS_NN_WGRAD_CONV with cin and cout, SPS_NN_WGRAD_IN1 with cout,
+ * SPS_NN_WGRAD_OUT1 with cin); 0 for invalid sizes. */
+#define SPS_NN_WGRAD_CONV 0
+#define SPS_NN_WGRAD_IN1 1
+#define SPS_NN_WGRAD_OUT1 2
+size_t sps_nn_wgrad_workspace(int kind, int n, int h, int w, int cin,
+                              int cout);
+
 /* Weight gradient of sps_nn_conv3x3 (PLAIN epilogue):
  *   dweight[3r+s][k][c] += sum_{n,h,w} x[n,h+r-1,w+s-1,c] * dpre[n,h,w,k]
- * accumulated in FP32 into dweight (caller zeroes it).  n*h*w % 32 == 0. */
+ * accumulated in FP32 into dweight.  n*h*w % 32 == 0. */
 int sps_nn_conv3x3_wgrad(const void* x, const void* dpre, int n, int h, int w,
-                         int cin, int cout, float* dweight, void* stream);
+                         int cin, int cout, float* dweight, float* workspace,
+                         size_t ws_floats, void* stream);
 
 /* First layer, one FP32 input channel:
  *   pre[n,h,w,k] = bias[k] + sum_t x[n, shift_t(h,w)] * weight[t][k],
@@ -58,7 +72,8 @@ int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
                        int act, void* y, void* pre, void* stream);
 /* dweight[t][k] += sum x[shift_t(p)] * dpre[p][k] (FP32 accumulate). */
 int sps_nn_conv3x3_in1_wgrad(const float* x, const void* dpre, int n, int h,
-                             int w, int cout, float* dweight, void* stream);
+                             int w, int cout, float* dweight, float* workspace,
+                             size_t ws_floats, void* stream);
 
 /* Last layer, one FP32 output channel:
  *   y[n,h,w] = bias[0] + sum_{t,c} x[n, shift_t(h,w), c] * weight[t][c],
@@ -70,7 +85,8 @@ int sps_nn_conv3x3_out1(const void* x, int n, int h, int w, int cin,
  * dweight[t][c] += sum_p x[shift_t(p)][c] * dy[p] (FP32). */
 int sps_nn_conv3x3_out1_bwd(const void* x, const float* dy, int n, int h,
                             int w, int cin, const float* weight, void* dx,
-                            float* dweight, void* stream);
+                            float* dweight, float* workspace, size_t ws_floats,
+                            void* stream);
 
 /* dx = dy * silu'(pre), all BF16, count elements. */
 int sps_nn_silu_bwd(const void* dy, const void* pre, void* dx, size_t count,
@@ -82,6 +98,13 @@ int sps_nn_pool2(const void* x, int n, int h, int w, int c, int mode, void* y,
  * average pooling with scale 1/4). */
 int sps_nn_unpool2(const void* x, int n, int h, int w, int c, float scale,
                    void* y, void* stream);
+
+/* y[s][i] = x[s][i] - mean_i x[s][i] for each of n samples of len FP32
+ * values: the mass-conserving projection of the surrogate's increment (it is
+ * its own adjoint, so the backward pass applies it to the gradient).  The
+ * mean is an FP64 sum in a fixed order, one workgroup per sample; x and y may
+ * alias. */
+int sps_nn_center(const float* x, float* y, int n, int64_t len, void* stream);
 
 #ifdef __cplusplus
 }

@@ -192,14 +192,16 @@ def load() -> ctypes.CDLL:
     I, F = c_int, ctypes.c_float
     sig.update({
         "sps_nn_conv3x3": (c_int, [P, I, I, I, I, P, P, I, I, I, P, P, P, P]),
-        "sps_nn_conv3x3_wgrad": (c_int, [P, P, I, I, I, I, I, P, P]),
+        "sps_nn_conv3x3_wgrad": (c_int, [P, P, I, I, I, I, I, P, P, c_size_t, P]),
+        "sps_nn_wgrad_workspace": (c_size_t, [I, I, I, I, I, I]),
         "sps_nn_conv3x3_in1": (c_int, [P, I, I, I, P, P, I, I, P, P, P]),
-        "sps_nn_conv3x3_in1_wgrad": (c_int, [P, P, I, I, I, I, P, P]),
+        "sps_nn_conv3x3_in1_wgrad": (c_int, [P, P, I, I, I, I, P, P, c_size_t, P]),
         "sps_nn_conv3x3_out1": (c_int, [P, I, I, I, I, P, P, P, P]),
-        "sps_nn_conv3x3_out1_bwd": (c_int, [P, P, I, I, I, I, P, P, P, P]),
+        "sps_nn_conv3x3_out1_bwd": (c_int, [P, P, I, I, I, I, P, P, P, P, c_size_t, P]),
         "sps_nn_silu_bwd": (c_int, [P, P, P, c_size_t, P]),
         "sps_nn_pool2": (c_int, [P, I, I, I, I, I, P, P]),
         "sps_nn_unpool2": (c_int, [P, I, I, I, I, F, P, P]),
+        "sps_nn_center": (c_int, [P, P, I, ctypes.c_int64, P]),
     })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

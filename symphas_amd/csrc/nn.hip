@@ -17,6 +17,7 @@
 // The weight gradient reduces over pixels, which are strided in NHWC, so it
 // stages a 32-pixel row chunk of dpre and the three shifted input rows in LDS
 // and gathers the fragments from there.
+#include <algorithm>
 #include <cstdint>
 
 #include "common.hpp"
@@ -184,6 +185,239 @@ __global__ __launch_bounds__(256) void conv_fwd(
   }
 }
 
+// Forward on spatial tiles staged in LDS.  A workgroup owns TH = NW*WN rows
+// x 16 columns of one image and KB = 16*MW*WM output channels; the 4 waves
+// split that WM x WN ways, each wave holding MW x NW accumulator tiles
+// (16 channels x 16 pixels each, one N tile = 16 columns of one row).  Per
+// 32-channel chunk of the input the tile plus its 1-pixel periodic halo,
+// (TH+2) x 18 pixels, is copied to LDS once (double-buffered across chunks)
+// and the 9 taps read their shifted B fragments from there: each activation
+// value leaves HBM/L2 once per chunk instead of once per tap.  A fragments
+// (weights, [tap][out][in]) are 16-byte global loads that every workgroup
+// shares through L2.  A pixel's 32 channels are padded to 40 in LDS so the
+// 16 pixels of one fragment start 20 banks apart.
+constexpr int kTileCols = 16;
+constexpr int kHaloCols = kTileCols + 2;
+constexpr int kPixStride = 40;  // bf16 per staged pixel (32 channels + pad)
+
+template <int MW, int NW, int WM, int WN, int EPI>
+__global__ __launch_bounds__(256) void conv_fwd_tile(
+    const uint16_t* __restrict__ x, int N, int H, int W, int C,
+    const uint16_t* __restrict__ wt, const float* __restrict__ bias, int K,
+    int act, const uint16_t* __restrict__ skip, uint16_t* __restrict__ y,
+    uint16_t* __restrict__ pre) {
+  constexpr int TH = NW * WN;
+  constexpr int KB = 16 * MW * WM;
+  constexpr int TILE = (TH + 2) * kHaloCols;        // staged pixels
+  constexpr int PIECES = TILE * 4;                  // 16-byte pieces per chunk
+  constexpr int PER_THREAD = (PIECES + 255) / 256;
+  // Epilogue through LDS when the block's FP32 results fit 40 KiB: the
+  // workgroup then writes whole runs of pixels x channels per store (the
+  // accumulator layout gives each lane 4 channels of one pixel, 8 bytes)
+  constexpr int SKB = KB + 4;  // staged pixel stride: float4 writes of 16
+                               // pixels start 16 bytes x 9 apart (no conflicts)
+  constexpr int OUT_FLOATS = TH * kTileCols * SKB;
+  constexpr bool STAGED = OUT_FLOATS * 4 <= 40960;
+  constexpr int TILE_BYTES = 2 * TILE * kPixStride * 2;
+  constexpr int LDS_BYTES =
+      STAGED && OUT_FLOATS * 4 > TILE_BYTES ? OUT_FLOATS * 4 : TILE_BYTES;
+  __shared__ __align__(16) unsigned char lds[LDS_BYTES];
+  uint16_t(*tile)[TILE * kPixStride] =
+      reinterpret_cast<uint16_t(*)[TILE * kPixStride]>(lds);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int tiles_w = W / kTileCols, tiles_h = H / TH;
+  int t = blockIdx.x;
+  const int tw = t % tiles_w;
+  t /= tiles_w;
+  const int th = t % tiles_h;
+  const int n = t / tiles_h;
+  const int h0 = th * TH, w0 = tw * kTileCols;
+  const int k0 = blockIdx.y * KB + 16 * MW * wm;
+  const int cq = (lane >> 4) * 8;
+
+  // global source offsets of this thread's staging pieces (channel 0)
+  int64_t src[PER_THREAD];
+  int dst[PER_THREAD];
+#pragma unroll
+  for (int i = 0; i < PER_THREAD; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    const int pix = e >> 2, q = e & 3;
+    const int tr = pix / kHaloCols, tc = pix - tr * kHaloCols;
+    const int row = wrapi(h0 + tr - 1, H), col = wrapi(w0 + tc - 1, W);
+    src[i] = e < PIECES ? ((int64_t)(n * H + row) * W + col) * C + 8 * q : -1;
+    dst[i] = pix * kPixStride + 8 * q;
+  }
+  u32x4 stage[PER_THREAD];
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i)
+      if (src[i] >= 0)
+        stage[i] = *reinterpret_cast<const u32x4*>(x + src[i] + c0);
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; ++i)
+      if (src[i] >= 0)
+        *reinterpret_cast<u32x4*>(&tile[buf][dst[i]]) = stage[i];
+  };
+
+  f32x4v acc[MW][NW];
+#pragma unroll
+  for (int m = 0; m < MW; ++m)
+#pragma unroll
+    for (int j = 0; j < NW; ++j) acc[m][j] = f32x4v{0.f, 0.f, 0.f, 0.f};
+
+  const int nchunks = C / 32;
+  fetch(0);
+  put(0);
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int buf = ch & 1;
+    const int c0 = 32 * ch;
+    // weight fragments are loaded one tap ahead of their MFMAs (loading all
+    // nine taps' at once costs the waves a SIMD can hold)
+    bf16x8 apre[2][MW];
+    auto load_a = [&](int tap, bf16x8 (&dst)[MW]) {
+#pragma unroll
+      for (int m = 0; m < MW; ++m)
+        dst[m] = load_frag(wt + ((int64_t)tap * K + k0 + 16 * m + (lane & 15)) * C +
+                           c0 + cq);
+    };
+    load_a(0, apre[0]);
+    __syncthreads();  // tile[buf] written; tile[buf ^ 1] free
+    if (ch + 1 < nchunks) fetch(32 * (ch + 1));
+    const uint16_t* tb = tile[buf];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dr = tap / 3, ds = tap % 3;  // halo-shifted offsets 0..2
+      bf16x8 b[NW];
+      if (tap + 1 < 9) load_a(tap + 1, apre[(tap + 1) & 1]);
+      const bf16x8 (&a)[MW] = apre[tap & 1];
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const int tr = wn * NW + j + dr;
+        const int tc = (lane & 15) + ds;
+        b[j] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const u32x4*>(
+                        tb + (tr * kHaloCols + tc) * kPixStride + cq));
+      }
+#pragma unroll
+      for (int m = 0; m < MW; ++m)
+#pragma unroll
+        for (int j = 0; j < NW; ++j)
+          acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[j],
+                                                              acc[m][j], 0, 0, 0);
+    }
+    if (ch + 1 < nchunks) put(buf ^ 1);
+  }
+
+  if constexpr (STAGED) {
+    // stage[pixel][channel] in FP32, pixel = row * 16 + column of the tile
+    float* stage = reinterpret_cast<float*>(lds);
+    const int kb0 = blockIdx.y * KB;
+    __syncthreads();  // every wave is done with the input tile
+#pragma unroll
+    for (int m = 0; m < MW; ++m) {
+      const int cl = 16 * MW * wm + 16 * m + 4 * (lane >> 4);
+      float bv[4] = {0.f, 0.f, 0.f, 0.f};
+      if (bias) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bv[i] = bias[kb0 + cl + i];
+      }
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const int px = (wn * NW + j) * kTileCols + (lane & 15);
+        *reinterpret_cast<f32x4v*>(&stage[px * SKB + cl]) =
+            f32x4v{acc[m][j][0] + bv[0], acc[m][j][1] + bv[1], acc[m][j][2] + bv[2],
+                   acc[m][j][3] + bv[3]};
+      }
+    }
+    __syncthreads();
+    constexpr int G = KB / 8;  // 8-channel groups per pixel
+    if constexpr (EPI == SPS_NN_EPI_PLAIN) {
+      for (int e = threadIdx.x; e < TH * kTileCols * G; e += 256) {
+        const int px = e / G, g = e - px * G;
+        const int hh = h0 + px / kTileCols, ww = w0 + px % kTileCols;
+        float v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = stage[px * SKB + 8 * g + i];
+        const int64_t o = ((int64_t)(n * H + hh) * W + ww) * K + kb0 + 8 * g;
+        if (pre) store8(pre + o, v);
+        if (act) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) v[i] = silu(v[i]);
+        }
+        store8(y + o, v);
+      }
+    } else {
+      // output (2*TH) x 32 pixels, each from tile pixel (r/2, c/2)
+      for (int e = threadIdx.x; e < 4 * TH * kTileCols * G; e += 256) {
+        const int opx = e / G, g = e - opx * G;
+        const int orow = opx / (2 * kTileCols), ocol = opx % (2 * kTileCols);
+        const int px = (orow >> 1) * kTileCols + (ocol >> 1);
+        const int64_t o = ((int64_t)(n * 2 * H + 2 * h0 + orow) * (2 * W) + 2 * w0 + ocol) * K +
+                          kb0 + 8 * g;
+        float u[8];
+        load8(skip + o, u);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) u[i] += stage[px * SKB + 8 * g + i];
+        if (pre) store8(pre + o, u);
+        if (act) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) u[i] = silu(u[i]);
+        }
+        store8(y + o, u);
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int m = 0; m < MW; ++m) {
+    const int chn = k0 + 16 * m + 4 * (lane >> 4);  // 4 consecutive channels
+    float bv[4] = {0.f, 0.f, 0.f, 0.f};
+    if (bias) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) bv[i] = bias[chn + i];
+    }
+#pragma unroll
+    for (int j = 0; j < NW; ++j) {
+      const int hh = h0 + wn * NW + j, ww = w0 + (lane & 15);
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[m][j][i] + bv[i];
+      if constexpr (EPI == SPS_NN_EPI_PLAIN) {
+        const int64_t o = ((int64_t)(n * H + hh) * W + ww) * K + chn;
+        if (pre) store4(pre + o, v);
+        if (act) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = silu(v[i]);
+        }
+        store4(y + o, v);
+      } else {
+#pragma unroll
+        for (int a2 = 0; a2 < 2; ++a2)
+#pragma unroll
+          for (int b2 = 0; b2 < 2; ++b2) {
+            const int64_t o =
+                ((int64_t)(n * 2 * H + 2 * hh + a2) * (2 * W) + 2 * ww + b2) * K +
+                chn;
+            float u[4];
+            load4(skip + o, u);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u[i] += v[i];
+            if (pre) store4(pre + o, u);
+            if (act) {
+#pragma unroll
+              for (int i = 0; i < 4; ++i) u[i] = silu(u[i]);
+            }
+            store4(y + o, u);
+          }
+      }
+    }
+  }
+}
+
 // Weight gradient: tile KT x CT of [k][c] for all 9 taps, reduction over
 // 32-pixel row chunks staged in LDS.  Wave w owns k rows [w%2 * KT/2, +KT/2)
 // and c columns [w/2 * CT/2, +CT/2).
@@ -193,7 +427,7 @@ __global__ __launch_bounds__(256) void conv_fwd(
 template <int KT, int CT, bool ROWS>
 __global__ __launch_bounds__(256) void conv_wgrad(
     const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy, int N,
-    int H, int W, int C, int K, float* __restrict__ dw, int nchunks,
+    int H, int W, int C, int K, float* __restrict__ part, int nchunks,
     int chunks_per_block) {
   constexpr int TK = KT / 32, TC = CT / 32;  // MFMA tiles per wave
   __shared__ uint16_t sdy[32][KT + 2];
@@ -282,7 +516,8 @@ __global__ __launch_bounds__(256) void conv_wgrad(
       }
     }
   }
-  if (first >= last) return;
+  // this block's partial sums (zeros if it had no chunk): slice blockIdx.z
+  float* dw = part + (int64_t)blockIdx.z * 9 * K * C;
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
@@ -293,7 +528,7 @@ __global__ __launch_bounds__(256) void conv_wgrad(
         for (int i = 0; i < 4; ++i) {
           const int k = k0 + kw + 16 * tk + 4 * (lane >> 4) + i;
           const int c = c0 + cw + 16 * tc + (lane & 15);
-          atomicAdd(&dw[((int64_t)tap * K + k) * C + c], acc[tap][tk][tc][i]);
+          dw[((int64_t)tap * K + k) * C + c] = acc[tap][tk][tc][i];
         }
 }
 
@@ -335,12 +570,110 @@ __global__ __launch_bounds__(256) void in1_fwd(const float* __restrict__ x,
   }
 }
 
+// The same first layer with its stores staged through LDS: each thread
+// computes all K channels of one pixel exactly as in1_fwd does (bias, then
+// the 9 taps in order, FMA; the weights are uniform across the wave), then
+// the workgroup writes its 256 consecutive pixels x K channels, one
+// contiguous run, in 16-byte pieces.
+template <int K>
+__global__ __launch_bounds__(256) void in1_fwd_staged(
+    const float* __restrict__ x, int N, int H, int W,
+    const float* __restrict__ wt, const float* __restrict__ bias, int act,
+    uint16_t* __restrict__ y, uint16_t* __restrict__ pre) {
+  __shared__ __align__(16) uint16_t sy[256 * K];
+  __shared__ __align__(16) uint16_t sp[256 * K];
+  const int64_t np = (int64_t)N * H * W;
+  const int64_t p0 = (int64_t)blockIdx.x * 256;
+  const int64_t p = p0 + threadIdx.x;
+  if (p < np) {
+    const int HW = H * W;
+    const int n = p / HW, r = p - (int64_t)n * HW, h = r / W, w = r - h * W;
+    float xv[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+      xv[t] = x[(int64_t)(n * H + wrapi(h + t / 3 - 1, H)) * W +
+                wrapi(w + t % 3 - 1, W)];
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 8) {
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        float a = bias ? bias[k0 + i] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 9; ++t) a = fmaf(xv[t], wt[t * K + k0 + i], a);
+        v[i] = a;
+      }
+      if (pre) store8(&sp[threadIdx.x * K + k0], v);
+      if (act) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = silu(v[i]);
+      }
+      store8(&sy[threadIdx.x * K + k0], v);
+    }
+  }
+  __syncthreads();
+  const int64_t valid = np - p0 < 256 ? np - p0 : 256;
+  const int pieces = (int)valid * K / 8;
+  for (int e = threadIdx.x; e < pieces; e += 256) {
+    *reinterpret_cast<u32x4*>(y + p0 * K + 8 * e) =
+        *reinterpret_cast<const u32x4*>(&sy[8 * e]);
+    if (pre)
+      *reinterpret_cast<u32x4*>(pre + p0 * K + 8 * e) =
+          *reinterpret_cast<const u32x4*>(&sp[8 * e]);
+  }
+}
+
+// The last layer on 16 x 16 pixel tiles: the tile and its periodic halo
+// (18 x 18 pixels x C channels) are staged in LDS once, so each activation
+// is read from HBM once instead of once per tap; the weights are uniform
+// across the wave.  Same arithmetic order as out1_fwd (bias, then taps, then
+// channels in order).
+template <int C>
+__global__ __launch_bounds__(256) void out1_fwd_tile(
+    const uint16_t* __restrict__ x, int N, int H, int W,
+    const float* __restrict__ wt, const float* __restrict__ bias,
+    float* __restrict__ y) {
+  constexpr int T = 16, HT = T + 2;
+  constexpr int PS = C + 8;  // padded pixel stride
+  __shared__ __align__(16) uint16_t tile[HT * HT * PS];
+  const int tiles_w = W / T, tiles_h = H / T;
+  int b = blockIdx.x;
+  const int tw = b % tiles_w;
+  b /= tiles_w;
+  const int th = b % tiles_h;
+  const int n = b / tiles_h;
+  const int h0 = th * T, w0 = tw * T;
+  constexpr int groups = C / 8;
+  for (int e = threadIdx.x; e < HT * HT * groups; e += 256) {
+    const int pix = e / groups, q = e - pix * groups;
+    const int tr = pix / HT, tc = pix - tr * HT;
+    const int row = wrapi(h0 + tr - 1, H), col = wrapi(w0 + tc - 1, W);
+    *reinterpret_cast<u32x4*>(&tile[pix * PS + 8 * q]) =
+        *reinterpret_cast<const u32x4*>(x + ((int64_t)(n * H + row) * W + col) * C +
+                                        8 * q);
+  }
+  __syncthreads();
+  const int ty = threadIdx.x / T, tx = threadIdx.x % T;
+  float v = bias ? bias[0] : 0.f;
+  for (int t = 0; t < 9; ++t) {
+    const uint16_t* xp = &tile[((ty + t / 3) * HT + tx + t % 3) * PS];
+#pragma unroll
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      float xv[8];
+      load8(xp + c0, xv);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v = fmaf(xv[i], wt[t * C + c0 + i], v);
+    }
+  }
+  y[(int64_t)(n * H + h0 + ty) * W + w0 + tx] = v;
+}
+
 // dweight[t][k] += sum_p x[shift_t(p)] * dy[p][k]: 256-pixel chunks staged
 // in LDS, then thread (t, k) pairs reduce the chunk.
 __global__ __launch_bounds__(256) void in1_wgrad(const float* __restrict__ x,
                                                  const uint16_t* __restrict__ dy,
                                                  int N, int H, int W, int K,
-                                                 float* __restrict__ dw,
+                                                 float* __restrict__ part,
                                                  int chunks_per_block) {
   __shared__ float sx[9][256];
   __shared__ float sd[256][65];
@@ -381,10 +714,11 @@ __global__ __launch_bounds__(256) void in1_wgrad(const float* __restrict__ x,
       }
     }
   }
+  float* dw = part + (int64_t)blockIdx.x * 9 * K;  // this block's partials
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int tk = threadIdx.x + 256 * q;
-    if (tk < 9 * K) atomicAdd(&dw[tk], acc[q]);
+    if (tk < 9 * K) dw[tk] = acc[q];
   }
 }
 
@@ -444,7 +778,7 @@ __global__ __launch_bounds__(256) void out1_bwd_data(
 __global__ __launch_bounds__(256) void out1_wgrad(const uint16_t* __restrict__ x,
                                                   const float* __restrict__ dy,
                                                   int N, int H, int W, int C,
-                                                  float* __restrict__ dw,
+                                                  float* __restrict__ part,
                                                   int chunks_per_block) {
   __shared__ float sd[256];
   __shared__ int soff[9][256];
@@ -483,10 +817,80 @@ __global__ __launch_bounds__(256) void out1_wgrad(const uint16_t* __restrict__ x
       }
     }
   }
+  float* dw = part + (int64_t)blockIdx.x * 9 * C;  // this block's partials
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int tc = threadIdx.x + 256 * q;
-    if (tc < 9 * C) atomicAdd(&dw[tc], acc[q]);
+    if (tc < 9 * C) dw[tc] = acc[q];
+  }
+}
+
+// Weight gradient of the last layer on 16 x 16 pixel tiles: the x tile with
+// its periodic halo (18 x 18 pixels x C channels, BF16) and the tile's dy
+// (FP32) are staged in LDS, and thread (t, c) sums x[shift_t(p)][c] * dy[p]
+// over the tile's pixels in a fixed order.  A block walks tiles blockIdx.x,
+// + gridDim.x, ... and writes its partial sums to part[blockIdx.x][9][C].
+template <int C>
+__global__ __launch_bounds__(256) void out1_wgrad_tile(
+    const uint16_t* __restrict__ x, const float* __restrict__ dy, int N, int H,
+    int W, float* __restrict__ part) {
+  constexpr int T = 16, HT = T + 2, PS = C + 8, G = C / 8;
+  __shared__ __align__(16) uint16_t sx[HT * HT * PS];
+  __shared__ float sd[T * T];
+  const int tiles_w = W / T, tiles_h = H / T;
+  const int ntiles = N * tiles_h * tiles_w;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int b = tile;
+    const int tw = b % tiles_w;
+    b /= tiles_w;
+    const int th = b % tiles_h;
+    const int n = b / tiles_h;
+    const int h0 = th * T, w0 = tw * T;
+    __syncthreads();  // the previous tile's reads are done
+    for (int e = threadIdx.x; e < HT * HT * G; e += 256) {
+      const int pix = e / G, q = e - pix * G;
+      const int tr = pix / HT, tc = pix - tr * HT;
+      const int row = wrapi(h0 + tr - 1, H), col = wrapi(w0 + tc - 1, W);
+      *reinterpret_cast<u32x4*>(&sx[pix * PS + 8 * q]) =
+          *reinterpret_cast<const u32x4*>(x + ((int64_t)(n * H + row) * W + col) * C + 8 * q);
+    }
+    sd[threadIdx.x] =
+        dy[(int64_t)(n * H + h0 + threadIdx.x / T) * W + w0 + threadIdx.x % T];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int tc = threadIdx.x + 256 * q;
+      if (tc < 9 * C) {
+        const int t = tc / C, ch = tc - t * C;
+        const int dr = t / 3, ds = t % 3;
+        float a = acc[q];
+        for (int py = 0; py < T; ++py)
+#pragma unroll 4
+          for (int px = 0; px < T; ++px)
+            a = fmaf(bf2f(sx[((py + dr) * HT + px + ds) * PS + ch]), sd[py * T + px], a);
+        acc[q] = a;
+      }
+    }
+  }
+  float* dw = part + (int64_t)blockIdx.x * 9 * C;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int tc = threadIdx.x + 256 * q;
+    if (tc < 9 * C) dw[tc] = acc[q];
+  }
+}
+
+// dw[i] += sum over z of part[z][i], z in order: the deterministic sum of the
+// per-block partial weight gradients.
+__global__ __launch_bounds__(256) void add_partials(const float* __restrict__ part,
+                                                    int nz, int64_t count,
+                                                    float* __restrict__ dw) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * 256) {
+    float s = 0.f;
+    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * count + i];
+    dw[i] += s;
   }
 }
 
@@ -544,6 +948,23 @@ __global__ __launch_bounds__(256) void unpool2_k(const uint16_t* __restrict__ x,
   }
 }
 
+// One workgroup per sample: FP64 sum in a fixed order (strided per thread,
+// then xor-shuffle and LDS across the 4 waves), then y = x - mean.
+__global__ __launch_bounds__(256) void center_k(const float* x, float* y,
+                                                int64_t len) {
+  const float* xs = x + (int64_t)blockIdx.x * len;
+  float* ys = y + (int64_t)blockIdx.x * len;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < len; i += 256) s += (double)xs[i];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ double sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  const float mean = (float)(((sh[0] + sh[1]) + (sh[2] + sh[3])) / (double)len);
+  for (int64_t i = threadIdx.x; i < len; i += 256) ys[i] = xs[i] - mean;
+}
+
 int grid_for(int64_t count) {
   int64_t g = (count + 255) / 256;
   return (int)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
@@ -554,6 +975,62 @@ int launch_ok(const char* what) {
   if (e != hipSuccess) {
     set_error("%s launch failed: %s", what, hipGetErrorString(e));
     return SPS_E_HIP;
+  }
+  return SPS_OK;
+}
+
+// How each weight gradient splits its pixel reduction over workgroups; the
+// workspace holds one partial gradient per workgroup (per z slice).
+constexpr int64_t kPartFloats = (int64_t)16 << 20;  // partial sums, at most
+
+struct WgradSplit {
+  int blocks;    // workgroups that write a partial gradient
+  int per;       // chunks / tiles per workgroup
+  int64_t count; // floats of one partial gradient
+};
+
+WgradSplit conv_split(int64_t npix, int cin, int cout) {
+  const int kt = cout % 64 == 0 ? 64 : 32, ct = cin % 64 == 0 ? 64 : 32;
+  const int64_t nchunks = npix / 32;
+  const int64_t count = (int64_t)9 * cin * cout;
+  const int64_t tiles = (int64_t)(cin / ct) * (cout / kt);
+  int64_t nz = (2048 + tiles - 1) / tiles;
+  nz = std::min<int64_t>(nz, std::max<int64_t>(1, kPartFloats / count));
+  nz = std::max<int64_t>(1, std::min<int64_t>(nz, nchunks));
+  const int64_t per = (nchunks + nz - 1) / nz;
+  return {(int)((nchunks + per - 1) / per), (int)per, count};
+}
+
+WgradSplit chunk_split(int64_t np, int64_t count) {
+  const int64_t chunks = (np + 255) / 256;
+  const int64_t per = std::max<int64_t>(16, (chunks + 2047) / 2048);
+  return {(int)((chunks + per - 1) / per), (int)per, count};
+}
+
+bool out1_tiled(int h, int w, int c) {
+  return h % 16 == 0 && w % 16 == 0 && (c == 32 || c == 64);
+}
+
+WgradSplit out1_split(int n, int h, int w, int c) {
+  if (!out1_tiled(h, w, c)) return chunk_split((int64_t)n * h * w, 9 * c);
+  const int64_t tiles = (int64_t)n * (h / 16) * (w / 16);
+  const int64_t per = (tiles + 2047) / 2048;
+  return {(int)((tiles + per - 1) / per), (int)per, (int64_t)9 * c};
+}
+
+int add_parts(const float* part, const WgradSplit& sp, float* dw,
+              hipStream_t s) {
+  add_partials<<<grid_for(sp.count), 256, 0, s>>>(part, sp.blocks, sp.count, dw);
+  return launch_ok("weight-gradient partial sum");
+}
+
+int check_ws(const char* who, const WgradSplit& sp, const float* ws,
+             size_t ws_floats) {
+  if (!ws || (int64_t)ws_floats < (int64_t)sp.blocks * sp.count) {
+    set_error("%s: workspace of %zu floats is smaller than the %lld it needs "
+              "(sps_nn_wgrad_workspace)", who, ws_floats,
+              (long long)sp.blocks * sp.count);
+    return SPS_E_INVALID;
   }
   return SPS_OK;
 }
@@ -580,13 +1057,39 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
     return SPS_E_INVALID;
   }
   hipStream_t s = (hipStream_t)stream;
-  const int nt = cout % 64 == 0 ? 4 : 2;
-  dim3 grid((unsigned)((npix / 64 + 3) / 4), (unsigned)(cout / (16 * nt)));
   const uint16_t* xx = (const uint16_t*)x;
   const uint16_t* ww = (const uint16_t*)weight;
   const uint16_t* sk = (const uint16_t*)skip;
   uint16_t* yy = (uint16_t*)y;
   uint16_t* pp = (uint16_t*)pre;
+  // LDS-tiled kernel when the image splits into whole tiles: 16 columns by
+  // 8 or 16 rows, 32 / 64 / 128 output channels per workgroup
+  if (w % kTileCols == 0) {
+    const int tiles_w = w / kTileCols;
+#define SPS_CONV_TILE(MW, NW, WM, WN)                                           \
+  do {                                                                         \
+    constexpr int TH_ = (NW) * (WN), KB_ = 16 * (MW) * (WM);                   \
+    dim3 g((unsigned)(n * (h / TH_) * tiles_w), (unsigned)(cout / KB_));      \
+    if (epilogue == SPS_NN_EPI_PLAIN)                                          \
+      conv_fwd_tile<MW, NW, WM, WN, 0><<<g, 256, 0, s>>>(                       \
+          xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
+    else                                                                       \
+      conv_fwd_tile<MW, NW, WM, WN, 1><<<g, 256, 0, s>>>(                       \
+          xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
+    return launch_ok("sps_nn_conv3x3");                                        \
+  } while (0)
+    // the upsample-add epilogue stages through LDS: 64 channels x 8 rows
+    if (epilogue == SPS_NN_EPI_UPSAMPLE_ADD && cout % 64 == 0 && h % 8 == 0)
+      SPS_CONV_TILE(4, 2, 1, 4);
+    if (cout % 128 == 0 && h % 8 == 0) SPS_CONV_TILE(4, 4, 2, 2);
+    if (cout % 64 == 0 && h % 16 == 0) SPS_CONV_TILE(4, 4, 1, 4);
+    if (cout % 64 == 0 && h % 8 == 0) SPS_CONV_TILE(4, 2, 1, 4);
+    if (h % 16 == 0) SPS_CONV_TILE(2, 4, 1, 4);
+    if (h % 8 == 0) SPS_CONV_TILE(2, 2, 1, 4);
+#undef SPS_CONV_TILE
+  }
+  const int nt = cout % 64 == 0 ? 4 : 2;
+  dim3 grid((unsigned)((npix / 64 + 3) / 4), (unsigned)(cout / (16 * nt)));
   if (epilogue == SPS_NN_EPI_PLAIN) {
     if (nt == 4)
       conv_fwd<4, 0><<<grid, 256, 0, s>>>(xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp, (int)npix);
@@ -601,8 +1104,20 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
   return launch_ok("sps_nn_conv3x3");
 }
 
+size_t sps_nn_wgrad_workspace(int kind, int n, int h, int w, int cin,
+                              int cout) {
+  if (n < 1 || h < 1 || w < 1 || cin < 1 || cout < 1) return 0;
+  const int64_t np = (int64_t)n * h * w;
+  WgradSplit sp{0, 0, 0};
+  if (kind == SPS_NN_WGRAD_CONV) sp = conv_split(np, cin, cout);
+  else if (kind == SPS_NN_WGRAD_IN1) sp = chunk_split(np, (int64_t)9 * cout);
+  else if (kind == SPS_NN_WGRAD_OUT1) sp = out1_split(n, h, w, cin);
+  return (size_t)((int64_t)sp.blocks * sp.count);
+}
+
 int sps_nn_conv3x3_wgrad(const void* x, const void* dpre, int n, int h, int w,
-                         int cin, int cout, float* dweight, void* stream) {
+                         int cin, int cout, float* dweight, float* workspace,
+                         size_t ws_floats, void* stream) {
   const int64_t npix = (int64_t)n * h * w;
   if (!x || !dpre || !dweight || n < 1 || h < 1 || w < 1 || npix % 32 ||
       npix > (int64_t)1 << 30 || cin % 32 || cout % 32 || cin < 32 ||
@@ -610,20 +1125,22 @@ int sps_nn_conv3x3_wgrad(const void* x, const void* dpre, int n, int h, int w,
     set_error("sps_nn_conv3x3_wgrad: need n*h*w, cin, cout multiples of 32");
     return SPS_E_INVALID;
   }
+  const WgradSplit sp = conv_split(npix, cin, cout);
+  SPS_TRY(check_ws("sps_nn_conv3x3_wgrad", sp, workspace, ws_floats));
   const bool rows = w % 32 == 0;
   const int nchunks = (int)(npix / 32);
-  const int per = 32;
+  const int per = sp.per;
   const int kt = cout % 64 == 0 ? 64 : 32;
   const int ct = cin % 64 == 0 ? 64 : 32;
-  dim3 grid(cin / ct, cout / kt, (nchunks + per - 1) / per);
+  dim3 grid(cin / ct, cout / kt, sp.blocks);
   hipStream_t s = (hipStream_t)stream;
   const uint16_t* xx = (const uint16_t*)x;
   const uint16_t* dd = (const uint16_t*)dpre;
 #define SPS_WGRAD(KT_, CT_)                                                   \
   (rows ? (conv_wgrad<KT_, CT_, true><<<grid, 256, 0, s>>>(                   \
-               xx, dd, n, h, w, cin, cout, dweight, nchunks, per), 0)        \
+               xx, dd, n, h, w, cin, cout, workspace, nchunks, per), 0)      \
         : (conv_wgrad<KT_, CT_, false><<<grid, 256, 0, s>>>(                  \
-               xx, dd, n, h, w, cin, cout, dweight, nchunks, per), 0))
+               xx, dd, n, h, w, cin, cout, workspace, nchunks, per), 0))
   if (kt == 64 && ct == 64)
     SPS_WGRAD(64, 64);
   else if (kt == 64)
@@ -633,7 +1150,8 @@ int sps_nn_conv3x3_wgrad(const void* x, const void* dpre, int n, int h, int w,
   else
     SPS_WGRAD(32, 32);
 #undef SPS_WGRAD
-  return launch_ok("sps_nn_conv3x3_wgrad");
+  SPS_TRY(launch_ok("sps_nn_conv3x3_wgrad"));
+  return add_parts(workspace, sp, dweight, s);
 }
 
 int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
@@ -645,24 +1163,34 @@ int sps_nn_conv3x3_in1(const float* x, int n, int h, int w,
     set_error("sps_nn_conv3x3_in1: cout a multiple of 8 in 8..64, act 0/1");
     return SPS_E_INVALID;
   }
-  in1_fwd<<<(unsigned)((np + 255) / 256), 256, 0, (hipStream_t)stream>>>(
-      x, n, h, w, weight, bias, cout, act, (uint16_t*)y, (uint16_t*)pre);
+  const unsigned blocks = (unsigned)((np + 255) / 256);
+  if (cout == 32)
+    in1_fwd_staged<32><<<blocks, 256, 0, (hipStream_t)stream>>>(
+        x, n, h, w, weight, bias, act, (uint16_t*)y, (uint16_t*)pre);
+  else if (cout == 64)
+    in1_fwd_staged<64><<<blocks, 256, 0, (hipStream_t)stream>>>(
+        x, n, h, w, weight, bias, act, (uint16_t*)y, (uint16_t*)pre);
+  else
+    in1_fwd<<<blocks, 256, 0, (hipStream_t)stream>>>(
+        x, n, h, w, weight, bias, cout, act, (uint16_t*)y, (uint16_t*)pre);
   return launch_ok("sps_nn_conv3x3_in1");
 }
 
 int sps_nn_conv3x3_in1_wgrad(const float* x, const void* dpre, int n, int h,
-                             int w, int cout, float* dweight, void* stream) {
+                             int w, int cout, float* dweight, float* workspace,
+                             size_t ws_floats, void* stream) {
   const int64_t np = (int64_t)n * h * w;
   if (!x || !dpre || !dweight || np < 1 || cout < 8 || cout > 64 || cout % 8) {
     set_error("sps_nn_conv3x3_in1_wgrad: cout a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
-  const int per = 16;  // 256-pixel chunks per block
-  const int64_t chunks = (np + 255) / 256;
-  in1_wgrad<<<(unsigned)((chunks + per - 1) / per), 256, 0,
-              (hipStream_t)stream>>>(x, (const uint16_t*)dpre, n, h, w, cout,
-                                     dweight, per);
-  return launch_ok("sps_nn_conv3x3_in1_wgrad");
+  const WgradSplit sp = chunk_split(np, (int64_t)9 * cout);
+  SPS_TRY(check_ws("sps_nn_conv3x3_in1_wgrad", sp, workspace, ws_floats));
+  hipStream_t s = (hipStream_t)stream;
+  in1_wgrad<<<(unsigned)sp.blocks, 256, 0, s>>>(x, (const uint16_t*)dpre, n, h,
+                                                 w, cout, workspace, sp.per);
+  SPS_TRY(launch_ok("sps_nn_conv3x3_in1_wgrad"));
+  return add_parts(workspace, sp, dweight, s);
 }
 
 int sps_nn_conv3x3_out1(const void* x, int n, int h, int w, int cin,
@@ -673,31 +1201,47 @@ int sps_nn_conv3x3_out1(const void* x, int n, int h, int w, int cin,
     set_error("sps_nn_conv3x3_out1: cin a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
-  out1_fwd<<<(unsigned)((np + 255) / 256), 256, 0, (hipStream_t)stream>>>(
-      (const uint16_t*)x, n, h, w, cin, weight, bias, y);
+  const unsigned tiles = (unsigned)(n * (h / 16) * (w / 16));
+  if (h % 16 == 0 && w % 16 == 0 && cin == 32)
+    out1_fwd_tile<32><<<tiles, 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)x, n, h, w, weight, bias, y);
+  else if (h % 16 == 0 && w % 16 == 0 && cin == 64)
+    out1_fwd_tile<64><<<tiles, 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)x, n, h, w, weight, bias, y);
+  else
+    out1_fwd<<<(unsigned)((np + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)x, n, h, w, cin, weight, bias, y);
   return launch_ok("sps_nn_conv3x3_out1");
 }
 
 int sps_nn_conv3x3_out1_bwd(const void* x, const float* dy, int n, int h,
                             int w, int cin, const float* weight, void* dx,
-                            float* dweight, void* stream) {
+                            float* dweight, float* workspace, size_t ws_floats,
+                            void* stream) {
   const int64_t np = (int64_t)n * h * w;
   if (!x || !dy || !weight || np < 1 || cin < 8 || cin > 64 || cin % 8) {
     set_error("sps_nn_conv3x3_out1_bwd: cin a multiple of 8 in 8..64");
     return SPS_E_INVALID;
   }
   hipStream_t s = (hipStream_t)stream;
+  const WgradSplit sp = out1_split(n, h, w, cin);
+  if (dweight) SPS_TRY(check_ws("sps_nn_conv3x3_out1_bwd", sp, workspace, ws_floats));
   if (dx) {
     out1_bwd_data<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(
         dy, n, h, w, cin, weight, (uint16_t*)dx);
     SPS_TRY(launch_ok("sps_nn_conv3x3_out1_bwd (data)"));
   }
   if (dweight) {
-    const int per = 16;
-    const int64_t chunks = (np + 255) / 256;
-    out1_wgrad<<<(unsigned)((chunks + per - 1) / per), 256, 0, s>>>(
-        (const uint16_t*)x, dy, n, h, w, cin, dweight, per);
+    const uint16_t* xx = (const uint16_t*)x;
+    if (out1_tiled(h, w, cin) && cin == 32)
+      out1_wgrad_tile<32><<<(unsigned)sp.blocks, 256, 0, s>>>(xx, dy, n, h, w, workspace);
+    else if (out1_tiled(h, w, cin))
+      out1_wgrad_tile<64><<<(unsigned)sp.blocks, 256, 0, s>>>(xx, dy, n, h, w, workspace);
+    else
+      out1_wgrad<<<(unsigned)sp.blocks, 256, 0, s>>>(xx, dy, n, h, w, cin, workspace,
+                                                      sp.per);
     SPS_TRY(launch_ok("sps_nn_conv3x3_out1_bwd (weight)"));
+    SPS_TRY(add_parts(workspace, sp, dweight, s));
   }
   return SPS_OK;
 }
@@ -722,6 +1266,16 @@ int sps_nn_pool2(const void* x, int n, int h, int w, int c, int mode, void* y,
   pool2_k<<<grid_for((int64_t)n * h * w * c / 4), 256, 0, (hipStream_t)stream>>>(
       (const uint16_t*)x, n, h, w, c, mode, (uint16_t*)y);
   return launch_ok("sps_nn_pool2");
+}
+
+int sps_nn_center(const float* x, float* y, int n, int64_t len,
+                  void* stream) {
+  if (!x || !y || n < 1 || len < 1) {
+    set_error("sps_nn_center: NULL pointer or empty samples");
+    return SPS_E_INVALID;
+  }
+  center_k<<<(unsigned)n, 256, 0, (hipStream_t)stream>>>(x, y, len);
+  return launch_ok("sps_nn_center");
 }
 
 int sps_nn_unpool2(const void* x, int n, int h, int w, int c, float scale,

@@ -57,10 +57,22 @@ def conv3x3_raw(x, w_bf16, bias, act, epi=EPI_PLAIN, skip=None, want_pre=False):
     return y, pre
 
 
+WGRAD_CONV, WGRAD_IN1, WGRAD_OUT1 = 0, 1, 2
+
+
+def wgrad_workspace(kind, N, H, W, C, K, device):
+    """FP32 scratch for a deterministic weight gradient (per-workgroup partial
+    sums, added in a fixed order)."""
+    n = _lib().sps_nn_wgrad_workspace(kind, N, H, W, C, K)
+    return torch.empty(max(int(n), 1), device=device, dtype=torch.float32)
+
+
 def conv3x3_wgrad_raw(x, dpre, K):
     N, H, W, C = x.shape
     dw = torch.zeros((9, K, C), device=x.device, dtype=torch.float32)
-    _check(_lib().sps_nn_conv3x3_wgrad(_ptr(x), _ptr(dpre), N, H, W, C, K, _ptr(dw), _stream()))
+    ws = wgrad_workspace(WGRAD_CONV, N, H, W, C, K, x.device)
+    _check(_lib().sps_nn_conv3x3_wgrad(_ptr(x), _ptr(dpre), N, H, W, C, K, _ptr(dw), _ptr(ws),
+                                       ws.numel(), _stream()))
     return dw
 
 
@@ -126,7 +138,8 @@ class _In1(torch.autograd.Function):
         N, H, W = x.shape
         K = w.shape[1]
         y = torch.empty((N, H, W, K), device=x.device, dtype=torch.bfloat16)
-        pre = torch.empty_like(y) if act == ACT_SILU else None
+        # the pre-activation is kept only when a backward pass will need it
+        pre = torch.empty_like(y) if act == ACT_SILU and any(ctx.needs_input_grad) else None
         _check(_lib().sps_nn_conv3x3_in1(_ptr(x.contiguous()), N, H, W, _ptr(w.detach().contiguous()),
                                          _ptr(b.detach()), K, act, _ptr(y), _ptr(pre), _stream()))
         ctx.act = act
@@ -141,8 +154,9 @@ class _In1(torch.autograd.Function):
         N, H, W = x.shape
         K = w.shape[1]
         dw = torch.zeros((9, K), device=x.device, dtype=torch.float32)
+        ws = wgrad_workspace(WGRAD_IN1, N, H, W, 1, K, x.device)
         _check(_lib().sps_nn_conv3x3_in1_wgrad(_ptr(x.contiguous()), _ptr(dpre), N, H, W, K,
-                                               _ptr(dw), _stream()))
+                                               _ptr(dw), _ptr(ws), ws.numel(), _stream()))
         db = dpre.float().sum(dim=(0, 1, 2))
         return None, dw, db, None
 
@@ -165,9 +179,10 @@ class _Out1(torch.autograd.Function):
         dy = dy.contiguous().float()
         dx = torch.empty_like(x)
         dw = torch.zeros((9, C), device=x.device, dtype=torch.float32)
+        ws = wgrad_workspace(WGRAD_OUT1, N, H, W, C, 1, x.device)
         _check(_lib().sps_nn_conv3x3_out1_bwd(_ptr(x.contiguous()), _ptr(dy), N, H, W, C,
                                               _ptr(w.detach().contiguous()), _ptr(dx), _ptr(dw),
-                                              _stream()))
+                                              _ptr(ws), ws.numel(), _stream()))
         return dx, dw
 
 
@@ -179,6 +194,24 @@ class _Pool(torch.autograd.Function):
     @staticmethod
     def backward(ctx, dy):
         return unpool2_raw(dy.contiguous().to(torch.bfloat16), 0.25)
+
+
+def center_raw(d):
+    """d [N, ...] FP32 minus each sample's mean (FP64 sum, fixed order)."""
+    out = torch.empty_like(d)
+    _check(_lib().sps_nn_center(_ptr(d), _ptr(out), d.shape[0], d[0].numel(), _stream()))
+    return out
+
+
+class _Center(torch.autograd.Function):
+    # the projection I - 11^T/n is symmetric: its backward is itself
+    @staticmethod
+    def forward(ctx, d):
+        return center_raw(d.contiguous())
+
+    @staticmethod
+    def backward(ctx, g):
+        return center_raw(g.contiguous())
 
 
 class UNet(torch.nn.Module):
@@ -232,7 +265,7 @@ class UNet(torch.nn.Module):
             x = _Conv.apply(x, self.dec_w[2 * i], self.dec_b[2 * i], ACT_SILU, EPI_UPADD, skips[l], g)
             x = _Conv.apply(x, self.dec_w[2 * i + 1], self.dec_b[2 * i + 1], ACT_SILU, EPI_PLAIN, None, g)
         d = _Out1.apply(x, self.w_out)
-        return d - d.mean(dim=(1, 2), keepdim=True)
+        return _Center.apply(d)
 
     def forward(self, psi):
         return psi + self.out_scale * self.increment(psi)
@@ -251,3 +284,34 @@ class UNet(torch.nn.Module):
             f += 2 * 9 * self.ch[l + 1] * self.ch[l] * hw_c + 2 * 9 * self.ch[l] * self.ch[l] * hw
         f += 2 * 9 * self.ch[0] * H * W
         return f
+
+
+class GraphedForward:
+    """Inference of a network replayed from captured HIP graphs, one per input
+    shape.  Every kernel of the forward pass runs on the capturing stream
+    (the convolutions take torch's current stream) and nothing in it clears
+    memory with a memset, so a replay repeats the captured launches exactly;
+    it removes the per-layer host work of the eager forward.  The returned
+    tensor is a copy: later replays do not change it."""
+
+    def __init__(self, net):
+        self.net = net
+        self.graphs = {}
+
+    def __call__(self, x):
+        key = tuple(x.shape)
+        if key not in self.graphs:
+            static_in = x.detach().clone()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side), torch.no_grad():
+                self.net(static_in)  # warm: allocations and lazy set-up
+            torch.cuda.current_stream().wait_stream(side)
+            g = torch.cuda.CUDAGraph()
+            with torch.no_grad(), torch.cuda.graph(g):
+                static_out = self.net(static_in)
+            self.graphs[key] = (g, static_in, static_out)
+        g, static_in, static_out = self.graphs[key]
+        static_in.copy_(x)
+        g.replay()
+        return static_out.clone()

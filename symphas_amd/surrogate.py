@@ -25,6 +25,7 @@ import numpy as np
 import torch
 
 from . import abi, inputs, models
+from . import nn
 from .nn import UNet
 from .solver import FDBatch
 
@@ -193,6 +194,8 @@ class SurrogateSolver:
 
     def __init__(self, net, batch, n, k=50, tol=1e-3, check_every=4):
         self.net, self.k, self.tol, self.check_every = net, k, tol, check_every
+        # the U-Net's inference replays a captured HIP graph per batch shape
+        self.forward = nn.GraphedForward(net) if isinstance(net, nn.UNet) else net
         self.referee = PlainSolver(batch, n)
 
     def advance(self, psi, n_adv):
@@ -213,7 +216,7 @@ class SurrogateSolver:
             idx = None if all_live else torch.from_numpy(live).to(psi.device)
             x = psi if all_live else psi.index_select(0, idx)
             with torch.no_grad():
-                pred = self.net(x)
+                pred = self.forward(x)
             rep.surrogate_calls += 1
             m0, m1 = x.double().sum((1, 2)), pred.double().sum((1, 2))
             mass_bad = (m1 - m0).abs() > self.tol * x.double().abs().sum((1, 2))
@@ -266,11 +269,14 @@ def time_plain(psi, steps, n):
 
 
 def time_network(net, psi, reps=5):
+    """Seconds per inference of net on psi, as the controller runs it (the
+    U-Net through its captured graph)."""
+    fwd = nn.GraphedForward(net) if isinstance(net, nn.UNet) else net
     with torch.no_grad():
-        net(psi)
+        fwd(psi)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
-            net(psi)
+            fwd(psi)
         torch.cuda.synchronize()
     return (time.perf_counter() - t0) / reps

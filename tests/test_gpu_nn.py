@@ -85,7 +85,8 @@ def test_conv_backward_weight(gpu_lib, N, H, W, C, K):
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max() * np.sqrt(N * H * W)
 
 
-@pytest.mark.parametrize("N,H,W,K", [(2, 16, 16, 32), (3, 12, 20, 64), (1, 9, 7, 8)])
+@pytest.mark.parametrize("N,H,W,K", [(2, 16, 16, 32), (3, 12, 20, 64), (1, 9, 7, 8),
+                                     (2, 32, 48, 64)])
 def test_thin_layers(gpu_lib, N, H, W, K):
     # 720 and 63 pixels leave a partial 256-pixel chunk in the weight gradients
     print(f"[case] thin layers {N}x{H}x{W} K={K}", flush=True)
@@ -102,8 +103,9 @@ def test_thin_layers(gpu_lib, N, H, W, K):
     assert within_ulp(np64(pre), ref).all()
     dpre = co.bf16_round(rnd((N, H, W, K), 15))
     dw = torch.zeros((9, K), device=DEV)
+    ws = nn.wgrad_workspace(nn.WGRAD_IN1, N, H, W, 1, K, DEV)
     nn._check(lib.sps_nn_conv3x3_in1_wgrad(nn._ptr(xt), nn._ptr(bf(dpre)), N, H, W, K,
-                                           nn._ptr(dw), nn._stream()))
+                                           nn._ptr(dw), nn._ptr(ws), ws.numel(), nn._stream()))
     refw = co.conv3x3_bwd_weight(x[..., None].astype(np.float64), dpre)[:, :, 0]
     assert np.allclose(dw.cpu().numpy(), refw, rtol=1e-4, atol=1e-4)
     # last layer
@@ -119,9 +121,10 @@ def test_thin_layers(gpu_lib, N, H, W, K):
     dy = rnd((N, H, W), 18).astype(np.float32)
     dx = torch.empty((N, H, W, K), device=DEV, dtype=torch.bfloat16)
     dwo = torch.zeros((9, K), device=DEV)
+    wso = nn.wgrad_workspace(nn.WGRAD_OUT1, N, H, W, K, 1, DEV)
     nn._check(lib.sps_nn_conv3x3_out1_bwd(nn._ptr(bf(h)), nn._ptr(torch.from_numpy(dy).to(DEV)),
                                           N, H, W, K, nn._ptr(wot), nn._ptr(dx), nn._ptr(dwo),
-                                          nn._stream()))
+                                          nn._ptr(wso), wso.numel(), nn._stream()))
     refdx = co.conv3x3_bwd_data(dy[..., None].astype(np.float64), wo[:, None, :].astype(np.float64))
     assert within_ulp(np64(dx), refdx, ulps=1.01).all()
     refdw = co.conv3x3_bwd_weight(h, dy[..., None].astype(np.float64))[:, 0, :]
@@ -219,3 +222,60 @@ def test_unet_forward_and_gradient_vs_cpu(gpu_lib):
         rel = float((g - r).norm() / max(float(r.norm()), 1e-30))
         print(f"grad {name:10s} rel L2 {rel:.2e}")
         assert rel < 5e-2, name
+
+
+def test_center_removes_each_samples_mean(gpu_lib):
+    d = torch.from_numpy(rnd((3, 8, 16), 30).astype(np.float32)).to(DEV)
+    out = nn.center_raw(d).cpu().numpy().astype(np.float64)
+    ref = d.cpu().numpy().astype(np.float64)
+    ref = ref - ref.mean(axis=(1, 2), keepdims=True)
+    assert np.abs(out - ref).max() < 1e-6
+    assert np.abs(out.sum(axis=(1, 2))).max() < 1e-4
+
+
+def test_graphed_forward_replays_match_eager(gpu_lib):
+    """The captured forward is replayed several times with different inputs,
+    with eager work and synchronisations between the replays; every replay
+    must equal the eager forward of its input bit for bit."""
+    print("[case] graphed U-Net forward", flush=True)
+    torch.manual_seed(0)
+    net = nn.UNet(seed=3).to(DEV)
+    fwd = nn.GraphedForward(net)
+    inputs_ = [torch.from_numpy(rnd((4, 64, 64), 40 + i, 0.1).astype(np.float32)).to(DEV)
+               for i in range(3)]
+    for rep in range(2):
+        for x in inputs_:
+            with torch.no_grad():
+                want = net(x)
+            _ = torch.randn(1000, device=DEV).sum()  # eager work between replays
+            torch.cuda.synchronize()
+            got = fwd(x)
+            torch.cuda.synchronize()
+            assert torch.equal(got, want), (rep, (got - want).abs().max().item())
+
+
+def test_weight_gradients_are_deterministic(gpu_lib):
+    """Per-workgroup partial sums added in a fixed order: two calls on the same
+    inputs give the same bits, for the MFMA gradient and both thin layers."""
+    print("[case] deterministic weight gradients", flush=True)
+    N, H, W, C, K = 4, 32, 64, 64, 64
+    x = bf(rnd((N, H, W, C), 50))
+    dy = bf(rnd((N, H, W, K), 51))
+    a, b = nn.conv3x3_wgrad_raw(x, dy, K), nn.conv3x3_wgrad_raw(x, dy, K)
+    assert torch.equal(a, b)
+    lib = gpu_lib
+    xin = torch.from_numpy(rnd((N, H, W), 52).astype(np.float32)).to(DEV)
+    outs = []
+    for _ in range(2):
+        dw = torch.zeros((9, K), device=DEV)
+        ws = nn.wgrad_workspace(nn.WGRAD_IN1, N, H, W, 1, K, DEV)
+        nn._check(lib.sps_nn_conv3x3_in1_wgrad(nn._ptr(xin), nn._ptr(dy), N, H, W, K, nn._ptr(dw),
+                                               nn._ptr(ws), ws.numel(), nn._stream()))
+        dwo = torch.zeros((9, C), device=DEV)
+        wso = nn.wgrad_workspace(nn.WGRAD_OUT1, N, H, W, C, 1, DEV)
+        wo = torch.zeros((9, C), device=DEV)
+        nn._check(lib.sps_nn_conv3x3_out1_bwd(nn._ptr(x), nn._ptr(xin), N, H, W, C, nn._ptr(wo),
+                                              None, nn._ptr(dwo), nn._ptr(wso), wso.numel(),
+                                              nn._stream()))
+        outs.append((dw.clone(), dwo.clone()))
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
