@@ -349,6 +349,8 @@ int sps_fdr_step(sps_fdr_plan* plan, int nsteps);
 /* The whole domain's field: box interior values inside, empty outside. */
 int sps_fdr_download(sps_fdr_plan* plan, void* host, int layout);
 int sps_fdr_region(const sps_fdr_plan* plan, sps_fdr_region_info* out);
+/* Wait for all work enqueued on the regional plan. */
+int sps_fdr_sync(sps_fdr_plan* plan);
 
 /* ------------------------------------------------------------- checkpoints */
 

@@ -188,6 +188,7 @@ def load() -> ctypes.CDLL:
         "sps_fdr_step": (c_int, [P, c_int]),
         "sps_fdr_download": (c_int, [P, P, c_int]),
         "sps_fdr_region": (c_int, [P, POINTER(RegionInfo)]),
+        "sps_fdr_sync": (c_int, [P]),
     }
     I, F = c_int, ctypes.c_float
     sig.update({

@@ -674,6 +674,16 @@ int sps_fdr_download(sps_fdr_plan* p, void* host, int layout) {
   return SPS_OK;
 }
 
+int sps_fdr_sync(sps_fdr_plan* p) {
+  using namespace sps;
+  if (!p) {
+    set_error("plan is NULL");
+    return SPS_E_INVALID;
+  }
+  SPS_HIP_TRY(hipStreamSynchronize(p->stream));
+  return SPS_OK;
+}
+
 int sps_fdr_region(const sps_fdr_plan* p, sps_fdr_region_info* out) {
   using namespace sps;
   if (!p || !out) {

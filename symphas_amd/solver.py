@@ -512,3 +512,6 @@ class RegionalPlan:
         info = abi.RegionInfo()
         abi.check(self.lib.sps_fdr_region(self._h, byref(info)))
         return info
+
+    def sync(self):
+        abi.check(self.lib.sps_fdr_sync(self._h))

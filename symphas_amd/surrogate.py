@@ -123,10 +123,11 @@ class TrainLog:
     iters: int = 0
     seconds: float = 0.0
     losses: list = field(default_factory=list)
+    evaluations: list = field(default_factory=list)  # (iterations, evaluate(net))
 
 
 def train(net, iters=400, batch=32, n=256, k=50, horizon=20000, lr=3e-4, seed=1,
-          log_every=50, verbose=True):
+          log_every=50, verbose=True, evaluate=None, eval_at=()):
     """Train on pairs generated on the fly by the GPU solver (Adam, constant
     learning rate: a probe of lr in {3e-4, 1e-3} and output scale in {0.01,
     0.1} on 128^2 fields picked 3e-4 with scale 0.1)."""
@@ -146,6 +147,8 @@ def train(net, iters=400, batch=32, n=256, k=50, horizon=20000, lr=3e-4, seed=1,
             log.losses.append((it, lv))
             if verbose:
                 print(f"[train] iter {it} rel-increment loss {lv:.4e}", flush=True)
+        if evaluate is not None and it + 1 in eval_at:
+            log.evaluations.append((it + 1, float(evaluate(net))))
     torch.cuda.synchronize()
     log.iters = iters
     log.seconds = time.perf_counter() - t0
