@@ -289,9 +289,13 @@ int sps_fd_copy_device(sps_fd_plan* plan, void* dev, int direction);
  * (datatypes/inc/grid.h:2038-2157).  The field is stored on a box of the
  * periodic domain with a boundary layer of SPS_BOUNDARY_DEPTH cells per
  * side; every site outside the box reads as `empty`.  A step evaluates the
- * right-hand side and forward Euler over the box interior only, in the
- * reference order (FP64 results equal the CPU SolverFT's on the box), so
- * its cost follows the box, not the domain.
+ * right-hand side and forward Euler over the box interior only, so its cost
+ * follows the box, not the domain.  desc->path selects the arithmetic as for
+ * sps_fd_create: SPS_PATH_REFERENCE runs the reference order (FP64 results
+ * equal the CPU SolverFT's on the box); the canonical form (SPS_PATH_AUTO
+ * or SPS_PATH_FUSED) runs the fused form of the streamed kernels one step at
+ * a time (a box that spans the domain equals sps_fd_step at one step per
+ * launch bit for bit).
  *
  * Update before each right-hand side (SolverSystemFDwSD::update,
  * update_regional_system, sol/inc/boundarysystem.h:437-479):

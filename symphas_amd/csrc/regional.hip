@@ -26,6 +26,7 @@
 #include <new>
 #include <vector>
 
+#include "lanes.hpp"
 #include "plan.hpp"
 #include "ref_kernels.hpp"
 
@@ -205,6 +206,78 @@ __global__ __launch_bounds__(256) void box_scatter_full(
   }
 }
 
+// Fused form on a box (the canonical form of fused.hpp's kernels, one step per
+// pass pair): pass A evaluates mu~ = S + psi*p(psi) at every box site within
+// one site of the interior, pass B the update psi' = psi + q(psi) +
+// k2*(S[mu~] - w*mu~) over the interior; the radius-one form is pass B alone
+// on psi.  Each expression, its operand order and its FMAs are those of the
+// streamed kernels (fused_tile.hpp, fused3d.hip), so a box that spans the
+// domain gives their one-step-per-launch results bit for bit.
+template <typename T, int DIM>
+__device__ __forceinline__ T box_nsum(const T* f, int64_t i, int64_t sy,
+                                      int64_t sz) {
+  if (DIM == 2) return (f[i - 1] + f[i + 1]) + (f[i + sy] + f[i - sy]);
+  return ((f[i - 1] + f[i - sy]) + (f[i + 1] + f[i + sy])) + (f[i - sz] + f[i + sz]);
+}
+
+// One thread per box site, (64 x 4) threads per workgroup over (x, y) and
+// one grid layer per z: no index division.
+template <typename T, int DIM, int POLY>
+__global__ __launch_bounds__(256) void box_mu(const T* __restrict__ psi,
+                                              T* __restrict__ mu, BoxGeom g,
+                                              FusedCoef<T> c) {
+  const int lx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int ly = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int lz = blockIdx.z;
+  // sites within one of the interior: their neighbours are in the box
+  if (lx < kB - 1 || lx >= g.b[0] - kB + 1 || ly < kB - 1 || ly >= g.b[1] - kB + 1)
+    return;
+  if (DIM == 3 && (lz < kB - 1 || lz >= g.b[2] - kB + 1)) return;
+  const int64_t sy = g.b[0], sz = (int64_t)g.b[0] * g.b[1];
+  const int64_t i = lx + sy * ly + sz * lz;
+  const T sn = box_nsum<T, DIM>(psi, i, sy, sz);
+  const T x = psi[i];
+  const T t = horner<POLY - 1>(c.p, x);
+  mu[i] = fmad(t, x, POLY == 3 ? sn : c.ks * sn);
+}
+
+template <typename T, int DIM, int R, bool HASA, int POLY>
+__global__ __launch_bounds__(256) void box_update(const T* __restrict__ psi,
+                                                  const T* __restrict__ mu,
+                                                  T* __restrict__ out,
+                                                  BoxGeom g, FusedCoef<T> c) {
+  const int lx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int ly = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int lz = blockIdx.z;
+  if (lx >= g.b[0] || ly >= g.b[1]) return;
+  const int64_t sy = g.b[0], sz = (int64_t)g.b[0] * g.b[1];
+  const int64_t i = lx + sy * ly + sz * lz;
+  constexpr T w = T(2 * DIM);
+  const bool interior = lx >= kB && lx < g.b[0] - kB && ly >= kB &&
+                        ly < g.b[1] - kB &&
+                        (DIM == 2 || (lz >= kB && lz < g.b[2] - kB));
+  if (!interior) {
+    out[i] = psi[i];
+    return;
+  }
+  if constexpr (R == 2) {
+    const T sn = box_nsum<T, DIM>(mu, i, sy, sz);
+    T base = psi[i];
+    if constexpr (HASA) base = base + horner<POLY>(c.q, base);
+    out[i] = fmad(c.k2, fmad(-w, mu[i], sn), base);
+  } else {
+    const T sn = box_nsum<T, DIM>(psi, i, sy, sz);
+    const T x = psi[i];
+    T base = x;
+    if constexpr (HASA) base = base + horner<POLY>(c.q, x);
+    out[i] = fmad(c.k1, fmad(-w, x, sn), base);
+  }
+}
+
+template <typename T, int DIM, int R, bool HASA, int POLY>
+int launch_box_fused(const T* psi, T* mu, T* out, const BoxGeom& g,
+                     const FusedCoef<T>& c, int64_t sites, hipStream_t s);
+
 int grid_for(int64_t work) {
   const int64_t blocks = (work + 255) / 256;
   return (int)std::max<int64_t>(1, std::min<int64_t>(blocks, 8192));
@@ -258,6 +331,34 @@ bool cover_interval(const int* occ, int n, int& start, int& length) {
   return true;
 }
 
+template <typename T, int DIM, int R, bool HASA, int POLY>
+int launch_box_fused(const T* psi, T* mu, T* out, const BoxGeom& g,
+                     const FusedCoef<T>& c, int64_t sites, hipStream_t s) {
+  (void)sites;
+  const dim3 grid((g.b[0] + 63) / 64, (g.b[1] + 3) / 4, DIM == 3 ? g.b[2] : 1);
+  if constexpr (R == 2) {
+    box_mu<T, DIM, POLY><<<grid, 256, 0, s>>>(psi, mu, g, c);
+    SPS_TRY(check_launch("regional fused mu"));
+  }
+  box_update<T, DIM, R, HASA, POLY><<<grid, 256, 0, s>>>(psi, mu, out, g, c);
+  return check_launch("regional fused update");
+}
+
+// The instance of the fused box step for a lowered form (radius, A present,
+// polynomial bound), as the streamed kernels dispatch it.
+template <typename T, int DIM>
+int box_fused_step(const T* psi, T* mu, T* out, const BoxGeom& g,
+                   const FusedCoef<T>& c, int64_t sites, hipStream_t s) {
+  const bool hasa = c.dega >= 0;
+#define SPS_BOX(R_, A_, P_) launch_box_fused<T, DIM, R_, A_, P_>(psi, mu, out, g, c, sites, s)
+  if (c.radius == 1)
+    return c.poly == 3 ? (hasa ? SPS_BOX(1, true, 3) : SPS_BOX(1, false, 3))
+                       : (hasa ? SPS_BOX(1, true, 6) : SPS_BOX(1, false, 6));
+  return c.poly == 3 ? (hasa ? SPS_BOX(2, true, 3) : SPS_BOX(2, false, 3))
+                     : (hasa ? SPS_BOX(2, true, 6) : SPS_BOX(2, false, 6));
+#undef SPS_BOX
+}
+
 int mod_host(int64_t v, int n) {
   const int64_t r = v % n;
   return (int)(r < 0 ? r + n : r);
@@ -279,6 +380,10 @@ struct sps_fdr_plan {
   double* minval = nullptr;
   sps::RefParams<float> rp32;
   sps::RefParams<double> rp64;
+  bool fused = false;      // canonical form: the fused box step
+  void* mu = nullptr;      // its mu~ box (radius two)
+  sps::FusedCoef<float> fc32;
+  sps::FusedCoef<double> fc64;
   int o[3] = {0, 0, 0}, b[3] = {1, 1, 1};
   bool has_data = false, has_box = true;
   int next_resize = 0, index = 0;
@@ -446,7 +551,27 @@ struct sps_fdr_plan {
 
   int step_once() {
     SPS_TRY(update());
-    if (has_box) {
+    if (has_box && fused) {
+      const int nxt = cur ^ 1;
+      const sps::BoxGeom g = geom();
+      int rc;
+      if (d.dtype == SPS_F32)
+        rc = d.dim == 2 ? sps::box_fused_step<float, 2>((const float*)box[cur], (float*)mu,
+                                                       (float*)box[nxt], g, fc32,
+                                                       box_sites(), stream)
+                        : sps::box_fused_step<float, 3>((const float*)box[cur], (float*)mu,
+                                                       (float*)box[nxt], g, fc32,
+                                                       box_sites(), stream);
+      else
+        rc = d.dim == 2 ? sps::box_fused_step<double, 2>((const double*)box[cur], (double*)mu,
+                                                        (double*)box[nxt], g, fc64,
+                                                        box_sites(), stream)
+                        : sps::box_fused_step<double, 3>((const double*)box[cur], (double*)mu,
+                                                        (double*)box[nxt], g, fc64,
+                                                        box_sites(), stream);
+      if (rc != SPS_OK) return rc;
+      cur = nxt;
+    } else if (has_box) {
       const int nxt = cur ^ 1;
       int rc;
       if (d.dtype == SPS_F32) {
@@ -473,6 +598,7 @@ struct sps_fdr_plan {
     if (occ) (void)hipFree(occ);
     if (partial) (void)hipFree(partial);
     if (minval) (void)hipFree(minval);
+    if (mu) (void)hipFree(mu);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -532,6 +658,19 @@ int sps_fdr_create(const sps_fd_desc* desc, const sps_fd_regional* params,
   }
   make_ref_params(*desc, p->rp32);
   make_ref_params(*desc, p->rp64);
+  // the canonical form takes the fused box step unless the reference order
+  // is asked for
+  const Lowered low = lower(*desc);
+  if (desc->path == SPS_PATH_FUSED && !low.fusable) {
+    set_error("fused path unavailable: %s", low.why_not);
+    delete p;
+    return SPS_E_UNSUPPORTED;
+  }
+  p->fused = low.fusable && desc->path != SPS_PATH_REFERENCE;
+  if (p->fused) {
+    make_fused_coef(*desc, low, p->fc32);
+    make_fused_coef(*desc, low, p->fc64);
+  }
   int rc = SPS_OK;
   auto alloc = [&](void** ptr, size_t bytes) {
     if (rc != SPS_OK) return;
@@ -555,6 +694,7 @@ int sps_fdr_create(const sps_fd_desc* desc, const sps_fd_regional* params,
         sizeof(int) * (desc->n[0] + desc->n[1] + (desc->dim == 3 ? desc->n[2] : 0)));
   alloc((void**)&p->partial, sizeof(double) * kMinBlocks);
   alloc((void**)&p->minval, sizeof(double));
+  if (p->fused) alloc(&p->mu, p->cap * p->esize);
   if (rc != SPS_OK) {
     p->release();
     delete p;

@@ -59,24 +59,29 @@ def test_regional_rejects_bad_arguments_before_any_hip_call():
         assert b"regional parameters" in lib.sps_fd_last_error()
 
 
+REF = abi.PATH_REFERENCE
 CASES = [
-    # name, desc factory, field factory, params, steps
-    ("mb2_fixed", lambda: models.model_b((64, 48), dt=0.01, dtype=abi.F64),
+    # name, desc factory (path: the reference order), field factory, params, steps
+    ("mb2_fixed", lambda path=REF: models.model_b((64, 48), dt=0.01, dtype=abi.F64, path=path),
      lambda d: _blob(interior_shape(d), (20, 14), (12, 8), noise_seed=1), _params(every=0.04), 60),
-    ("mb2_resize", lambda: models.model_b((64, 48), dt=0.01, dtype=abi.F64),
+    ("mb2_resize", lambda path=REF: models.model_b((64, 48), dt=0.01, dtype=abi.F64, path=path),
      lambda d: _blob(interior_shape(d), (50, 30), (10, 12), noise_seed=2),
      _params(fixed=0, every=0.05), 80),
-    ("ac2_wrapping", lambda: models.allen_cahn((40, 36), h=0.5, dt=0.01, dtype=abi.F64),
+    ("ac2_wrapping", lambda path=REF: models.allen_cahn((40, 36), h=0.5, dt=0.01, dtype=abi.F64,
+                                                        path=path),
      lambda d: _blob(interior_shape(d), (34, 30), (9, 9), value=0.8, noise_seed=3) - 0.8,
      _params(fixed=0, every=0.02, factor=1.5), 50),
-    ("mb3_resize", lambda: models.model_b((28, 24, 20), dt=0.005, dtype=abi.F64,
-                                          stencil=models.Stencil3d2h(7, 10, 25)),
+    ("mb3_resize", lambda path=REF: models.model_b((28, 24, 20), dt=0.005, dtype=abi.F64,
+                                                   stencil=models.Stencil3d2h(7, 10, 25),
+                                                   path=path),
      lambda d: _blob(interior_shape(d), (6, 8, 5), (7, 6, 5), noise_seed=4),
      _params(fixed=0, every=0.02), 24),
-    ("mb2_lap9_bilap17", lambda: models.model_b((40, 40), dt=0.005, dtype=abi.F64,
-                                                stencil=models.Stencil2d2h(9, 6, 17)),
+    ("mb2_lap9_bilap17", lambda path=REF: models.model_b((40, 40), dt=0.005, dtype=abi.F64,
+                                                         stencil=models.Stencil2d2h(9, 6, 17),
+                                                         path=path),
      lambda d: _blob(interior_shape(d), (10, 12), (8, 6), noise_seed=5), _params(every=0.03), 40),
 ]
+FUSABLE = [c for c in CASES if c[0] != "mb2_lap9_bilap17"]
 
 
 @pytest.mark.gpu
@@ -89,6 +94,47 @@ def test_regional_fp64_bitwise_vs_oracle(gpu_lib, name, mk, field, params, steps
     want, st = po.regional_run(d, _oparams(params), psi, steps)
     _same_region(info, st, d.dim)
     assert info.index == steps
+    assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,mk,field,params,steps", FUSABLE, ids=[c[0] for c in FUSABLE])
+def test_regional_fused_fp64_vs_oracle(gpu_lib, name, mk, field, params, steps):
+    """The fused box step (the canonical form) rounds differently from the
+    reference order: within 1e-12 relative L2 of the oracle, same box."""
+    d = mk(abi.PATH_FUSED)
+    psi = field(d)
+    print(f"regional fused {name}: steps {steps}", flush=True)
+    got, info = _run_gpu(d, params, psi, steps)
+    want, st = po.regional_run(d, _oparams(params), psi, steps)
+    _same_region(info, st, d.dim)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["mb2", "ac2", "mb3"])
+def test_regional_fused_spanning_box_equals_one_step_launches(gpu_lib, kind):
+    """A box that spans the domain runs the streamed kernels' arithmetic: FP32
+    results equal the plain plan at one step per launch bit for bit."""
+    if kind == "mb2":
+        d = models.model_b((64, 48), dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
+    elif kind == "ac2":
+        d = models.allen_cahn((64, 40), h=0.5, dt=0.01, dtype=abi.F32, path=abi.PATH_FUSED)
+    else:
+        d = models.model_b((32, 24, 20), dt=0.005, dtype=abi.F32, path=abi.PATH_FUSED,
+                           stencil=models.Stencil3d2h(7, 10, 25))
+    psi = inputs.uniform(interior_shape(d), seed=23).astype(np.float32)
+    print(f"regional fused spanning {kind}", flush=True)
+    got, info = _run_gpu(d, _params(factor=60.0, every=0.05), psi, 60)
+    assert [info.dims[a] for a in range(d.dim)] == [d.n[a] + 6 for a in range(d.dim)]
+    with FDPlan(d) as p:
+        if d.dim == 2:
+            p.tune(1, 16, 0, 2)
+        else:
+            p.tune(1, 4, 16, 0)
+        p.upload(psi)
+        p.step(60)
+        want = p.download()
     assert np.array_equal(got, want), np.abs(got - want).max()
 
 
