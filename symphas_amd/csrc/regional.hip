@@ -274,6 +274,69 @@ __global__ __launch_bounds__(256) void box_update(const T* __restrict__ psi,
   }
 }
 
+// The 2D fused box step in one pass: a workgroup stages psi on its 64 x 16
+// output tile plus a 2-site halo in LDS, evaluates mu~ on the tile plus a
+// 1-site halo there, then writes the update (layer sites copied).  Same
+// expressions as box_mu / box_update, so the same bits, with one launch and
+// one read and write of the box per step.
+constexpr int kBTX = 64, kBTY = 16;
+
+template <typename T, int R, bool HASA, int POLY>
+__global__ __launch_bounds__(256) void box_fused2d_tile(const T* __restrict__ psi,
+                                                        T* __restrict__ out,
+                                                        BoxGeom g, FusedCoef<T> c) {
+  constexpr int PX = kBTX + 4, PY = kBTY + 4;  // psi tile with a 2-site halo
+  constexpr int MX = kBTX + 2, MY = kBTY + 2;  // mu~ tile with a 1-site halo
+  __shared__ T sp[PY * PX];
+  __shared__ T sm[R == 2 ? MY * MX : 1];
+  const int x0 = blockIdx.x * kBTX, y0 = blockIdx.y * kBTY;
+  const int bx = g.b[0], by = g.b[1];
+  for (int e = threadIdx.x; e < PY * PX; e += 256) {
+    const int ty = e / PX, tx = e - ty * PX;
+    const int lx = x0 + tx - 2, ly = y0 + ty - 2;
+    sp[e] = (lx >= 0 && lx < bx && ly >= 0 && ly < by) ? psi[(int64_t)ly * bx + lx] : T(0);
+  }
+  __syncthreads();
+  if constexpr (R == 2) {
+    for (int e = threadIdx.x; e < MY * MX; e += 256) {
+      const int ty = e / MX, tx = e - ty * MX;
+      const int lx = x0 + tx - 1, ly = y0 + ty - 1;
+      if (lx < kB - 1 || lx >= bx - kB + 1 || ly < kB - 1 || ly >= by - kB + 1) continue;
+      const int i = (ty + 1) * PX + tx + 1;  // the site in sp
+      const T sn = (sp[i - 1] + sp[i + 1]) + (sp[i + PX] + sp[i - PX]);
+      const T x = sp[i];
+      const T t = horner<POLY - 1>(c.p, x);
+      sm[e] = fmad(t, x, POLY == 3 ? sn : c.ks * sn);
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < kBTY * kBTX; e += 256) {
+    const int ty = e / kBTX, tx = e - ty * kBTX;
+    const int lx = x0 + tx, ly = y0 + ty;
+    if (lx >= bx || ly >= by) continue;
+    const int ip = (ty + 2) * PX + tx + 2;
+    const int64_t o = (int64_t)ly * bx + lx;
+    const bool interior = lx >= kB && lx < bx - kB && ly >= kB && ly < by - kB;
+    if (!interior) {
+      out[o] = sp[ip];
+      continue;
+    }
+    if constexpr (R == 2) {
+      const int im = (ty + 1) * MX + tx + 1;
+      const T sn = (sm[im - 1] + sm[im + 1]) + (sm[im + MX] + sm[im - MX]);
+      T base = sp[ip];
+      if constexpr (HASA) base = base + horner<POLY>(c.q, base);
+      out[o] = fmad(c.k2, fmad(T(-4), sm[im], sn), base);
+    } else {
+      const T sn = (sp[ip - 1] + sp[ip + 1]) + (sp[ip + PX] + sp[ip - PX]);
+      const T x = sp[ip];
+      T base = x;
+      if constexpr (HASA) base = base + horner<POLY>(c.q, x);
+      out[o] = fmad(c.k1, fmad(T(-4), x, sn), base);
+    }
+  }
+}
+
 template <typename T, int DIM, int R, bool HASA, int POLY>
 int launch_box_fused(const T* psi, T* mu, T* out, const BoxGeom& g,
                      const FusedCoef<T>& c, int64_t sites, hipStream_t s);
@@ -335,6 +398,11 @@ template <typename T, int DIM, int R, bool HASA, int POLY>
 int launch_box_fused(const T* psi, T* mu, T* out, const BoxGeom& g,
                      const FusedCoef<T>& c, int64_t sites, hipStream_t s) {
   (void)sites;
+  if constexpr (DIM == 2) {
+    const dim3 tiles((g.b[0] + kBTX - 1) / kBTX, (g.b[1] + kBTY - 1) / kBTY);
+    box_fused2d_tile<T, R, HASA, POLY><<<tiles, 256, 0, s>>>(psi, out, g, c);
+    return check_launch("regional fused step");
+  }
   const dim3 grid((g.b[0] + 63) / 64, (g.b[1] + 3) / 4, DIM == 3 ? g.b[2] : 1);
   if constexpr (R == 2) {
     box_mu<T, DIM, POLY><<<grid, 256, 0, s>>>(psi, mu, g, c);
