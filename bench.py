@@ -191,7 +191,7 @@ def shape_sweep():
                 # the autotuner's extra heights: m 4-wave workgroups per CU
                 hl = (k * 2 + vec - 1) // vec
                 nstrips = -(-N // ((64 - 2 * hl) * vec))
-                for m in range(2, 7):
+                for m in range(1, 7):
                     nch = m * cus * 4 // nstrips
                     ty = -(-N // nch) if nch >= 1 else 0
                     if 4 <= ty <= 512 and ty not in rows_set:

@@ -8,8 +8,12 @@ namespace {
 // Halo layout: VEC values per lane (16 or 8 bytes), HL halo lanes per strip
 // side.  Halving VEC doubles the strips, so the same number of waves covers
 // chunks twice as tall, whose pipeline fill (2*K*R rows) costs less.
+// Waves per workgroup.  One-wave workgroups let the dispatcher spread the
+// waves evenly over the SIMDs when there are only one or two per SIMD.
+constexpr int kHaloWaves = 1;
+
 template <typename T, int VEC, int K, int R, bool HASA, int POLY>
-__global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
+__global__ __launch_bounds__(64 * kHaloWaves) void fused2d_halo(const T* __restrict__ in,
                                                     T* __restrict__ out, int X,
                                                     int Y, int TY, int nstrips,
                                                     int nitems,
@@ -20,7 +24,7 @@ __global__ __launch_bounds__(256) void fused2d_halo(const T* __restrict__ in,
   // the work item is wave-uniform: keep it (and the chunk bounds and row
   // loop derived from it) in scalar registers
   const int item = __builtin_amdgcn_readfirstlane(
-      xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+      xcd_remap(blockIdx.x, gridDim.x) * kHaloWaves + (threadIdx.x >> 6));
   if (item >= nitems) return;
   const int strip = item % nstrips;
   const int chunk = item / nstrips;
@@ -42,9 +46,9 @@ int launch_halo(const T* in, T* out, int X, int Y, int TY,
   const int nstrips = (X + W - 1) / W;
   const int nchunks = (Y + TY - 1) / TY;
   const int nitems = nstrips * nchunks;
-  const int nblocks = (nitems + 3) / 4;
+  const int nblocks = (nitems + kHaloWaves - 1) / kHaloWaves;
   fused2d_halo<T, VEC, K, R, HASA, POLY>
-      <<<nblocks, 256, 0, s>>>(in, out, X, Y, TY, nstrips, nitems, c);
+      <<<nblocks, 64 * kHaloWaves, 0, s>>>(in, out, X, Y, TY, nstrips, nitems, c);
   return SPS_OK;
 }
 

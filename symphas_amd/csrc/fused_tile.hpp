@@ -14,6 +14,8 @@
 // 64-2*HL inner lanes store.  Any X that is a multiple of VEC.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "lanes.hpp"
 
@@ -83,8 +85,78 @@ struct Stage {
   }
 };
 
+// Keep v in a VGPR from here on (an empty asm that "modifies" it), so the
+// compiler does not re-copy a scalar into a VGPR at every use.
+template <typename T>
+__device__ __forceinline__ void pin_vgpr(T& v) {
+  asm volatile("" : "+v"(v));
+}
+
+// Raw buffer access to one row of bytes (GFX9 descriptor: stride 0, 32-bit
+// data format).  A lane whose offset lies past the row reads 0 and its store
+// is dropped by the hardware, which masks halo lanes without a branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row,
+                                                          int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), (short)0,
+                                           bytes, 0x00020000);
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r,
+                                         unsigned off, T (&v)[VEC]) {
+  constexpr int B = VEC * (int)sizeof(T);
+  static_assert(B == 8 || B == 16, "8 or 16 bytes per lane");
+  if constexpr (B == 8) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    __builtin_memcpy(v, &x, B);
+  } else {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    __builtin_memcpy(v, &x, B);
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r,
+                                          unsigned off, const T (&v)[VEC]) {
+  constexpr int B = VEC * (int)sizeof(T);
+  static_assert(B == 8 || B == 16, "8 or 16 bytes per lane");
+  if constexpr (B == 8) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 x;
+    __builtin_memcpy(&x, v, B);
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, off, 0, 0);
+  } else {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 x;
+    __builtin_memcpy(&x, v, B);
+    __builtin_amdgcn_raw_buffer_store_b128(x, r, off, 0, 0);
+  }
+}
+
+// Calls f(std::integral_constant<int, U>) for U = B .. E-1, so that ring
+// slots and stream phases derived from U are compile-time constants.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // Process one strip x chunk.  col: first column this lane loads (already
 // wrapped into [0, X)); y0/ty: first output row and row count of the chunk.
+//
+// The chunk streams NI = ty + 2*K*R rows: stage s has valid input from
+// stream row 2*R*s on, and output row o leaves the last stage with stream
+// row o + 2*K*R.  The schedule is split so that the hot loop carries no
+// per-row control flow:
+//   - fill (rows 0 .. 2*K*R-1, unrolled): row i runs the stages whose input
+//     is valid (s <= i / (2*R)) and stores nothing;
+//   - steady (groups of D rows): every stage, every row stored;
+//   - last group: the same with a uniform check per row.
+// Rows are prefetched D ahead into a register ring (the row in slot
+// i mod D is replaced by row i + D as soon as it is read).  Loads wrap
+// modulo Y like the rows themselves, so every address is inside the field.
 template <typename T, int VEC, int K, int R, bool HASA, int POLY>
 __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
                                             T* __restrict__ out, int X, int Y,
@@ -92,7 +164,9 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
                                             bool store_lane,
                                             const FusedCoef<T>& c) {
   constexpr int LAG = K * R;
-  const int NI = ty + 2 * LAG;
+  constexpr int FILL = 2 * LAG;
+  constexpr int D = 9;  // a multiple of 3: slot and phase repeat together
+  const int NI = ty + FILL;
   int rload = (y0 - LAG) % Y;
   if (rload < 0) rload += Y;
 
@@ -100,48 +174,65 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
 #pragma unroll
   for (int s = 0; s < K; ++s) st[s].clear();
 
-  T bufA[3][VEC], bufB[3][VEC];
+  // A VOP3 reads one scalar operand: the coefficient that meets another
+  // scalar in the first FMA of a Horner chain is held in a VGPR.
+  FusedCoef<T> cv = c;
+  if constexpr (R == 2) pin_vgpr(cv.p[POLY - 2]);
+  if constexpr (HASA) pin_vgpr(cv.q[POLY - 1]);
 
-  auto load3 = [&](T (&buf)[3][VEC], int i) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (i + q < NI) {
-        load_vec<VEC>(in + (int64_t)rload * X + col, buf[q]);
-        rload = rload + 1 == Y ? 0 : rload + 1;
-      }
-    }
+  // rows are addressed by a wave-uniform descriptor per row and the lane's
+  // byte offset in it; lanes that do not store get an offset past the row
+  const int row_bytes = X * (int)sizeof(T);
+  const unsigned ld_off = (unsigned)col * (unsigned)sizeof(T);
+  const unsigned st_off = store_lane ? ld_off : 0x80000000u;
+  T ring[D][VEC];
+  auto load_row = [&](T (&b)[VEC]) {
+    buf_load<T, VEC>(row_rsrc(in + (int64_t)rload * X, row_bytes), ld_off, b);
+    rload = rload + 1 == Y ? 0 : rload + 1;
   };
-  auto process3 = [&](T (&buf)[3][VEC], int i) {
+  static_for<0, D>([&](auto u) { load_row(ring[decltype(u)::value]); });
+
+  // stream row with compile-time slot/phase index I through stages 0..NS-1
+  auto run_row = [&](auto ic, auto nsc, T (&v)[VEC]) {
+    constexpr int I = decltype(ic)::value, NS = decltype(nsc)::value;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (i + q < NI) {
+    for (int j = 0; j < VEC; ++j) v[j] = ring[I % D][j];
+    static_for<0, NS>([&](auto sc) {
+      st[decltype(sc)::value].template push<I % 3>(cv, v);
+    });
+  };
+
+  static_for<0, FILL>([&](auto ic) {
+    constexpr int I = decltype(ic)::value;
+    constexpr int NS = I / (2 * R) + 1 < K ? I / (2 * R) + 1 : K;
+    T v[VEC];
+    run_row(ic, std::integral_constant<int, NS>{}, v);
+    load_row(ring[I % D]);
+  });
+
+  T* dst = out + (int64_t)y0 * X;
+  int i = FILL;
+  for (; i + 2 * D <= NI; i += D) {
+    static_for<0, D>([&](auto uc) {
+      constexpr int I = FILL + decltype(uc)::value;
+      T v[VEC];
+      run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+      load_row(ring[I % D]);
+      buf_store<T, VEC>(row_rsrc(dst, row_bytes), st_off, v);
+      dst += X;
+    });
+  }
+  for (; i < NI; i += D) {
+    static_for<0, D>([&](auto uc) {
+      constexpr int U = decltype(uc)::value, I = FILL + U;
+      if (i + U < NI) {
         T v[VEC];
-#pragma unroll
-        for (int j = 0; j < VEC; ++j) v[j] = buf[q][j];
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-          // stage s has valid input from stream row 2*R*s on (each stage
-          // delays by R rows and needs 2*R rows of fill); before that it
-          // would only turn garbage into garbage, so it is skipped.  Its
-          // valid outputs depend only on pushes of valid rows.
-          if (s > 0 && i + q < 2 * R * s) continue;
-          if (q == 0) st[s].template push<0>(c, v);
-          if (q == 1) st[s].template push<1>(c, v);
-          if (q == 2) st[s].template push<2>(c, v);
-        }
-        const int orow = i + q - 2 * LAG;
-        if (orow >= 0 && store_lane)
-          store_vec<VEC>(out + (int64_t)(y0 + orow) * X + col, v);
+        run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+        if (i + U + D < NI) load_row(ring[I % D]);
+        buf_store<T, VEC>(row_rsrc(dst, row_bytes), st_off, v);
+        dst += X;
       }
-    }
-  };
-
-  load3(bufA, 0);
-  for (int i = 0; i < NI; i += 6) {
-    load3(bufB, i + 3);
-    process3(bufA, i);
-    load3(bufA, i + 6);
-    process3(bufB, i + 3);
+    });
   }
 }
 

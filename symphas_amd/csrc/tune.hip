@@ -135,7 +135,7 @@ extern "C" int sps_fd_autotune(sps_fd_plan* p, int use_cache, int* out_k,
                                             : std::vector<int>{16, 32, 64, 128};
         const int hl = (k * R + vec - 1) / vec;
         const int nstrips = (X + (64 - 2 * hl) * vec - 1) / ((64 - 2 * hl) * vec);
-        for (int m = 2; m <= 6; ++m) {
+        for (int m = 1; m <= 6; ++m) {
           const long nch = (long)m * cus * 4 / nstrips;
           if (nch < 1) continue;
           const long ty = (Y + nch - 1) / nch;
