@@ -92,47 +92,6 @@ __device__ __forceinline__ void pin_vgpr(T& v) {
   asm volatile("" : "+v"(v));
 }
 
-// Raw buffer access to one row of bytes (GFX9 descriptor: stride 0, 32-bit
-// data format).  A lane whose offset lies past the row reads 0 and its store
-// is dropped by the hardware, which masks halo lanes without a branch.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void* row,
-                                                          int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), (short)0,
-                                           bytes, 0x00020000);
-}
-
-template <typename T, int VEC>
-__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r,
-                                         unsigned off, T (&v)[VEC]) {
-  constexpr int B = VEC * (int)sizeof(T);
-  static_assert(B == 8 || B == 16, "8 or 16 bytes per lane");
-  if constexpr (B == 8) {
-    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    __builtin_memcpy(v, &x, B);
-  } else {
-    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    __builtin_memcpy(v, &x, B);
-  }
-}
-
-template <typename T, int VEC>
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r,
-                                          unsigned off, const T (&v)[VEC]) {
-  constexpr int B = VEC * (int)sizeof(T);
-  static_assert(B == 8 || B == 16, "8 or 16 bytes per lane");
-  if constexpr (B == 8) {
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    u32x2 x;
-    __builtin_memcpy(&x, v, B);
-    __builtin_amdgcn_raw_buffer_store_b64(x, r, off, 0, 0);
-  } else {
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    u32x4 x;
-    __builtin_memcpy(&x, v, B);
-    __builtin_amdgcn_raw_buffer_store_b128(x, r, off, 0, 0);
-  }
-}
-
 // Calls f(std::integral_constant<int, U>) for U = B .. E-1, so that ring
 // slots and stream phases derived from U are compile-time constants.
 template <int B, int E, typename F>
@@ -187,7 +146,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
   const unsigned st_off = store_lane ? ld_off : 0x80000000u;
   T ring[D][VEC];
   auto load_row = [&](T (&b)[VEC]) {
-    buf_load<T, VEC>(row_rsrc(in + (int64_t)rload * X, row_bytes), ld_off, b);
+    buf_load<T, VEC>(buffer_rsrc(in + (int64_t)rload * X, row_bytes), ld_off, 0, b);
     rload = rload + 1 == Y ? 0 : rload + 1;
   };
   static_for<0, D>([&](auto u) { load_row(ring[decltype(u)::value]); });
@@ -218,7 +177,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
       T v[VEC];
       run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
       load_row(ring[I % D]);
-      buf_store<T, VEC>(row_rsrc(dst, row_bytes), st_off, v);
+      buf_store<T, VEC>(buffer_rsrc(dst, row_bytes), st_off, 0, v);
       dst += X;
     });
   }
@@ -229,7 +188,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
         T v[VEC];
         run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
         if (i + U + D < NI) load_row(ring[I % D]);
-        buf_store<T, VEC>(row_rsrc(dst, row_bytes), st_off, v);
+        buf_store<T, VEC>(buffer_rsrc(dst, row_bytes), st_off, 0, v);
         dst += X;
       }
     });

@@ -135,6 +135,56 @@ __device__ __forceinline__ void store_vec(double* p, const double (&v)[VEC]) {
   }
 }
 
+// Raw buffer access (GFX9 descriptor: stride 0, 32-bit data format) to
+// `bytes` bytes from `base`: lane byte offset `off` plus a wave-uniform
+// offset `soff`.  A lane whose offset lies past the range reads 0 and its
+// store is dropped by the hardware, which masks lanes without a branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* base,
+                                                             int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           bytes, 0x00020000);
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_load(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                         int soff, T (&v)[VEC]) {
+  constexpr int B = VEC * (int)sizeof(T);
+  static_assert(B == 4 || B == 8 || B == 16, "4, 8 or 16 bytes per lane");
+  if constexpr (B == 4) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+    __builtin_memcpy(v, &x, B);
+  } else if constexpr (B == 8) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+    __builtin_memcpy(v, &x, B);
+  } else {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0);
+    __builtin_memcpy(v, &x, B);
+  }
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r,
+                                          unsigned off, int soff,
+                                          const T (&v)[VEC]) {
+  constexpr int B = VEC * (int)sizeof(T);
+  static_assert(B == 4 || B == 8 || B == 16, "4, 8 or 16 bytes per lane");
+  if constexpr (B == 4) {
+    unsigned x;
+    __builtin_memcpy(&x, v, B);
+    __builtin_amdgcn_raw_buffer_store_b32(x, r, off, soff, 0);
+  } else if constexpr (B == 8) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    u32x2 x;
+    __builtin_memcpy(&x, v, B);
+    __builtin_amdgcn_raw_buffer_store_b64(x, r, off, soff, 0);
+  } else {
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 x;
+    __builtin_memcpy(&x, v, B);
+    __builtin_amdgcn_raw_buffer_store_b128(x, r, off, soff, 0);
+  }
+}
+
 // Spread consecutive workgroup ids (which the dispatcher deals round-robin to
 // the 8 XCDs) so that each XCD receives a contiguous range of tiles and the
 // halo rows neighbouring tiles share stay in one L2.  XCD x receives blocks
