@@ -293,9 +293,10 @@ int sps_fd_copy_device(sps_fd_plan* plan, void* dev, int direction);
  * follows the box, not the domain.  desc->path selects the arithmetic as for
  * sps_fd_create: SPS_PATH_REFERENCE runs the reference order (FP64 results
  * equal the CPU SolverFT's on the box); the canonical form (SPS_PATH_AUTO
- * or SPS_PATH_FUSED) runs the fused form of the streamed kernels one step at
- * a time (a box that spans the domain equals sps_fd_step at one step per
- * launch bit for bit).
+ * or SPS_PATH_FUSED) runs the fused form of the streamed kernels (a box that
+ * spans the domain equals sps_fd_step at one step per launch bit for bit).
+ * A 2D box whose layer is frozen along both axes runs up to 4 such steps per
+ * launch between updates, with the same results as one step per launch.
  *
  * Update before each right-hand side (SolverSystemFDwSD::update,
  * update_regional_system, sol/inc/boundarysystem.h:437-479):
@@ -326,7 +327,10 @@ typedef struct {
   double resize_factor;     /* params::regional_resize_factor (2.0), >= 1 */
   double resize_time;       /* params::regional_resize_time (3.0), > 0 */
   int32_t fixed_resize;     /* params::regional_resize_is_fixed (1) */
-  int32_t reserved;
+  int32_t steps_per_launch; /* fused form, 2D box with a frozen layer: 0 or
+                               4 = four steps per launch between updates,
+                               1..3 = one (no counterpart in the reference;
+                               the results are the same) */
   double dims_relative_eps; /* REGIONAL_GRID_RELATIVE_DIMS_EPS (0.05) */
 } sps_fd_regional;         /* defaults: sol/src/boundarysystem.cpp:24-27,
                               sol/inc/boundarysystem.h:137 */

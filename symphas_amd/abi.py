@@ -64,7 +64,7 @@ class Regional(ctypes.Structure):
     # sps_fd_regional; defaults are the reference's parameters
     _fields_ = [("cutoff_eps", c_double), ("resize_factor", c_double),
                 ("resize_time", c_double), ("fixed_resize", c_int32),
-                ("reserved", c_int32), ("dims_relative_eps", c_double)]
+                ("steps_per_launch", c_int32), ("dims_relative_eps", c_double)]
 
 
 def regional_defaults() -> "Regional":

@@ -102,8 +102,20 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Process one strip x chunk.  col: first column this lane loads (already
-// wrapped into [0, X)); y0/ty: first output row and row count of the chunk.
+// Box mode of the stream (a regional grid's box, X x Y with a layer of
+// `depth` sites): rows outside [0, Y) and columns outside [0, X) read 0 (no
+// periodic wrap), and after each stage a site outside the interior
+// [depth, X-depth) x [depth, Y-depth) keeps its input value, so the layer
+// stays frozen.  cols: bit j set when the lane's value j lies in an interior
+// column.
+struct BoxMask {
+  int depth = 0;
+  unsigned cols = 0;
+};
+
+// Process one strip x chunk.  col: first column this lane loads (wrapped into
+// [0, X); in box mode it may lie outside); y0/ty: first output row and row
+// count of the chunk.
 //
 // The chunk streams NI = ty + 2*K*R rows: stage s has valid input from
 // stream row 2*R*s on, and output row o leaves the last stage with stream
@@ -116,18 +128,20 @@ __device__ __forceinline__ void static_for(F&& f) {
 // Rows are prefetched D ahead into a register ring (the row in slot
 // i mod D is replaced by row i + D as soon as it is read).  Loads wrap
 // modulo Y like the rows themselves, so every address is inside the field.
-template <typename T, int VEC, int K, int R, bool HASA, int POLY>
+template <typename T, int VEC, int K, int R, bool HASA, int POLY, bool BOX = false>
 __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
                                             T* __restrict__ out, int X, int Y,
                                             int y0, int ty, int col,
                                             bool store_lane,
-                                            const FusedCoef<T>& c) {
+                                            const FusedCoef<T>& c,
+                                            const BoxMask& bm = BoxMask{}) {
   constexpr int LAG = K * R;
   constexpr int FILL = 2 * LAG;
   constexpr int D = 9;  // a multiple of 3: slot and phase repeat together
   const int NI = ty + FILL;
-  int rload = (y0 - LAG) % Y;
-  if (rload < 0) rload += Y;
+  const int ybase = y0 - LAG;  // row of stream row 0
+  int rload = BOX ? ybase : ybase % Y;
+  if (!BOX && rload < 0) rload += Y;
 
   Stage<T, VEC, R, HASA, POLY> st[K];
 #pragma unroll
@@ -142,22 +156,43 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
   // rows are addressed by a wave-uniform descriptor per row and the lane's
   // byte offset in it; lanes that do not store get an offset past the row
   const int row_bytes = X * (int)sizeof(T);
-  const unsigned ld_off = (unsigned)col * (unsigned)sizeof(T);
+  const bool col_in = !BOX || (col >= 0 && col < X);
+  const unsigned ld_off = col_in ? (unsigned)col * (unsigned)sizeof(T) : 0x80000000u;
   const unsigned st_off = store_lane ? ld_off : 0x80000000u;
   T ring[D][VEC];
   auto load_row = [&](T (&b)[VEC]) {
-    buf_load<T, VEC>(buffer_rsrc(in + (int64_t)rload * X, row_bytes), ld_off, 0, b);
-    rload = rload + 1 == Y ? 0 : rload + 1;
+    if constexpr (BOX) {
+      // a row outside the box: an empty range, every lane reads 0
+      const bool row_in = (unsigned)rload < (unsigned)Y;
+      buf_load<T, VEC>(buffer_rsrc(in + (int64_t)(row_in ? rload : 0) * X,
+                                   row_in ? row_bytes : 0),
+                       ld_off, 0, b);
+      ++rload;
+    } else {
+      buf_load<T, VEC>(buffer_rsrc(in + (int64_t)rload * X, row_bytes), ld_off, 0, b);
+      rload = rload + 1 == Y ? 0 : rload + 1;
+    }
   };
   static_for<0, D>([&](auto u) { load_row(ring[decltype(u)::value]); });
 
-  // stream row with compile-time slot/phase index I through stages 0..NS-1
-  auto run_row = [&](auto ic, auto nsc, T (&v)[VEC]) {
+  // stream row `row` (compile-time slot/phase index I) through stages
+  // 0..NS-1; in box mode a stage's output outside the interior keeps the
+  // stage's input value (P[S0] of a radius-two stage, P[S1] of radius one)
+  auto run_row = [&](auto ic, auto nsc, int row, T (&v)[VEC]) {
     constexpr int I = decltype(ic)::value, NS = decltype(nsc)::value;
 #pragma unroll
     for (int j = 0; j < VEC; ++j) v[j] = ring[I % D][j];
     static_for<0, NS>([&](auto sc) {
-      st[decltype(sc)::value].template push<I % 3>(cv, v);
+      constexpr int S = decltype(sc)::value;
+      st[S].template push<I % 3>(cv, v);
+      if constexpr (BOX) {
+        const int yo = ybase + row - R * (S + 1);
+        const bool row_in = yo >= bm.depth && yo < Y - bm.depth;
+        constexpr int KEEP = R == 2 ? I % 3 : (I + 1) % 3;
+#pragma unroll
+        for (int j = 0; j < VEC; ++j)
+          if (!(row_in && ((bm.cols >> j) & 1u))) v[j] = st[S].P[KEEP][j];
+      }
     });
   };
 
@@ -165,7 +200,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
     constexpr int I = decltype(ic)::value;
     constexpr int NS = I / (2 * R) + 1 < K ? I / (2 * R) + 1 : K;
     T v[VEC];
-    run_row(ic, std::integral_constant<int, NS>{}, v);
+    run_row(ic, std::integral_constant<int, NS>{}, I, v);
     load_row(ring[I % D]);
   });
 
@@ -175,7 +210,8 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
     static_for<0, D>([&](auto uc) {
       constexpr int I = FILL + decltype(uc)::value;
       T v[VEC];
-      run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+      run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{},
+              i + decltype(uc)::value, v);
       load_row(ring[I % D]);
       buf_store<T, VEC>(buffer_rsrc(dst, row_bytes), st_off, 0, v);
       dst += X;
@@ -186,7 +222,7 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
       constexpr int U = decltype(uc)::value, I = FILL + U;
       if (i + U < NI) {
         T v[VEC];
-        run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+        run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, i + U, v);
         if (i + U + D < NI) load_row(ring[I % D]);
         buf_store<T, VEC>(buffer_rsrc(dst, row_bytes), st_off, 0, v);
         dst += X;

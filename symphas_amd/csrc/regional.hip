@@ -24,8 +24,10 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
+#include "fused_tile.hpp"
 #include "lanes.hpp"
 #include "plan.hpp"
 #include "ref_kernels.hpp"
@@ -394,6 +396,84 @@ bool cover_interval(const int* occ, int n, int& start, int& length) {
   return true;
 }
 
+// Four fused box steps per launch, for a 2D box whose layer is frozen along
+// both axes (no axis spans the domain): the row stream of the whole-domain
+// kernel (fused_tile.hpp) in box mode.  A wave owns a strip of 64 lanes x VEC
+// columns and streams a chunk of rows through the four stages; rows and
+// columns outside the box read 0, and after each stage a layer site keeps
+// its value.  Each site sees box_mu / box_update's expressions, so the
+// result equals four one-step launches bit for bit.
+constexpr int kBoxSteps = 4;
+
+template <typename T, int VEC, int R, bool HASA, int POLY>
+__global__ __launch_bounds__(64) void box_stream2d(const T* __restrict__ in,
+                                                   T* __restrict__ out, int bx, int by,
+                                                   int TY, int nstrips, int nitems,
+                                                   FusedCoef<T> c) {
+  constexpr int HL = (kBoxSteps * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  const int lane = threadIdx.x;
+  const int item = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x));
+  if (item >= nitems) return;
+  const int strip = item % nstrips;
+  const int chunk = item / nstrips;
+  const int cb = strip * W + (lane - HL) * VEC;
+  const bool store_lane = lane >= HL && lane < 64 - HL && cb < bx;
+  BoxMask bm;
+  bm.depth = kB;
+#pragma unroll
+  for (int j = 0; j < VEC; ++j)
+    if (cb + j >= kB && cb + j < bx - kB) bm.cols |= 1u << j;
+  const int y0 = chunk * TY;
+  const int ty = min(TY, by - y0);
+  stream_tile<T, VEC, kBoxSteps, R, HASA, POLY, true>(in, out, bx, by, y0, ty, cb,
+                                                       store_lane, c, bm);
+}
+
+// Chunk height of the box stream: about two waves per SIMD, at least 16 rows
+// (shorter chunks spend most of their rows on the pipeline fill).
+inline int box_stream_rows(int by, int nstrips, int cus) {
+  const int64_t target = 8LL * cus;  // waves
+  const int64_t nch = std::max<int64_t>(1, target / std::max(1, nstrips));
+  return (int)std::max<int64_t>(16, (by + nch - 1) / nch);
+}
+
+template <typename T, int VEC, int R, bool HASA, int POLY>
+int launch_box_stream(const T* psi, T* out, const BoxGeom& g, const FusedCoef<T>& c,
+                      int cus, hipStream_t s) {
+  constexpr int HL = (kBoxSteps * R + VEC - 1) / VEC;
+  constexpr int W = (64 - 2 * HL) * VEC;
+  const int bx = g.b[0], by = g.b[1];
+  const int nstrips = (bx + W - 1) / W;
+  const int TY = box_stream_rows(by, nstrips, cus);
+  const int nitems = nstrips * ((by + TY - 1) / TY);
+  box_stream2d<T, VEC, R, HASA, POLY><<<nitems, 64, 0, s>>>(psi, out, bx, by, TY, nstrips,
+                                                           nitems, c);
+  return check_launch("regional fused 4-step launch");
+}
+
+// Four fused steps of a frozen-layer 2D box.  FP32 boxes of even width take
+// two values per lane (8-byte aligned rows), the others one.
+template <typename T>
+int box_multi_step(const T* psi, T* out, const BoxGeom& g, const FusedCoef<T>& c,
+                   int cus, hipStream_t s) {
+  const bool hasa = c.dega >= 0;
+  const bool two = sizeof(T) == 4 && g.b[0] % 2 == 0;
+#define SPS_BOXS(V_, R_, A_, P_) launch_box_stream<T, V_, R_, A_, P_>(psi, out, g, c, cus, s)
+#define SPS_BOXR(V_)                                                                   \
+  (c.radius == 1                                                                       \
+       ? (c.poly == 3 ? (hasa ? SPS_BOXS(V_, 1, true, 3) : SPS_BOXS(V_, 1, false, 3))  \
+                      : (hasa ? SPS_BOXS(V_, 1, true, 6) : SPS_BOXS(V_, 1, false, 6))) \
+       : (c.poly == 3 ? (hasa ? SPS_BOXS(V_, 2, true, 3) : SPS_BOXS(V_, 2, false, 3))  \
+                      : (hasa ? SPS_BOXS(V_, 2, true, 6) : SPS_BOXS(V_, 2, false, 6))))
+  if constexpr (sizeof(T) == 4) {
+    if (two) return SPS_BOXR(2);
+  }
+  return SPS_BOXR(1);
+#undef SPS_BOXR
+#undef SPS_BOXS
+}
+
 template <typename T, int DIM, int R, bool HASA, int POLY>
 int launch_box_fused(const T* psi, T* mu, T* out, const BoxGeom& g,
                      const FusedCoef<T>& c, int64_t sites, hipStream_t s) {
@@ -455,6 +535,7 @@ struct sps_fdr_plan {
   int o[3] = {0, 0, 0}, b[3] = {1, 1, 1};
   bool has_data = false, has_box = true;
   int next_resize = 0, index = 0;
+  int cus = 256;  // compute units of the plan's device (box stream chunking)
   double empty = 0.0, cutoff = 0.0, time = 0.0;
 
   sps::BoxGeom geom() const {
@@ -619,6 +700,14 @@ struct sps_fdr_plan {
 
   int step_once() {
     SPS_TRY(update());
+    SPS_TRY(box_step());
+    ++index;
+    time += d.dt;  // run_model's time += dt (modules-models.h:489)
+    return SPS_OK;
+  }
+
+  // one step of the box (no update, no time advance)
+  int box_step() {
     if (has_box && fused) {
       const int nxt = cur ^ 1;
       const sps::BoxGeom g = geom();
@@ -654,8 +743,71 @@ struct sps_fdr_plan {
       if (rc != SPS_OK) return rc;
       cur = nxt;
     }
-    ++index;
-    time += d.dt;  // run_model's time += dt (modules-models.h:489)
+    return SPS_OK;
+  }
+
+  // Steps per run of the fused 2D box: a box whose layer is frozen along both
+  // axes (one spanning the domain refreshes its layer every step) runs 4
+  // steps per launch; rp.steps_per_launch below 4 keeps one step per launch.
+  int fused_run_limit() const {
+    if (!fused || !has_box || d.dim != 2) return 1;
+    if (rp.steps_per_launch != 0 && rp.steps_per_launch < sps::kBoxSteps) return 1;
+    for (int a = 0; a < 2; ++a)
+      if (b[a] - 2 * sps::kB == d.n[a]) return 1;
+    return sps::kBoxSteps;
+  }
+
+  // How many of the next steps, from this one on and at most `limit`, find
+  // nothing to do in update(): the first is the one whose update() just ran,
+  // each later one only if its time (accumulated as step_once does) has not
+  // reached the next resize count.
+  int quiet_steps(int limit) const {
+    int m = 1;
+    double t = time;
+    while (m < limit) {
+      t += d.dt;
+      if ((int)(t / rp.resize_time) >= next_resize) break;
+      ++m;
+    }
+    return m;
+  }
+
+  // n steps: runs of steps with no update work between them go through the
+  // multi-step box kernel (4 or 2 per launch), the rest one at a time.  Each
+  // site sees the same expressions either way.
+  int advance(int n) {
+    while (n > 0) {
+      SPS_TRY(update());
+      // decided after update(): a resize may have changed the box
+      const int limit = std::min(n, fused_run_limit());
+      const int m = limit >= 2 ? quiet_steps(limit) : 1;
+      int done = 0;
+      while (m - done >= sps::kBoxSteps) {
+        const int nxt = cur ^ 1;
+        const sps::BoxGeom g = geom();
+        const int rc =
+            d.dtype == SPS_F32
+                ? sps::box_multi_step<float>((const float*)box[cur], (float*)box[nxt], g,
+                                             fc32, cus, stream)
+                : sps::box_multi_step<double>((const double*)box[cur], (double*)box[nxt],
+                                              g, fc64, cus, stream);
+        if (rc != SPS_OK) return rc;
+        cur = nxt;
+        for (int j = 0; j < sps::kBoxSteps; ++j) {
+          ++index;
+          time += d.dt;
+        }
+        done += sps::kBoxSteps;
+      }
+      // the rest of the run one step per launch (their updates find nothing
+      // to do, and the first step's update already ran)
+      for (; done < m; ++done) {
+        SPS_TRY(box_step());
+        ++index;
+        time += d.dt;
+      }
+      n -= done;
+    }
     return SPS_OK;
   }
 
@@ -678,9 +830,11 @@ int validate_regional(const sps_fd_regional& r) {
       !std::isfinite(r.resize_factor) || !(r.resize_time > 0) ||
       !std::isfinite(r.resize_time) || !(r.dims_relative_eps >= 0) ||
       !std::isfinite(r.dims_relative_eps) ||
-      (r.fixed_resize != 0 && r.fixed_resize != 1)) {
+      (r.fixed_resize != 0 && r.fixed_resize != 1) || r.steps_per_launch < 0 ||
+      r.steps_per_launch > 4) {
     sps::set_error("regional parameters: cutoff_eps finite, resize_factor >= 1, "
-                   "resize_time > 0, dims_relative_eps >= 0, fixed_resize 0 or 1");
+                   "resize_time > 0, dims_relative_eps >= 0, fixed_resize 0 or 1, "
+                   "steps_per_launch 0..4");
     return SPS_E_INVALID;
   }
   return SPS_OK;
@@ -754,6 +908,16 @@ int sps_fdr_create(const sps_fd_desc* desc, const sps_fd_regional* params,
     p->stream = nullptr;
     set_error("hipStreamCreate failed: %s", hipGetErrorString(se));
     rc = SPS_E_HIP;
+  }
+  if (rc == SPS_OK) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess ||
+        p->cus < 1) {
+      set_error("device query failed");
+      rc = SPS_E_HIP;
+    }
   }
   alloc(&p->box[0], p->cap * p->esize);
   alloc(&p->box[1], p->cap * p->esize);
@@ -838,8 +1002,7 @@ int sps_fdr_step(sps_fdr_plan* p, int nsteps) {
     set_error("no field uploaded");
     return SPS_E_STATE;
   }
-  for (int i = 0; i < nsteps; ++i) SPS_TRY(p->step_once());
-  return SPS_OK;
+  return p->advance(nsteps);
 }
 
 int sps_fdr_download(sps_fdr_plan* p, void* host, int layout) {

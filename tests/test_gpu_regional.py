@@ -25,8 +25,8 @@ def _blob(shape_np, lo, size, value=0.5, noise_seed=None):
     return psi
 
 
-def _params(fixed=1, factor=2.0, every=3.0, eps=1e-6, rel=0.05):
-    return abi.Regional(eps, factor, every, fixed, 0, rel)
+def _params(fixed=1, factor=2.0, every=3.0, eps=1e-6, rel=0.05, spl=0):
+    return abi.Regional(eps, factor, every, fixed, spl, rel)
 
 
 def _oparams(p):
@@ -54,7 +54,7 @@ def test_regional_rejects_bad_arguments_before_any_hip_call():
     assert lib.sps_fdr_create(ctypes.byref(d), None, ctypes.byref(h)) == abi.E_UNSUPPORTED
     d = models.model_b((16, 16), dtype=abi.F64)
     for bad in (_params(factor=0.5), _params(every=0.0), _params(fixed=2),
-                _params(rel=-1.0), _params(eps=float("nan"))):
+                _params(rel=-1.0), _params(eps=float("nan")), _params(spl=5), _params(spl=-1)):
         assert lib.sps_fdr_create(ctypes.byref(d), ctypes.byref(bad), ctypes.byref(h)) == abi.E_INVALID
         assert b"regional parameters" in lib.sps_fd_last_error()
 
@@ -136,6 +136,50 @@ def test_regional_fused_spanning_box_equals_one_step_launches(gpu_lib, kind):
         p.step(60)
         want = p.download()
     assert np.array_equal(got, want), np.abs(got - want).max()
+
+
+MULTI = [
+    # name, desc factory, field factory, params (resizes cut the runs of steps)
+    ("mb2_free", lambda dt: models.model_b((96, 80), dt=0.01, dtype=dt, path=abi.PATH_FUSED),
+     lambda d: _blob(interior_shape(d), (30, 24), (14, 10), noise_seed=11),
+     dict(fixed=0, every=0.13)),
+    ("mb2_fixed", lambda dt: models.model_b((64, 48), dt=0.01, dtype=dt, path=abi.PATH_FUSED),
+     lambda d: _blob(interior_shape(d), (20, 14), (12, 8), noise_seed=1), dict(every=0.05)),
+    ("ac2_wrapping", lambda dt: models.allen_cahn((72, 64), h=0.5, dt=0.01, dtype=dt,
+                                                  path=abi.PATH_FUSED),
+     lambda d: _blob(interior_shape(d), (60, 50), (16, 16), value=0.8, noise_seed=3) - 0.8,
+     dict(fixed=0, every=0.07, factor=1.5)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [abi.F32, abi.F64], ids=["f32", "f64"])
+@pytest.mark.parametrize("name,mk,field,kw", MULTI, ids=[c[0] for c in MULTI])
+def test_regional_multi_step_equals_one_step_launches(gpu_lib, name, mk, field, kw, dtype):
+    """A box whose layer is frozen runs up to 4 fused steps per launch
+    between updates: the same field, box and time as one step per launch,
+    bit for bit, whatever the run lengths (steps per call, resizes)."""
+    d = mk(dtype)
+    psi = field(d)
+    if dtype == abi.F32:
+        psi = psi.astype(np.float32)
+    calls = [7, 13, 1, 4, 22, 3, 10]
+    results = {}
+    for spl in (1, 4, 0):
+        print(f"regional multi-step {name} dtype {dtype} steps_per_launch {spl}", flush=True)
+        with RegionalPlan(d, _params(spl=spl, **kw)) as rp:
+            rp.upload(psi)
+            for n in calls:
+                rp.step(n)
+            info = rp.region()
+            results[spl] = (rp.download(), list(info.origin)[:2], list(info.dims)[:2],
+                            info.time, info.index)
+    one = results[1]
+    assert 0 < one[2][0] - 6 < d.n[0] and 0 < one[2][1] - 6 < d.n[1]  # frozen layer
+    for spl in (4, 0):
+        got = results[spl]
+        assert got[1:] == one[1:]
+        assert np.array_equal(got[0], one[0]), np.abs(got[0] - one[0]).max()
 
 
 @pytest.mark.gpu
