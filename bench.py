@@ -788,9 +788,10 @@ def extra_regional(torch, stream):
             "speedup_fused": t_fused / t_regf,
             "domain_Gsite_updates_per_s_equivalent_fused": sites * steps / t_regf / 1e9,
             "note": "the regional plan steps only the box, in the reference order "
-                    "(bit-identical to the CPU SolverFT in FP64) or the fused form one step "
-                    "per pass pair; rates are quoted per domain site for comparison with "
-                    "whole-domain plans (the fused whole-domain plan fuses 4 steps per launch)"}
+                    "(bit-identical to the CPU SolverFT in FP64) or the fused form (four steps "
+                    "per launch between resizes while the box's layer is frozen); rates are "
+                    "quoted per domain site for comparison with whole-domain plans (the fused "
+                    "whole-domain plan also fuses 4 steps per launch)"}
 
 
 def extra_configs(torch, stream, args):
