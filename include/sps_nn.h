@@ -46,8 +46,11 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
 
 /* Weight gradients are deterministic: each workgroup reduces its share of
  * the pixels in a fixed order into its own partial gradient in a caller-owned
- * FP32 workspace, and one pass then adds the partials to dweight in
- * workgroup order.  The same inputs give the same bits on every call.
+ * FP32 workspace; a fixed tree then adds the partials to dweight (runs of
+ * 32 partials summed in order, in place in the workspace, until at most 32
+ * remain, which are added to dweight in order).  The same inputs give the
+ * same bits on every call.  The workspace is scratch: its contents are
+ * consumed.
  * sps_nn_wgrad_workspace gives the workspace size in floats for a layer
  * (kind SPS_NN_WGRAD_CONV with cin and cout, SPS_NN_WGRAD_IN1 with cout,
  * SPS_NN_WGRAD_OUT1 with cin); 0 for invalid sizes. */

@@ -881,15 +881,32 @@ __global__ __launch_bounds__(256) void out1_wgrad_tile(
   }
 }
 
-// dw[i] += sum over z of part[z][i], z in order: the deterministic sum of the
-// per-block partial weight gradients.
+// The deterministic sum of the per-block partial weight gradients, as a
+// fixed tree: the live partials sit in slots m*S of the workspace (m < M,
+// `count` floats per slot).  sum_partial_groups adds each run of gs of them,
+// in order, into the run's first slot (one thread per (run, weight), in
+// place); add_partials then adds the last M <= gs, in order, into dw.
+__global__ __launch_bounds__(256) void sum_partial_groups(float* __restrict__ part,
+                                                          int64_t count, int M,
+                                                          int64_t S, int gs) {
+  const int64_t ng = (M + gs - 1) / gs;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < ng * count;
+       t += (int64_t)gridDim.x * 256) {
+    const int64_t g = t / count, i = t - g * count;
+    const int m0 = (int)g * gs, m1 = min(M, m0 + gs);
+    float s = 0.f;
+    for (int m = m0; m < m1; ++m) s += part[(int64_t)m * S * count + i];
+    part[(int64_t)m0 * S * count + i] = s;
+  }
+}
+
 __global__ __launch_bounds__(256) void add_partials(const float* __restrict__ part,
-                                                    int nz, int64_t count,
+                                                    int M, int64_t S, int64_t count,
                                                     float* __restrict__ dw) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * 256) {
     float s = 0.f;
-    for (int z = 0; z < nz; ++z) s += part[(int64_t)z * count + i];
+    for (int m = 0; m < M; ++m) s += part[(int64_t)m * S * count + i];
     dw[i] += s;
   }
 }
@@ -1018,9 +1035,19 @@ WgradSplit out1_split(int n, int h, int w, int c) {
   return {(int)((tiles + per - 1) / per), (int)per, (int64_t)9 * c};
 }
 
-int add_parts(const float* part, const WgradSplit& sp, float* dw,
-              hipStream_t s) {
-  add_partials<<<grid_for(sp.count), 256, 0, s>>>(part, sp.blocks, sp.count, dw);
+// The workspace's partials are consumed (summed in place).
+int add_parts(float* part, const WgradSplit& sp, float* dw, hipStream_t s) {
+  constexpr int kGroup = 32;
+  int M = sp.blocks;
+  int64_t S = 1;
+  while (M > kGroup) {
+    const int64_t ng = (M + kGroup - 1) / kGroup;
+    sum_partial_groups<<<grid_for(ng * sp.count), 256, 0, s>>>(part, sp.count, M, S, kGroup);
+    SPS_TRY(launch_ok("weight-gradient partial sum"));
+    M = (int)ng;
+    S *= kGroup;
+  }
+  add_partials<<<grid_for(sp.count), 256, 0, s>>>(part, M, S, sp.count, dw);
   return launch_ok("weight-gradient partial sum");
 }
 
