@@ -149,6 +149,15 @@ MULTI = [
                                                   path=abi.PATH_FUSED),
      lambda d: _blob(interior_shape(d), (60, 50), (16, 16), value=0.8, noise_seed=3) - 0.8,
      dict(fixed=0, every=0.07, factor=1.5)),
+    # boxes wider than one strip: 152..196 columns (even: two values per lane
+    # in FP32) and 79..85 columns (odd: one value per lane)
+    ("mb2_wide", lambda dt: models.model_b((256, 160), dt=0.01, dtype=dt, path=abi.PATH_FUSED),
+     lambda d: _blob(interior_shape(d), (60, 40), (70, 30), noise_seed=5),
+     dict(fixed=0, every=0.11)),
+    ("ac2_wide_odd", lambda dt: models.allen_cahn((200, 120), h=0.5, dt=0.01, dtype=dt,
+                                                  path=abi.PATH_FUSED),
+     lambda d: _blob(interior_shape(d), (40, 30), (47, 25), value=0.8, noise_seed=5) - 0.8,
+     dict(fixed=0, every=0.09, factor=1.5)),
 ]
 
 
