@@ -12,8 +12,9 @@
 #include <type_traits>
 #include <vector>
 
-#include "lanes.hpp"
 #include "ckpt.hpp"
+#include "fused_tile.hpp"
+#include "lanes.hpp"
 #include "multi.hpp"
 #include "plan.hpp"
 #include "textio.hpp"
@@ -200,51 +201,70 @@ __global__ __launch_bounds__(256) void fused_multi(
 #pragma unroll
   for (int s = 0; s < K; ++s) st[s].clear();
 
-  T bufA[3][2][VEC], bufB[3][2][VEC];
-  auto load3 = [&](T (&buf)[3][2][VEC], int i) {
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (i + q < NI) {
-        const int64_t o = (int64_t)rload * X + col;
-        load_vec<VEC>(in0 + o, buf[q][0]);
-        load_vec<VEC>(in1 + o, buf[q][1]);
-        rload = rload + 1 == Y ? 0 : rload + 1;
-      }
-    }
+  // The schedule of the single-field stream (stream_tile): the fill rows
+  // (0 .. 2*K*R-1, stage s from row 2*R*s on) unrolled, then groups of D
+  // rows with no control flow, then the last group with a check per row.
+  // Rows are prefetched D ahead into a register ring (both fields), and
+  // loaded and stored through a wave-uniform buffer descriptor per field
+  // and row; lanes that do not store get an offset past the row, whose
+  // stores the hardware drops.
+  constexpr int FILL = 2 * LAG;
+  constexpr int D = 3;  // a multiple of 3 (phase = row mod 3)
+  const int row_bytes = X * (int)sizeof(T);
+  const unsigned ld_off = (unsigned)col * (unsigned)sizeof(T);
+  const unsigned st_off = store_lane ? ld_off : 0x80000000u;
+  T ring[D][2][VEC];
+  auto load_row = [&](T (&b)[2][VEC]) {
+    const int64_t o = (int64_t)rload * X;
+    buf_load<T, VEC>(buffer_rsrc(in0 + o, row_bytes), ld_off, 0, b[0]);
+    buf_load<T, VEC>(buffer_rsrc(in1 + o, row_bytes), ld_off, 0, b[1]);
+    rload = rload + 1 == Y ? 0 : rload + 1;
   };
-  auto process3 = [&](T (&buf)[3][2][VEC], int i) {
+  static_for<0, D>([&](auto u) { load_row(ring[decltype(u)::value]); });
+
+  auto run_row = [&](auto ic, auto nsc, T (&v)[2][VEC]) {
+    constexpr int I = decltype(ic)::value, NS = decltype(nsc)::value;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      if (i + q < NI) {
+    for (int f = 0; f < 2; ++f)
+#pragma unroll
+      for (int j = 0; j < VEC; ++j) v[f][j] = ring[I % D][f][j];
+    static_for<0, NS>([&](auto sc) {
+      st[decltype(sc)::value].template push<I % 3>(c, v);
+    });
+  };
+  auto store_row = [&](int orow, const T (&v)[2][VEC]) {
+    const int64_t o = (int64_t)(y0 + orow) * X;
+    buf_store<T, VEC>(buffer_rsrc(out0 + o, row_bytes), st_off, 0, v[0]);
+    buf_store<T, VEC>(buffer_rsrc(out1 + o, row_bytes), st_off, 0, v[1]);
+  };
+
+  static_for<0, FILL>([&](auto ic) {
+    constexpr int I = decltype(ic)::value;
+    constexpr int NS = I / (2 * R) + 1 < K ? I / (2 * R) + 1 : K;
+    T v[2][VEC];
+    run_row(ic, std::integral_constant<int, NS>{}, v);
+    load_row(ring[I % D]);
+  });
+  int i = FILL;
+  for (; i + 2 * D <= NI; i += D) {
+    static_for<0, D>([&](auto uc) {
+      constexpr int U = decltype(uc)::value, I = FILL + U;
+      T v[2][VEC];
+      run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+      load_row(ring[I % D]);
+      store_row(i + U - FILL, v);
+    });
+  }
+  for (; i < NI; i += D) {
+    static_for<0, D>([&](auto uc) {
+      constexpr int U = decltype(uc)::value, I = FILL + U;
+      if (i + U < NI) {
         T v[2][VEC];
-#pragma unroll
-        for (int f = 0; f < 2; ++f)
-#pragma unroll
-          for (int j = 0; j < VEC; ++j) v[f][j] = buf[q][f][j];
-#pragma unroll
-        for (int s = 0; s < K; ++s) {
-          // stage s has valid input from stream row 2*R*s on (fill rows
-          // before that are skipped; valid outputs never read them)
-          if (s > 0 && i + q < 2 * R * s) continue;
-          if (q == 0) st[s].template push<0>(c, v);
-          if (q == 1) st[s].template push<1>(c, v);
-          if (q == 2) st[s].template push<2>(c, v);
-        }
-        const int orow = i + q - 2 * LAG;
-        if (orow >= 0 && store_lane) {
-          const int64_t o = (int64_t)(y0 + orow) * X + col;
-          store_vec<VEC>(out0 + o, v[0]);
-          store_vec<VEC>(out1 + o, v[1]);
-        }
+        run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{}, v);
+        if (i + U + D < NI) load_row(ring[I % D]);
+        store_row(i + U - FILL, v);
       }
-    }
-  };
-  load3(bufA, 0);
-  for (int i = 0; i < NI; i += 6) {
-    load3(bufB, i + 3);
-    process3(bufA, i);
-    load3(bufA, i + 6);
-    process3(bufB, i + 3);
+    });
   }
 }
 
@@ -1355,7 +1375,7 @@ int sps_fdm_autotune(sps_fdm_plan* p, int* out_k, int* out_rows,
         const int hl = (k * p->low.radius + vec - 1) / vec;
         const int w = (64 - 2 * hl) * vec;
         const int nstrips = (p->d.n[0] + w - 1) / w;
-        for (int m = 2; m <= 6; ++m) {
+        for (int m = 1; m <= 6; ++m) {
           const long nch = (long)m * cus * 4 / nstrips;
           if (nch < 1) continue;
           const long ty = (p->d.n[1] + nch - 1) / nch;
