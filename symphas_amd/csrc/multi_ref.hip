@@ -123,6 +123,14 @@ __device__ __forceinline__ int wrap(int c, int n) {
   return c < 0 ? c + n : c;
 }
 
+// c in [-2, n + 2): a neighbour at most two sites away.  For n >= 3 one
+// conditional add or subtract gives the same index as wrap without an
+// integer division.
+__device__ __forceinline__ int wrap_near(int c, int n) {
+  if (n < 3) return wrap(c, n);
+  return c < 0 ? c + n : (c >= n ? c - n : c);
+}
+
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void ref_multi_kernel(MRefParams<T> p) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -133,9 +141,9 @@ __global__ __launch_bounds__(256) void ref_multi_kernel(MRefParams<T> p) {
   Offsets o;
 #pragma unroll
   for (int d = -2; d <= 2; ++d) {
-    o.xo[d + 2] = wrap(x + d, p.n[0]);
-    o.yo[d + 2] = (int64_t)wrap(y + d, p.n[1]) * nx;
-    o.zo[d + 2] = DIM == 3 ? (int64_t)wrap(z + d, p.n[2]) * nxy : 0;
+    o.xo[d + 2] = wrap_near(x + d, p.n[0]);
+    o.yo[d + 2] = (int64_t)wrap_near(y + d, p.n[1]) * nx;
+    o.zo[d + 2] = DIM == 3 ? (int64_t)wrap_near(z + d, p.n[2]) * nxy : 0;
   }
   const int64_t idx = x + nx * y + nxy * z;
   for (int f = 0; f < p.nfields; ++f) {

@@ -143,6 +143,14 @@ __device__ __forceinline__ int wrap(int c, int n) {
   return c < 0 ? c + n : c;
 }
 
+// c in [-2, n + 2): a neighbour at most two sites away.  For n >= 3 one
+// conditional add or subtract gives the same index as wrap without an
+// integer division.
+__device__ __forceinline__ int wrap_near(int c, int n) {
+  if (n < 3) return wrap(c, n);
+  return c < 0 ? c + n : (c >= n ? c - n : c);
+}
+
 // The value one site takes: mode 0 psi + dt*rhs, 1 rhs, 2 one derivative of
 // psi.  `in` may point to global memory or to LDS (flat addressing); the
 // arithmetic is the same either way.
@@ -163,9 +171,9 @@ __device__ __forceinline__ T site_value(const RefParams<T>& p, const T* in,
       w.yo[d + 2] = (int64_t)(y + d) * nx;
       w.zo[d + 2] = DIM == 3 ? (int64_t)(z + d) * nxy : 0;
     } else {
-      w.xo[d + 2] = wrap(x + d, p.n[0]);
-      w.yo[d + 2] = (int64_t)wrap(y + d, p.n[1]) * nx;
-      w.zo[d + 2] = DIM == 3 ? (int64_t)wrap(z + d, p.n[2]) * nxy : 0;
+      w.xo[d + 2] = wrap_near(x + d, p.n[0]);
+      w.yo[d + 2] = (int64_t)wrap_near(y + d, p.n[1]) * nx;
+      w.zo[d + 2] = DIM == 3 ? (int64_t)wrap_near(z + d, p.n[2]) * nxy : 0;
     }
   }
   const int64_t idx = x + nx * y + nxy * z;
