@@ -30,18 +30,25 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// IEEE round-to-nearest-even FP32 -> BF16 (quiet NaN preserved).
+// FP32 -> BF16, round to nearest even: gfx950's v_cvt_pk_bf16_f32 (a NaN
+// stays a quiet NaN).  pack_bf16 converts two values in one instruction.
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2v{lo, hi}, bf16x2v));
 }
 __device__ __forceinline__ float bf2f(uint16_t h) {
   return __uint_as_float((uint32_t)h << 16);
 }
 
-__device__ __forceinline__ float silu(float v) { return v / (1.0f + __expf(-v)); }
+// v * sigmoid(v) with the hardware reciprocal (1 ulp; the result is rounded
+// to BF16 or feeds BF16 data)
+__device__ __forceinline__ float silu(float v) {
+  return v * __builtin_amdgcn_rcpf(1.0f + __expf(-v));
+}
 
 __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
@@ -50,8 +57,8 @@ __device__ __forceinline__ bf16x8 load_frag(const uint16_t* p) {
 // 4 consecutive BF16 values as one 8-byte access (callers keep 8-byte
 // alignment: channel offsets are multiples of 4)
 __device__ __forceinline__ void store4(uint16_t* p, const float (&v)[4]) {
-  const uint32_t lo = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
-  const uint32_t hi = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+  const uint32_t lo = pack_bf16(v[0], v[1]);
+  const uint32_t hi = pack_bf16(v[2], v[3]);
   *reinterpret_cast<uint2*>(p) = make_uint2(lo, hi);
 }
 __device__ __forceinline__ void load4(const uint16_t* p, float (&v)[4]) {
@@ -66,7 +73,7 @@ __device__ __forceinline__ void store8(uint16_t* p, const float (&v)[8]) {
   uint32_t w[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    w[i] = (uint32_t)f2bf(v[2 * i]) | ((uint32_t)f2bf(v[2 * i + 1]) << 16);
+    w[i] = pack_bf16(v[2 * i], v[2 * i + 1]);
   *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 __device__ __forceinline__ void load8(const uint16_t* p, float (&v)[8]) {
@@ -920,7 +927,7 @@ __global__ __launch_bounds__(256) void silu_bwd_k(const uint16_t* __restrict__ d
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * 256) {
     const float z = bf2f(pre[i]);
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
     dx[i] = f2bf(bf2f(dy[i]) * s * (1.0f + z * (1.0f - s)));
   }
 }
