@@ -206,6 +206,18 @@ __device__ __forceinline__ void stream_tile(const T* __restrict__ in,
 
   T* dst = out + (int64_t)y0 * X;
   int i = FILL;
+  // two groups per iteration while three fit, then one
+  for (; i + 3 * D <= NI; i += 2 * D) {
+    static_for<0, 2 * D>([&](auto uc) {
+      constexpr int I = FILL + decltype(uc)::value;
+      T v[VEC];
+      run_row(std::integral_constant<int, I>{}, std::integral_constant<int, K>{},
+              i + decltype(uc)::value, v);
+      load_row(ring[I % D]);
+      buf_store<T, VEC>(buffer_rsrc(dst, row_bytes), st_off, 0, v);
+      dst += X;
+    });
+  }
   for (; i + 2 * D <= NI; i += D) {
     static_for<0, D>([&](auto uc) {
       constexpr int I = FILL + decltype(uc)::value;
