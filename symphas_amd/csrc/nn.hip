@@ -437,8 +437,8 @@ __global__ __launch_bounds__(256) void conv_wgrad(
     int H, int W, int C, int K, float* __restrict__ part, int nchunks,
     int chunks_per_block) {
   constexpr int TK = KT / 32, TC = CT / 32;  // MFMA tiles per wave
-  __shared__ uint16_t sdy[32][KT + 2];
-  __shared__ uint16_t sx[ROWS ? 3 : 9][ROWS ? 34 : 32][CT + 2];
+  __shared__ __align__(16) uint16_t sdy[32][KT + 2];
+  __shared__ __align__(16) uint16_t sx[ROWS ? 3 : 9][ROWS ? 34 : 32][CT + 2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c0 = blockIdx.x * CT, k0 = blockIdx.y * KT;
   const int kw = (wave & 1) * (KT / 2), cw = (wave >> 1) * (CT / 2);
@@ -455,41 +455,51 @@ __global__ __launch_bounds__(256) void conv_wgrad(
   const int last = min(nchunks, first + chunks_per_block);
   for (int ch = first; ch < last; ++ch) {
     __syncthreads();
+    // staging in pieces of 8 channels: one 16-byte global load, written to
+    // the padded LDS row as four 32-bit stores (rows are 4-byte aligned)
+    auto put8 = [](uint16_t* dst, const uint16_t* src) {
+      const u32x4 q = *reinterpret_cast<const u32x4*>(src);
+      uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+      d[0] = q.x;
+      d[1] = q.y;
+      d[2] = q.z;
+      d[3] = q.w;
+    };
     if constexpr (ROWS) {
       const int n = ch / (H * rowchunks);
       const int rem = ch - n * H * rowchunks;
       const int h = rem / rowchunks;
       const int w0 = (rem - h * rowchunks) * 32;
       // stage dy: 32 pixels x KT channels
-      for (int e = threadIdx.x; e < 32 * KT; e += 256) {
-        const int px = e / KT, k = e - px * KT;
-        sdy[px][k] = dy[((int64_t)(n * H + h) * W + w0 + px) * K + k0 + k];
+      for (int e = threadIdx.x; e < 32 * (KT / 8); e += 256) {
+        const int px = e / (KT / 8), k = (e - px * (KT / 8)) * 8;
+        put8(&sdy[px][k], dy + ((int64_t)(n * H + h) * W + w0 + px) * K + k0 + k);
       }
       // stage x: rows h-1..h+1, columns w0-1..w0+32, CT channels
-      for (int e = threadIdx.x; e < 3 * 34 * CT; e += 256) {
-        const int r = e / (34 * CT);
-        const int e2 = e - r * 34 * CT;
-        const int col = e2 / CT, c = e2 - col * CT;
+      for (int e = threadIdx.x; e < 3 * 34 * (CT / 8); e += 256) {
+        const int r = e / (34 * (CT / 8));
+        const int e2 = e - r * 34 * (CT / 8);
+        const int col = e2 / (CT / 8), c = (e2 - col * (CT / 8)) * 8;
         const int hh = wrapi(h + r - 1, H), ww = wrapi(w0 + col - 1, W);
-        sx[r][col][c] = x[((int64_t)(n * H + hh) * W + ww) * C + c0 + c];
+        put8(&sx[r][col][c], x + ((int64_t)(n * H + hh) * W + ww) * C + c0 + c);
       }
     } else {
       const int64_t pbase = (int64_t)ch * 32;
-      for (int e = threadIdx.x; e < 32 * KT; e += 256) {
-        const int px = e / KT, k = e - px * KT;
-        sdy[px][k] = dy[(pbase + px) * K + k0 + k];
+      for (int e = threadIdx.x; e < 32 * (KT / 8); e += 256) {
+        const int px = e / (KT / 8), k = (e - px * (KT / 8)) * 8;
+        put8(&sdy[px][k], dy + (pbase + px) * K + k0 + k);
       }
       const int HW = H * W;
-      for (int e = threadIdx.x; e < 9 * 32 * CT; e += 256) {
-        const int t = e / (32 * CT);
-        const int e2 = e - t * 32 * CT;
-        const int px = e2 / CT, c = e2 - px * CT;
+      for (int e = threadIdx.x; e < 9 * 32 * (CT / 8); e += 256) {
+        const int t = e / (32 * (CT / 8));
+        const int e2 = e - t * 32 * (CT / 8);
+        const int px = e2 / (CT / 8), c = (e2 - px * (CT / 8)) * 8;
         const int64_t p = pbase + px;
         const int n = (int)(p / HW);
         const int r = (int)(p - (int64_t)n * HW);
         const int h = r / W, w = r - h * W;
         const int hh = wrapi(h + t / 3 - 1, H), ww = wrapi(w + t % 3 - 1, W);
-        sx[t][px][c] = x[((int64_t)(n * H + hh) * W + ww) * C + c0 + c];
+        put8(&sx[t][px][c], x + ((int64_t)(n * H + hh) * W + ww) * C + c0 + c);
       }
     }
     __syncthreads();
