@@ -52,7 +52,7 @@ extern "C" {
 typedef enum {
   SPS_OK = 0,
   SPS_E_INVALID = -1,     /* bad argument or descriptor */
-  SPS_E_UNSUPPORTED = -2, /* valid request this build does not implement */
+  SPS_E_UNSUPPORTED = -2, /* valid request this library does not implement */
   SPS_E_HIP = -3,         /* HIP runtime error */
   SPS_E_NOMEM = -4,       /* device or host allocation failed */
   SPS_E_STATE = -5        /* call made in the wrong state (e.g. no data) */
