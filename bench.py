@@ -475,6 +475,7 @@ def extra_cfg5(torch, stream, iters):
     err = sg.rel_l2(out, plain)
     t_net = sg.time_network(net, start)
     flops = net.flops_per_sample(n, n) * batch
+    t_roof = net.roofline_seconds(n, n, batch)
     # surrogate alone (no checks) against the plain solver
     with torch.no_grad():
         pred = net(start)
@@ -499,7 +500,14 @@ def extra_cfg5(torch, stream, iters):
             "checks": rep.checks, "fallback_samples": rep.fallbacks,
             "network_forward_seconds": t_net,
             "mfma": {"flops_per_forward": flops, "achieved_TFLOPs": flops / t_net / 1e12,
-                     "peak_TFLOPs": 2500.0, "utilisation": flops / t_net / 2.5e15},
+                     "peak_TFLOPs": 2500.0, "utilisation": flops / t_net / 2.5e15,
+                     "combined_roofline_seconds": t_roof,
+                     "frac_of_combined_roofline": t_roof / t_net,
+                     "note": "the combined roofline sums max(FLOPs / 2.5 PFLOP/s, bytes / "
+                             "8 TB/s) over the pass's kernels; the 32-channel layers at 256^2, "
+                             "the upsample-add layers, the first and last layers are bound by "
+                             "their HBM bytes, so utilisation cannot exceed "
+                             "flops_per_forward / 2.5e15 / combined_roofline_seconds"},
             "plain_50_step_seconds_per_batch": t_plain / n_adv,
             "note": ("the specified network costs 13.4 GFLOP per sample per advance; 50 "
                      "plain steps of one 256^2 field cost ~3.3M site-updates, so the "

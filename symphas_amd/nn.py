@@ -285,6 +285,41 @@ class UNet(torch.nn.Module):
         f += 2 * 9 * self.ch[0] * H * W
         return f
 
+    def layer_costs(self, H, W):
+        """(name, FLOPs, HBM bytes) per sample for every kernel of one
+        inference pass: each convolution reads its input once and writes its
+        output once (BF16 activations; FP32 network input and increment; the
+        upsample-add layers also read the skip), each pooling reads its input
+        and writes a quarter of it, the centring reads and writes the
+        increment, and psi + scale * increment reads two fields and writes
+        one."""
+        L, ch = len(self.ch), self.ch
+        out = [("in1", 2 * 9 * ch[0] * H * W, 4 * H * W + 2 * ch[0] * H * W)]
+        for l in range(L):
+            hw = (H >> l) * (W >> l)
+            if l > 0:
+                phw = (H >> (l - 1)) * (W >> (l - 1))
+                out.append((f"pool{l}", 0, 2 * ch[l - 1] * (phw + phw // 4)))
+            cin = ch[0] if l == 0 else ch[l - 1]
+            out.append((f"enc{l}a", 2 * 9 * cin * ch[l] * hw, 2 * (cin + ch[l]) * hw))
+            out.append((f"enc{l}b", 2 * 9 * ch[l] * ch[l] * hw, 4 * ch[l] * hw))
+        for l in reversed(range(L - 1)):
+            hw_c = (H >> (l + 1)) * (W >> (l + 1))
+            hw = (H >> l) * (W >> l)
+            out.append((f"up{l}", 2 * 9 * ch[l + 1] * ch[l] * hw_c,
+                        2 * ch[l + 1] * hw_c + 2 * 2 * ch[l] * hw))
+            out.append((f"dec{l}", 2 * 9 * ch[l] * ch[l] * hw, 4 * ch[l] * hw))
+        out.append(("out1", 2 * 9 * ch[0] * H * W, 2 * ch[0] * H * W + 4 * H * W))
+        out.append(("center", 0, 8 * H * W))
+        out.append(("update", 0, 12 * H * W))
+        return out
+
+    def roofline_seconds(self, H, W, batch, peak_flops=2.5e15, peak_bytes=8e12):
+        """Lower bound on one inference pass of batch samples: the sum over its
+        kernels of max(FLOPs / MFMA peak, bytes / HBM peak)."""
+        return sum(max(f * batch / peak_flops, b * batch / peak_bytes)
+                   for _, f, b in self.layer_costs(H, W))
+
 
 class GraphedForward:
     """Inference of a network replayed from captured HIP graphs, one per input

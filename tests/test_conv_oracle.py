@@ -67,3 +67,26 @@ def test_bf16_rounding_matches_torch():
     assert np.array_equal(co.bf16_round(a), want)
     bits = co.to_bf16_bits(a)
     assert np.array_equal(co.from_bf16_bits(bits), want)
+
+
+def test_unet_roofline_counts():
+    """The per-kernel cost table behind the surrogate's combined roofline:
+    its FLOPs add up to the network's FLOP count, the 32-channel 256^2 layer
+    moves one BF16 input and one BF16 output, and with unlimited bandwidth the
+    bound is the FLOP count over the MFMA peak."""
+    import math
+
+    from symphas_amd.nn import UNet
+
+    net = UNet(seed=1)
+    costs = {name: (f, b) for name, f, b in net.layer_costs(256, 256)}
+    assert sum(f for f, _ in costs.values()) == net.flops_per_sample(256, 256)
+    assert costs["enc0b"] == (2 * 9 * 32 * 32 * 256 * 256, 2 * (2 * 256 * 256 * 32))
+    # upsample-add at the finest level: 64-channel 128^2 input, 32-channel
+    # 256^2 skip and output
+    assert costs["up0"][1] == 2 * 128 * 128 * 64 + 2 * 2 * 256 * 256 * 32
+    t_flop = net.roofline_seconds(256, 256, 64, peak_bytes=math.inf)
+    assert abs(t_flop / (net.flops_per_sample(256, 256) * 64 / 2.5e15) - 1) < 1e-12
+    t = net.roofline_seconds(256, 256, 64)
+    assert t > t_flop
+    assert t >= sum(b for _, b in costs.values()) * 64 / 8e12
