@@ -203,11 +203,14 @@ __global__ __launch_bounds__(256) void conv_fwd(
 // (weights, [tap][out][in]) are 16-byte global loads that every workgroup
 // shares through L2.  A pixel's 32 channels are padded to 40 in LDS so the
 // 16 pixels of one fragment start 20 banks apart.
+// NCH = 1 is the single-chunk form (32 input channels): one LDS buffer, and
+// the staging registers are free before the taps, which lets a CU hold more
+// workgroups; NCH = 0 loops over C / 32 chunks.
 constexpr int kTileCols = 16;
 constexpr int kHaloCols = kTileCols + 2;
 constexpr int kPixStride = 40;  // bf16 per staged pixel (32 channels + pad)
 
-template <int MW, int NW, int WM, int WN, int EPI>
+template <int MW, int NW, int WM, int WN, int EPI, int NCH>
 __global__ __launch_bounds__(256) void conv_fwd_tile(
     const uint16_t* __restrict__ x, int N, int H, int W, int C,
     const uint16_t* __restrict__ wt, const float* __restrict__ bias, int K,
@@ -225,7 +228,8 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
                                // pixels start 16 bytes x 9 apart (no conflicts)
   constexpr int OUT_FLOATS = TH * kTileCols * SKB;
   constexpr bool STAGED = OUT_FLOATS * 4 <= 40960;
-  constexpr int TILE_BYTES = 2 * TILE * kPixStride * 2;
+  constexpr int NBUF = NCH == 1 ? 1 : 2;
+  constexpr int TILE_BYTES = NBUF * TILE * kPixStride * 2;
   constexpr int LDS_BYTES =
       STAGED && OUT_FLOATS * 4 > TILE_BYTES ? OUT_FLOATS * 4 : TILE_BYTES;
   __shared__ __align__(16) unsigned char lds[LDS_BYTES];
@@ -276,11 +280,11 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
 #pragma unroll
     for (int j = 0; j < NW; ++j) acc[m][j] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
-  const int nchunks = C / 32;
+  const int nchunks = NCH ? NCH : C / 32;
   fetch(0);
   put(0);
   for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = ch & 1;
+    const int buf = NBUF == 1 ? 0 : ch & 1;
     const int c0 = 32 * ch;
     // weight fragments are loaded one tap ahead of their MFMAs (loading all
     // nine taps' at once costs the waves a SIMD can hold)
@@ -1114,11 +1118,14 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
   do {                                                                         \
     constexpr int TH_ = (NW) * (WN), KB_ = 16 * (MW) * (WM);                   \
     dim3 g((unsigned)(n * (h / TH_) * tiles_w), (unsigned)(cout / KB_));      \
-    if (epilogue == SPS_NN_EPI_PLAIN)                                          \
-      conv_fwd_tile<MW, NW, WM, WN, 0><<<g, 256, 0, s>>>(                       \
+    if (epilogue == SPS_NN_EPI_PLAIN && cin == 32)                             \
+      conv_fwd_tile<MW, NW, WM, WN, 0, 1><<<g, 256, 0, s>>>(                    \
+          xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
+    else if (epilogue == SPS_NN_EPI_PLAIN)                                     \
+      conv_fwd_tile<MW, NW, WM, WN, 0, 0><<<g, 256, 0, s>>>(                    \
           xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
     else                                                                       \
-      conv_fwd_tile<MW, NW, WM, WN, 1><<<g, 256, 0, s>>>(                       \
+      conv_fwd_tile<MW, NW, WM, WN, 1, 0><<<g, 256, 0, s>>>(                    \
           xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
     return launch_ok("sps_nn_conv3x3");                                        \
   } while (0)

@@ -30,8 +30,9 @@ def rnd(shape, seed, scale=1.0):
     return np.random.default_rng(seed).standard_normal(shape) * scale
 
 
-@pytest.mark.parametrize("N,H,W,C,K", [(2, 16, 16, 32, 64), (1, 8, 32, 64, 32),
-                                       (4, 32, 32, 128, 256), (1, 16, 64, 256, 128)])
+@pytest.mark.parametrize("N,H,W,C,K", [(2, 16, 16, 32, 64), (2, 16, 32, 32, 32),
+                                       (1, 8, 32, 64, 32), (4, 32, 32, 128, 256),
+                                       (1, 16, 64, 256, 128)])
 def test_conv_forward_plain(gpu_lib, N, H, W, C, K):
     print(f"[case] conv fwd {N}x{H}x{W} {C}->{K}", flush=True)
     x = co.bf16_round(rnd((N, H, W, C), 1))
