@@ -362,23 +362,37 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
         store8(y + o, v);
       }
     } else {
-      // output (2*TH) x 32 pixels, each from tile pixel (r/2, c/2)
-      for (int e = threadIdx.x; e < 4 * TH * kTileCols * G; e += 256) {
-        const int opx = e / G, g = e - opx * G;
-        const int orow = opx / (2 * kTileCols), ocol = opx % (2 * kTileCols);
-        const int px = (orow >> 1) * kTileCols + (ocol >> 1);
-        const int64_t o = ((int64_t)(n * 2 * H + 2 * h0 + orow) * (2 * W) + 2 * w0 + ocol) * K +
-                          kb0 + 8 * g;
-        float u[8];
-        load8(skip + o, u);
+      // output (2*TH) x 32 pixels, each from tile pixel (r/2, c/2); the
+      // skip pieces of U iterations are loaded before any is used, so their
+      // memory latencies overlap instead of adding up
+      constexpr int ITERS = 4 * TH * kTileCols * G / 256;
+      static_assert(4 * TH * kTileCols * G % 256 == 0, "whole iterations");
+      constexpr int U = ITERS % 4 == 0 ? 4 : 1;
+      for (int it = 0; it < ITERS; it += U) {
+        int64_t o[U];
+        int sp[U];
+        float u[U][8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) u[i] += stage[px * SKB + 8 * g + i];
-        if (pre) store8(pre + o, u);
-        if (act) {
-#pragma unroll
-          for (int i = 0; i < 8; ++i) u[i] = silu(u[i]);
+        for (int q = 0; q < U; ++q) {
+          const int e = threadIdx.x + 256 * (it + q);
+          const int opx = e / G, g = e - opx * G;
+          const int orow = opx / (2 * kTileCols), ocol = opx % (2 * kTileCols);
+          sp[q] = ((orow >> 1) * kTileCols + (ocol >> 1)) * SKB + 8 * g;
+          o[q] = ((int64_t)(n * 2 * H + 2 * h0 + orow) * (2 * W) + 2 * w0 + ocol) * K +
+                 kb0 + 8 * g;
+          load8(skip + o[q], u[q]);
         }
-        store8(y + o, u);
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) u[q][i] += stage[sp[q] + i];
+          if (pre) store8(pre + o[q], u[q]);
+          if (act) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) u[q][i] = silu(u[q][i]);
+          }
+          store8(y + o[q], u[q]);
+        }
       }
     }
     return;
