@@ -19,6 +19,7 @@
 // and gathers the fragments from there.
 #include <algorithm>
 #include <cstdint>
+#include <initializer_list>
 
 #include "common.hpp"
 #include "../../include/sps_nn.h"
@@ -1000,21 +1001,114 @@ __global__ __launch_bounds__(256) void unpool2_k(const uint16_t* __restrict__ x,
   }
 }
 
+// The same three passes on 8 consecutive channels per thread: one 16-byte
+// access per operand and 32-bit index arithmetic, used when the channel count
+// is a multiple of 8, the piece count fits 31 bits and the pointers are
+// 16-byte aligned.  Each value goes through the same operations in the same
+// order as in the one-value kernels above, so results are identical.
+__global__ __launch_bounds__(256) void silu_bwd_v8(const uint16_t* __restrict__ dy,
+                                                   const uint16_t* __restrict__ pre,
+                                                   uint16_t* __restrict__ dx,
+                                                   unsigned pieces) {
+  for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < pieces; i += gridDim.x * 256) {
+    float z[8], d[8], v[8];
+    load8(pre + 8 * (int64_t)i, z);
+    load8(dy + 8 * (int64_t)i, d);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z[k]));
+      v[k] = d[k] * s * (1.0f + z[k] * (1.0f - s));
+    }
+    store8(dx + 8 * (int64_t)i, v);
+  }
+}
+
+__global__ __launch_bounds__(256) void pool2_v8(const uint16_t* __restrict__ x,
+                                                int H, int W, int C, int mode,
+                                                unsigned pieces,
+                                                uint16_t* __restrict__ y) {
+  const unsigned G = (unsigned)C / 8, Ho = (unsigned)H / 2, Wo = (unsigned)W / 2;
+  const float scale = mode == 0 ? 0.25f : 1.0f;
+  const int64_t rowc = (int64_t)W * C;
+  for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < pieces; i += gridDim.x * 256) {
+    const unsigned g = i % G;
+    unsigned q = i / G;
+    const unsigned wo = q % Wo;
+    q /= Wo;
+    const unsigned ho = q % Ho;
+    const unsigned n = q / Ho;
+    const uint16_t* b = x + ((int64_t)(n * H + 2 * ho) * W + 2 * wo) * C + 8 * g;
+    float a0[8], a1[8], a2[8], a3[8], v[8];
+    load8(b, a0);
+    load8(b + C, a1);
+    load8(b + rowc, a2);
+    load8(b + rowc + C, a3);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = ((a0[k] + a1[k]) + (a2[k] + a3[k])) * scale;
+    store8(y + 8 * (int64_t)i, v);
+  }
+}
+
+__global__ __launch_bounds__(256) void unpool2_v8(const uint16_t* __restrict__ x,
+                                                  int H, int W, int C, float scale,
+                                                  unsigned pieces,
+                                                  uint16_t* __restrict__ y) {
+  const unsigned G = (unsigned)C / 8, Ho = 2u * H, Wo = 2u * W;
+  for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < pieces; i += gridDim.x * 256) {
+    const unsigned g = i % G;
+    unsigned q = i / G;
+    const unsigned wo = q % Wo;
+    q /= Wo;
+    const unsigned ho = q % Ho;
+    const unsigned n = q / Ho;
+    float v[8];
+    load8(x + ((int64_t)(n * H + ho / 2) * W + wo / 2) * C + 8 * g, v);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] *= scale;
+    store8(y + 8 * (int64_t)i, v);
+  }
+}
+
 // One workgroup per sample: FP64 sum in a fixed order (strided per thread,
 // then xor-shuffle and LDS across the 4 waves), then y = x - mean.
 __global__ __launch_bounds__(256) void center_k(const float* x, float* y,
                                                 int64_t len) {
   const float* xs = x + (int64_t)blockIdx.x * len;
   float* ys = y + (int64_t)blockIdx.x * len;
+  // U loads in flight per thread before their adds, which keep the order
+  // x[t], x[t + 256], x[t + 512], ... of thread t
+  constexpr int U = 16;
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < len; i += 256) s += (double)xs[i];
+  int64_t i = threadIdx.x;
+  for (; i + (U - 1) * 256 < len; i += U * 256) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = xs[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += (double)v[u];
+  }
+  for (; i < len; i += 256) s += (double)xs[i];
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
   __shared__ double sh[4];
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
   const float mean = (float)(((sh[0] + sh[1]) + (sh[2] + sh[3])) / (double)len);
-  for (int64_t i = threadIdx.x; i < len; i += 256) ys[i] = xs[i] - mean;
+  // each thread reads its U values before writing them (x and y may alias)
+  for (i = threadIdx.x; i + (U - 1) * 256 < len; i += U * 256) {
+    float v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = xs[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < U; ++u) ys[i + u * 256] = v[u] - mean;
+  }
+  for (; i < len; i += 256) ys[i] = xs[i] - mean;
+}
+
+bool aligned16(std::initializer_list<const void*> ps) {
+  for (const void* p : ps)
+    if (reinterpret_cast<uintptr_t>(p) % 16) return false;
+  return true;
 }
 
 int grid_for(int64_t count) {
@@ -1317,6 +1411,12 @@ int sps_nn_silu_bwd(const void* dy, const void* pre, void* dx, size_t count,
     set_error("sps_nn_silu_bwd: NULL pointer");
     return SPS_E_INVALID;
   }
+  if (count % 8 == 0 && count / 8 < ((size_t)1 << 31) &&
+      aligned16({dy, pre, dx})) {
+    silu_bwd_v8<<<grid_for((int64_t)(count / 8)), 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)dy, (const uint16_t*)pre, (uint16_t*)dx, (unsigned)(count / 8));
+    return launch_ok("sps_nn_silu_bwd");
+  }
   silu_bwd_k<<<grid_for((int64_t)count), 256, 0, (hipStream_t)stream>>>(
       (const uint16_t*)dy, (const uint16_t*)pre, (uint16_t*)dx, (int64_t)count);
   return launch_ok("sps_nn_silu_bwd");
@@ -1327,6 +1427,12 @@ int sps_nn_pool2(const void* x, int n, int h, int w, int c, int mode, void* y,
   if (!x || !y || h % 2 || w % 2 || c < 1 || (mode != 0 && mode != 1)) {
     set_error("sps_nn_pool2: even h and w, mode 0 or 1");
     return SPS_E_INVALID;
+  }
+  const int64_t pieces = (int64_t)n * (h / 2) * (w / 2) * (c / 8);
+  if (c % 8 == 0 && pieces < ((int64_t)1 << 31) && aligned16({x, y})) {
+    pool2_v8<<<grid_for(pieces), 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)x, h, w, c, mode, (unsigned)pieces, (uint16_t*)y);
+    return launch_ok("sps_nn_pool2");
   }
   pool2_k<<<grid_for((int64_t)n * h * w * c / 4), 256, 0, (hipStream_t)stream>>>(
       (const uint16_t*)x, n, h, w, c, mode, (uint16_t*)y);
@@ -1348,6 +1454,12 @@ int sps_nn_unpool2(const void* x, int n, int h, int w, int c, float scale,
   if (!x || !y || c < 1) {
     set_error("sps_nn_unpool2: invalid arguments");
     return SPS_E_INVALID;
+  }
+  const int64_t pieces = (int64_t)n * (2 * h) * (2 * w) * (c / 8);
+  if (c % 8 == 0 && pieces < ((int64_t)1 << 31) && aligned16({x, y})) {
+    unpool2_v8<<<grid_for(pieces), 256, 0, (hipStream_t)stream>>>(
+        (const uint16_t*)x, h, w, c, scale, (unsigned)pieces, (uint16_t*)y);
+    return launch_ok("sps_nn_unpool2");
   }
   unpool2_k<<<grid_for((int64_t)n * h * w * c * 4), 256, 0, (hipStream_t)stream>>>(
       (const uint16_t*)x, n, h, w, c, scale, (uint16_t*)y);

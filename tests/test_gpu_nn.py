@@ -132,10 +132,12 @@ def test_thin_layers(gpu_lib, N, H, W, K):
     assert np.allclose(dwo.cpu().numpy(), refdw, rtol=1e-4, atol=1e-4)
 
 
-def test_elementwise(gpu_lib):
-    print("[case] elementwise", flush=True)
-    a = co.bf16_round(rnd((2, 8, 16, 32), 19, 3.0))
-    d = co.bf16_round(rnd((2, 8, 16, 32), 20))
+@pytest.mark.parametrize("C", [32, 12])
+def test_elementwise(gpu_lib, C):
+    # 32 channels take the 8-channel kernels, 12 the one-value kernels
+    print(f"[case] elementwise C={C}", flush=True)
+    a = co.bf16_round(rnd((2, 8, 16, C), 19, 3.0))
+    d = co.bf16_round(rnd((2, 8, 16, C), 20))
     dx = nn.silu_bwd_raw(bf(d), bf(a))
     assert within_ulp(np64(dx), d * co.silu_grad(a), ulps=2).all()
     p = nn.pool2_raw(bf(a), 0)
@@ -232,6 +234,42 @@ def test_center_removes_each_samples_mean(gpu_lib):
     ref = ref - ref.mean(axis=(1, 2), keepdims=True)
     assert np.abs(out - ref).max() < 1e-6
     assert np.abs(out.sum(axis=(1, 2))).max() < 1e-4
+
+
+def _center_fixed_order(x):
+    """The centring's stated order: thread t of 256 sums x[t], x[t + 256], ...
+    in FP64 one after another, each wave of 64 combines its threads with an
+    xor butterfly, the four wave sums are added as (w0 + w1) + (w2 + w3), and
+    the FP32 mean is subtracted in FP32."""
+    out = np.empty_like(x)
+    for k in range(x.shape[0]):
+        xs = x[k].reshape(-1)
+        part = np.zeros(256)
+        for t in range(256):
+            acc = 0.0
+            for v in xs[t::256].tolist():
+                acc += v
+            part[t] = acc
+        lanes = np.arange(64)
+        sums = []
+        for w in range(4):
+            v = part[64 * w: 64 * w + 64].copy()
+            for o in (32, 16, 8, 4, 2, 1):
+                v = v + v[lanes ^ o]
+            sums.append(v[0])
+        mean = np.float32(((sums[0] + sums[1]) + (sums[2] + sums[3])) / xs.size)
+        out[k] = x[k] - mean
+    return out
+
+
+@pytest.mark.parametrize("shape", [(3, 8, 16), (2, 67, 73), (2, 256, 256)])
+def test_center_fixed_order_bitwise(gpu_lib, shape):
+    # 67 x 73 has 4891 values per sample: one batch of 16 loads per thread
+    # and a ragged tail; 256 x 256 is the surrogate's field
+    print(f"[case] center {shape}", flush=True)
+    x = (rnd(shape, 31) + 0.3).astype(np.float32)
+    out = nn.center_raw(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.array_equal(out, _center_fixed_order(x))
 
 
 def test_graphed_forward_replays_match_eager(gpu_lib):
