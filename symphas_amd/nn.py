@@ -128,7 +128,8 @@ class _Conv(torch.autograd.Function):
         wb = flip_weights(w.detach()).to(torch.bfloat16)
         dx, _ = conv3x3_raw(dlow, wb, None, ACT_NONE)
         dw = conv3x3_wgrad_raw(x.contiguous(), dlow, K)
-        db = dlow.float().sum(dim=(0, 1, 2))
+        # FP32 accumulation straight from the BF16 gradient (no FP32 copy)
+        db = torch.sum(dlow, dim=(0, 1, 2), dtype=torch.float32)
         return dx, dw, db, None, None, dskip, None
 
 
@@ -157,7 +158,7 @@ class _In1(torch.autograd.Function):
         ws = wgrad_workspace(WGRAD_IN1, N, H, W, 1, K, x.device)
         _check(_lib().sps_nn_conv3x3_in1_wgrad(_ptr(x.contiguous()), _ptr(dpre), N, H, W, K,
                                                _ptr(dw), _ptr(ws), ws.numel(), _stream()))
-        db = dpre.float().sum(dim=(0, 1, 2))
+        db = torch.sum(dpre, dim=(0, 1, 2), dtype=torch.float32)
         return None, dw, db, None
 
 
