@@ -204,9 +204,10 @@ __global__ __launch_bounds__(256) void conv_fwd(
 // (weights, [tap][out][in]) are 16-byte global loads that every workgroup
 // shares through L2.  A pixel's 32 channels are padded to 40 in LDS so the
 // 16 pixels of one fragment start 20 banks apart.
-// NCH = 1 is the single-chunk form (32 input channels): one LDS buffer, and
-// the staging registers are free before the taps, which lets a CU hold more
-// workgroups; NCH = 0 loops over C / 32 chunks.
+// NCH > 0 (C = 32 * NCH input channels) stages every chunk at once, one LDS
+// buffer per chunk, before the taps: the staging registers are free by then,
+// which lets a CU hold more workgroups.  NCH = 0 loops over C / 32 chunks
+// with two buffers, fetching the next chunk during the current one's taps.
 constexpr int kTileCols = 16;
 constexpr int kHaloCols = kTileCols + 2;
 constexpr int kPixStride = 40;  // bf16 per staged pixel (32 channels + pad)
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
                                // pixels start 16 bytes x 9 apart (no conflicts)
   constexpr int OUT_FLOATS = TH * kTileCols * SKB;
   constexpr bool STAGED = OUT_FLOATS * 4 <= 40960;
-  constexpr int NBUF = NCH == 1 ? 1 : 2;
+  constexpr int NBUF = NCH == 0 ? 2 : NCH;
   constexpr int TILE_BYTES = NBUF * TILE * kPixStride * 2;
   constexpr int LDS_BYTES =
       STAGED && OUT_FLOATS * 4 > TILE_BYTES ? OUT_FLOATS * 4 : TILE_BYTES;
@@ -261,18 +262,19 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
     src[i] = e < PIECES ? ((int64_t)(n * H + row) * W + col) * C + 8 * q : -1;
     dst[i] = pix * kPixStride + 8 * q;
   }
-  u32x4 stage[PER_THREAD];
-  auto fetch = [&](int c0) {
+  constexpr int NST = NCH > 1 ? NCH : 1;  // staging register sets
+  u32x4 stage[NST][PER_THREAD];
+  auto fetch = [&](int c0, int k) {
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i)
       if (src[i] >= 0)
-        stage[i] = *reinterpret_cast<const u32x4*>(x + src[i] + c0);
+        stage[k][i] = *reinterpret_cast<const u32x4*>(x + src[i] + c0);
   };
-  auto put = [&](int buf) {
+  auto put = [&](int buf, int k) {
 #pragma unroll
     for (int i = 0; i < PER_THREAD; ++i)
       if (src[i] >= 0)
-        *reinterpret_cast<u32x4*>(&tile[buf][dst[i]]) = stage[i];
+        *reinterpret_cast<u32x4*>(&tile[buf][dst[i]]) = stage[k][i];
   };
 
   f32x4v acc[MW][NW];
@@ -282,10 +284,13 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
     for (int j = 0; j < NW; ++j) acc[m][j] = f32x4v{0.f, 0.f, 0.f, 0.f};
 
   const int nchunks = NCH ? NCH : C / 32;
-  fetch(0);
-  put(0);
+  // NCH > 0: every chunk's loads in flight before the first LDS store
+#pragma unroll
+  for (int c = 0; c < NST; ++c) fetch(32 * c, c);
+#pragma unroll
+  for (int c = 0; c < NST; ++c) put(c, c);
   for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = NBUF == 1 ? 0 : ch & 1;
+    const int buf = NCH ? ch : ch & 1;
     const int c0 = 32 * ch;
     // weight fragments are loaded one tap ahead of their MFMAs (loading all
     // nine taps' at once costs the waves a SIMD can hold)
@@ -297,8 +302,10 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
                            c0 + cq);
     };
     load_a(0, apre[0]);
-    __syncthreads();  // tile[buf] written; tile[buf ^ 1] free
-    if (ch + 1 < nchunks) fetch(32 * (ch + 1));
+    // NCH = 0: tile[buf] written and tile[buf ^ 1] free; NCH > 0: every
+    // chunk's tile written
+    if (NCH == 0 || ch == 0) __syncthreads();
+    if (NCH == 0 && ch + 1 < nchunks) fetch(32 * (ch + 1), 0);
     const uint16_t* tb = tile[buf];
 #pragma unroll
     for (int tap = 0; tap < 9; ++tap) {
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(256) void conv_fwd_tile(
           acc[m][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[j],
                                                               acc[m][j], 0, 0, 0);
     }
-    if (ch + 1 < nchunks) put(buf ^ 1);
+    if (NCH == 0 && ch + 1 < nchunks) put(buf ^ 1, 0);
   }
 
   if constexpr (STAGED) {
@@ -1225,12 +1232,20 @@ int sps_nn_conv3x3(const void* x, int n, int h, int w, int cin,
 #define SPS_CONV_TILE(MW, NW, WM, WN)                                           \
   do {                                                                         \
     constexpr int TH_ = (NW) * (WN), KB_ = 16 * (MW) * (WM);                   \
+    /* two chunks staged at once only where registers allow 3 waves/SIMD */   \
+    constexpr int C2_ = (MW) == 2 ? 2 : 0;                                     \
     dim3 g((unsigned)(n * (h / TH_) * tiles_w), (unsigned)(cout / KB_));      \
     if (epilogue == SPS_NN_EPI_PLAIN && cin == 32)                             \
       conv_fwd_tile<MW, NW, WM, WN, 0, 1><<<g, 256, 0, s>>>(                    \
           xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
+    else if (epilogue == SPS_NN_EPI_PLAIN && cin == 64 && C2_)                 \
+      conv_fwd_tile<MW, NW, WM, WN, 0, C2_><<<g, 256, 0, s>>>(                  \
+          xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
     else if (epilogue == SPS_NN_EPI_PLAIN)                                     \
       conv_fwd_tile<MW, NW, WM, WN, 0, 0><<<g, 256, 0, s>>>(                    \
+          xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
+    else if (cin == 64 && C2_)                                                 \
+      conv_fwd_tile<MW, NW, WM, WN, 1, C2_><<<g, 256, 0, s>>>(                  \
           xx, n, h, w, cin, ww, bias, cout, act, sk, yy, pp);                  \
     else                                                                       \
       conv_fwd_tile<MW, NW, WM, WN, 1, 0><<<g, 256, 0, s>>>(                    \

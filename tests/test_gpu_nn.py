@@ -31,8 +31,8 @@ def rnd(shape, seed, scale=1.0):
 
 
 @pytest.mark.parametrize("N,H,W,C,K", [(2, 16, 16, 32, 64), (2, 16, 32, 32, 32),
-                                       (1, 8, 32, 64, 32), (4, 32, 32, 128, 256),
-                                       (1, 16, 64, 256, 128)])
+                                       (1, 8, 32, 64, 32), (2, 16, 32, 64, 32),
+                                       (4, 32, 32, 128, 256), (1, 16, 64, 256, 128)])
 def test_conv_forward_plain(gpu_lib, N, H, W, C, K):
     print(f"[case] conv fwd {N}x{H}x{W} {C}->{K}", flush=True)
     x = co.bf16_round(rnd((N, H, W, C), 1))
@@ -49,9 +49,10 @@ def test_conv_forward_plain(gpu_lib, N, H, W, C, K):
     assert within_ulp(np64(y0), co.conv3x3(x, w)).all()
 
 
-def test_conv_forward_upsample_add(gpu_lib):
-    print("[case] conv fwd upsample-add", flush=True)
-    N, H, W, C, K = 2, 8, 16, 64, 32
+@pytest.mark.parametrize("H,C", [(8, 64), (16, 64), (16, 128)])
+def test_conv_forward_upsample_add(gpu_lib, H, C):
+    print(f"[case] conv fwd upsample-add H={H} C={C}", flush=True)
+    N, W, K = 2, 16, 32
     x = co.bf16_round(rnd((N, H, W, C), 4))
     w = co.bf16_round(rnd((9, K, C), 5, 1.0 / np.sqrt(9 * C)))
     b = rnd(K, 6, 0.1).astype(np.float32)
